@@ -1,0 +1,177 @@
+/*
+ * rimp2_hip.h -- C ABI of the MI355X RI-MP2 energy library (librimp2_hip.so).
+ *
+ * This is the drop-in boundary that replaces the reference's Fortran FFI module
+ * gpu_interfaces.F90 (cuBLAS create/destroy/Dgemm/SetStream + get_dev_mem,
+ * gpu_interfaces.F90:8-55) and its native helper cuda_api.cu:4-9.  The Fortran
+ * binding is rimp2_gpu_amd/fortran/rimp2_hip_interfaces.F90; the Python host
+ * mirror is rimp2_gpu_amd/__init__.py (ctypes).
+ *
+ * Conventions (all entry points):
+ *   - every dimension is int64_t, every array is column-major (Fortran order);
+ *     (P|ia) and B are [naux, nvirt, nocc] with the auxiliary index fastest,
+ *     as Calpha(NBA,nvirt,nocc) in RIMP2.F90:48;
+ *   - every function returns int status, 0 = RIMP2_OK;  the text of the last
+ *     failure is available from rimp2_last_error;  no C++ exception crosses the
+ *     ABI;
+ *   - host-pointer entry points block until results are in host memory;
+ *     *_device entry points take device pointers and are asynchronous on the
+ *     context's stream (rimp2_sync waits);
+ *   - the library never frees caller memory;  one host thread per context.
+ */
+#ifndef RIMP2_HIP_H
+#define RIMP2_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define RIMP2_OK 0
+#define RIMP2_ERR_ARG 1        /* bad dimension, null pointer, unknown mode */
+#define RIMP2_ERR_HIP 2        /* HIP runtime failure (message has the HIP error) */
+#define RIMP2_ERR_NOT_SPD 3    /* metric not positive definite (Cholesky pivot <= 0) */
+#define RIMP2_ERR_STATE 4      /* call order: e.g. pair energies before a transform */
+#define RIMP2_ERR_NOMEM 5      /* device allocation failed / request exceeds free memory */
+#define RIMP2_ERR_NUMERIC 6    /* NaN/Inf input, or an orbital-energy denominator >= 0 */
+
+/* ---- precision modes of the pair contraction ---- */
+#define RIMP2_MODE_FP64 0      /* FP64 MFMA (v_mfma_f64_16x16x4), the plain path */
+#define RIMP2_MODE_FP32 1      /* FP32 MFMA (v_mfma_f32_32x32x2), FP32 accumulation */
+#define RIMP2_MODE_BF16X3 2    /* split BF16 hi+lo, 3 products, FP32 accumulation */
+#define RIMP2_MODE_I8S4 3      /* exact-slice: 4 int8 slices, 13 products, int32 acc */
+#define RIMP2_MODE_I8S5 4      /* exact-slice: 5 int8 slices, 15 products, int32 acc */
+#define RIMP2_NUM_MODES 5
+
+/* ---- kernel ids for the built-in event timers ---- */
+#define RIMP2_K_PAIR 0
+#define RIMP2_K_TRANSFORM 1
+#define RIMP2_K_PACK 2
+#define RIMP2_K_CHOLESKY 3
+#define RIMP2_K_TRTRI 4
+#define RIMP2_K_REDUCE 5
+#define RIMP2_K_BATCH 6
+#define RIMP2_K_MLP 7
+#define RIMP2_K_FEATURES 8
+#define RIMP2_NUM_KERNELS 9
+
+/* ---- context, device memory query, errors ---- */
+
+/* Replaces cublasCreate_v2 (gpu_interfaces.F90:11-15, called RIMP2.GPU.F90:81):
+ * creates a context on `device` owning a HIP stream, device buffers, graph
+ * cache and tile-shape table. */
+int rimp2_ctx_create(int device, void** ctx);
+/* Replaces cublasDestroy_v2 (gpu_interfaces.F90:18-22, called RIMP2.GPU.F90:347). */
+int rimp2_ctx_destroy(void* ctx);
+/* Replaces get_dev_mem (cuda_api.cu:4-9, gpu_interfaces.F90:47-53); same
+ * argument order (total, free), built on hipMemGetInfo for the current device. */
+int rimp2_get_dev_mem(size_t* total, size_t* free_bytes);
+/* Copies the last error text of ctx (or the library-global one when ctx is
+ * NULL) into buf (NUL-terminated, at most n bytes). */
+int rimp2_last_error(void* ctx, char* buf, size_t n);
+
+/* ---- the RI-MP2 energy path (host pointers, blocking) ---- */
+
+/* FP64 Cholesky V = L L^T and inverse L^{-1} of the SPD metric V[naux,naux] on
+ * the GPU.  The reference receives its 3-index tensor already fitted
+ * (Calpha, RIMP2.F90:48-49); this is the fitting step in front of it. */
+int rimp2_metric(void* ctx, int64_t naux, const double* V);
+/* B = L^{-1} (P|ia) as an FP64 MFMA GEMM, then packed for `mode`.  Requires
+ * rimp2_metric with the same naux.  Pia is [naux, nvirt, nocc]. */
+int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, int mode);
+/* Fused pair kernel on the transformed B of the last rimp2_transform:
+ *   eps_ij = sum_ab V_ab (2 V_ab - V_ba) / (e_i + e_j - e_a - e_b),
+ *   V_ab = sum_Q B^Q_ia B^Q_jb,   E = sum_ij eps_ij.
+ * This is RIMP2_calc_toccA (RIMP2.F90:381-417, t = g/Delta) combined with
+ * RIMP2_calc_gocc (RIMP2.F90:562-587, g) at U = identity, contracted to
+ * energies.  eps_pair is [nocc,nocc] (may be NULL); e_tot receives E. */
+int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v, int mode,
+                        double* eps_pair, double* e_tot);
+/* Whole path from host inputs to host energies (metric, transform, pairs). */
+int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, const double* V,
+                 const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot);
+/* Same path with every pointer in device memory; asynchronous on the context
+ * stream.  d_eps_pair [nocc,nocc] may be NULL; d_e_tot is one double. */
+int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Pia, const double* d_V,
+                        const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair, double* d_e_tot);
+
+/* ---- inspection of intermediate results (host pointers, blocking) ---- */
+int rimp2_get_cholesky(void* ctx, double* L);      /* L [naux,naux], lower, zeros above */
+int rimp2_get_metric_inverse(void* ctx, double* Linv); /* L^{-1} [naux,naux] */
+int rimp2_get_B(void* ctx, double* B);             /* FP64 B [naux,nvirt,nocc] */
+
+/* ---- batched DEC fragments ---- */
+typedef struct rimp2_frag_desc {
+  int64_t nocc, nvirt, naux;
+  const double* Pia;   /* [naux, nvirt, nocc] */
+  const double* V;     /* [naux, naux] */
+  const double* eps_o; /* [nocc] */
+  const double* eps_v; /* [nvirt] */
+} rimp2_frag_desc;
+/* Many fragments of uneven shape through one persistent, graph-captured pair
+ * launch (the analogue of the reference's serial fragment loop
+ * `do ifile=1,2`, RIMP2.F90:36).  e_frag[f] receives E of fragment f;
+ * eps_pair_packed (may be NULL) receives each fragment's [nocc,nocc] pair
+ * matrix, concatenated in fragment order. */
+int rimp2_batch(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode, double* e_frag,
+                double* eps_pair_packed);
+/* Device-resident variant: metric, transform and pack for every fragment are
+ * done beforehand by rimp2_batch_prepare (host inputs); rimp2_batch_run then
+ * replays the captured graph (pair kernel + reductions) and leaves energies in
+ * device memory (d_e_frag [nfrag]). */
+int rimp2_batch_prepare(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode);
+int rimp2_batch_run(void* ctx, double* d_e_frag);
+int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed);
+/* Tile size chosen by the autotuner for the prepared batch (32, 64 or 128). */
+int rimp2_batch_tile(void* ctx, int64_t* tile);
+
+/* ---- pair-screening surrogate (BF16 MLP 16 -> 64 -> 64 -> 1, GELU) ---- */
+#define RIMP2_NFEAT 16
+/* Per-pair features of the last transform: [npairs_unique, 16] float32,
+ * pairs ordered j-major (i <= j, index j*(j+1)/2 + i). */
+int rimp2_pair_features(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v,
+                        float* feats);
+int rimp2_mlp_init(void* ctx, uint64_t seed);
+/* One Adam step on a minibatch (host arrays); loss receives the mean squared
+ * error of the normalised target before the step. */
+int rimp2_mlp_train_step(void* ctx, int64_t n, const float* feats, const float* target, float lr, float* loss);
+int rimp2_mlp_predict(void* ctx, int64_t n, const float* feats, float* pred);
+int rimp2_mlp_get_params(void* ctx, float* params, int64_t n);
+int rimp2_mlp_set_params(void* ctx, const float* params, int64_t n);
+int64_t rimp2_mlp_num_params(void);
+/* Single forward+backward of the MLP on one minibatch: returns the loss and the
+ * gradient of every parameter (float32, same order as the parameter vector). */
+int rimp2_mlp_grad(void* ctx, int64_t n, const float* feats, const float* target, float* loss, float* grad);
+/* Screened energy: pairs whose predicted |eps_ij| < thresh are skipped, the
+ * rest run through the exact pair kernel.  nskipped receives the count. */
+int rimp2_screened_energy(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v, int mode,
+                          double thresh, double* eps_pair, double* e_tot, int64_t* nskipped);
+
+/* ---- device memory and stream helpers (for device-resident callers) ---- */
+int rimp2_device_alloc(void* ctx, size_t bytes, void** dptr);
+int rimp2_device_free(void* ctx, void* dptr);
+int rimp2_copy_h2d(void* ctx, void* dst, const void* src, size_t bytes);
+int rimp2_copy_d2h(void* ctx, void* dst, const void* src, size_t bytes);
+int rimp2_sync(void* ctx);
+
+/* ---- event timers on the context stream ---- */
+int rimp2_timing_enable(void* ctx, int on);
+/* Total milliseconds and launch count of kernel id `kid` since the last reset. */
+int rimp2_timing_get(void* ctx, int kid, double* total_ms, int64_t* count);
+int rimp2_timing_reset(void* ctx);
+
+/* ---- seeded synthetic inputs (host) ---- */
+/* (P|ia) ~ N(0,1)*0.1/sqrt(1+a/16), V = A A^T/naux + I with A ~ N(0,1),
+ * eps_o ~ U[-2,-0.3], eps_v ~ U[0.1,3.0]; any output pointer may be NULL. */
+int rimp2_gen_fragment(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double* Pia, double* V,
+                       double* eps_o, double* eps_v);
+/* nocc ~ U{4..32}, nvirt ~ U{32..256}, naux = 4 nvirt per fragment. */
+int rimp2_gen_batch_shapes(uint64_t seed, int64_t nfrag, int64_t* nocc, int64_t* nvirt, int64_t* naux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RIMP2_HIP_H */

@@ -1,0 +1,216 @@
+"""CPU FP64 oracle for the RI-MP2 hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker (or the timed CPU baseline); the
+product (rimp2_gpu_amd) never imports it.
+
+Functions follow the reference (januseriksen/rimp2_gpu) line by line as cited
+in rimp2_oracle.c; the MLP oracle follows mlp_oracle.c.
+
+Parity pins (tests/test_oracle.py):
+  toccA/tvirtA/toccB/tvirtB/transalpha/gocc/gvirt/steps5to8
+      -> closed-form values of the reference fixtures (generate.F90:77-128)
+         at the reference's 1e-10 tolerance, and an independent numpy einsum
+         on random non-constant inputs (index order / transposes).
+  pair_energies -> numpy brute force + math.fsum, nocc=nvirt=1 closed form,
+         s^4 scaling, orbital-permutation invariance, eps_ij <= 0.
+  cholesky / trsm -> scipy.linalg (LAPACK dpotrf / dtrtrs), L L^T = V,
+         B^T B = X^T V^-1 X, V = I gives B = X.
+  gen4dimfo -> numpy einsum (no reference fixture is reachable: parity of the
+         first-order K=nocc quirk is not exercised by this path).
+  mlp_* -> finite differences and a closed-form GELU (erf form).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "librimp2_oracle.so")
+_lib = None
+
+_i64 = ctypes.c_int64
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        ci = ctypes.c_int
+        sig = {
+            "oracle_toccA": [ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp],
+            "oracle_tvirtA": [ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp],
+            "oracle_toccB": [ci, ci, _dp, _dp, _dp],
+            "oracle_tvirtB": [ci, ci, _dp, _dp, _dp],
+            "oracle_transalpha1": [ci, ci, ci, _dp, _dp, _dp],
+            "oracle_transalpha2": [ci, ci, ci, ci, _dp, _dp, _dp],
+            "oracle_gocc": [ci, ci, ci, _dp, _dp],
+            "oracle_gvirt": [ci, ci, ci, _dp, _dp],
+            "oracle_gen4dimfo": [ci, _dp, ci, ci, _dp, ci, ci, _dp],
+            "oracle_steps5to8": [ci, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
+            "oracle_pair_energies": [_i64, _i64, _i64, _dp, _dp, _dp, _dp, _dp],
+            "oracle_pair_energies_subset": [_i64, _i64, _dp, _dp, _dp, _i64, ctypes.POINTER(_i64),
+                                            ctypes.POINTER(_i64), _dp],
+            "oracle_pair_V": [_i64, _i64, _dp, _i64, _i64, _dp],
+            "oracle_cholesky": [_i64, _dp, _dp],
+            "oracle_trsm_lower": [_i64, _i64, _dp, _dp, _dp],
+            "oracle_metric_transform": [_i64, _i64, _dp, _dp, _dp],
+            "oracle_num_threads": [],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int if name in ("oracle_cholesky", "oracle_metric_transform",
+                                                 "oracle_num_threads") else None
+        for name, (res, args) in MLP_SIG.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = res
+        _lib = L
+    return _lib
+
+
+MLP_SIG = {
+    "oracle_mlp_forward": (None, [_i64, _dp, _dp, _dp]),
+    "oracle_mlp_loss_grad": (ctypes.c_double, [_i64, _dp, _dp, _dp, _dp]),
+    "oracle_gelu": (ctypes.c_double, [ctypes.c_double]),
+    "oracle_mlp_num_params": (_i64, []),
+}
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _F(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def toccA(Calpha, EVocc, EVvirt, UoccEOST):
+    nba, nvirt, nocc = Calpha.shape
+    noccEOS = UoccEOST.shape[1]
+    t = np.zeros((nocc, noccEOS, nvirt, nvirt), order="F")
+    Calpha, EVocc, EVvirt, UoccEOST = _F(Calpha), _F(EVocc), _F(EVvirt), _F(UoccEOST)
+    lib().oracle_toccA(nvirt, nocc, noccEOS, nba, _p(Calpha), _p(EVocc), _p(EVvirt), _p(t), _p(UoccEOST))
+    return t
+
+
+def tvirtA(Calpha, EVocc, EVvirt, UvirtEOST):
+    nba, nvirt, nocc = Calpha.shape
+    nvirtEOS = UvirtEOST.shape[1]
+    t = np.zeros((nocc, nocc, nvirtEOS, nvirt), order="F")
+    Calpha, EVocc, EVvirt, UvirtEOST = _F(Calpha), _F(EVocc), _F(EVvirt), _F(UvirtEOST)
+    lib().oracle_tvirtA(nvirt, nocc, nvirtEOS, nba, _p(Calpha), _p(EVocc), _p(EVvirt), _p(t), _p(UvirtEOST))
+    return t
+
+
+def gocc(C3):
+    nba, nv, no = C3.shape
+    g = np.zeros((nv, no, nv, no), order="F")
+    C3 = _F(C3)
+    lib().oracle_gocc(nv, no, nba, _p(C3), _p(g))
+    return g
+
+
+def gvirt(C3):
+    nba, nv, no = C3.shape
+    g = np.zeros((nv, no, nv, no), order="F")
+    C3 = _F(C3)
+    lib().oracle_gvirt(nv, no, nba, _p(C3), _p(g))
+    return g
+
+
+def gen4dimfo(C3, C4):
+    nba, n1, n2 = C3.shape
+    _, n3, n4 = C4.shape
+    out = np.zeros((n1, n2, n3, n4), order="F")
+    C3, C4 = _F(C3), _F(C4)
+    lib().oracle_gen4dimfo(nba, _p(C3), n1, n2, _p(C4), n3, n4, _p(out))
+    return out
+
+
+def steps5to8(Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST):
+    nba, nvirt, nocc = Calpha.shape
+    noccEOS, nvirtEOS = UoccEOST.shape[1], UvirtEOST.shape[1]
+    args = [_F(x) for x in (Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST)]
+    toccEOS = np.zeros((nvirt, noccEOS, nvirt, noccEOS), order="F")
+    tvirtEOS = np.zeros((nvirtEOS, nocc, nvirtEOS, nocc), order="F")
+    goccEOS = np.zeros((nvirt, noccEOS, nvirt, noccEOS), order="F")
+    gvirtEOS = np.zeros((nvirtEOS, nocc, nvirtEOS, nocc), order="F")
+    lib().oracle_steps5to8(nvirt, nocc, noccEOS, nvirtEOS, nba, *[_p(a) for a in args], _p(toccEOS), _p(tvirtEOS),
+                           _p(goccEOS), _p(gvirtEOS))
+    return toccEOS, tvirtEOS, goccEOS, gvirtEOS
+
+
+def pair_energies(B, eps_o, eps_v):
+    """Canonical RI-MP2: returns (E, eps_pair[nocc,nocc])."""
+    naux, nvirt, nocc = B.shape
+    B, eo, ev = _F(B), _F(eps_o), _F(eps_v)
+    ep = np.zeros((nocc, nocc), order="F")
+    et = np.zeros(1)
+    lib().oracle_pair_energies(nocc, nvirt, naux, _p(B), _p(eo), _p(ev), _p(ep), _p(et))
+    return float(et[0]), ep
+
+
+def pair_energies_subset(B, eps_o, eps_v, pi, pj):
+    naux, nvirt, nocc = B.shape
+    B, eo, ev = _F(B), _F(eps_o), _F(eps_v)
+    pi = np.ascontiguousarray(pi, dtype=np.int64)
+    pj = np.ascontiguousarray(pj, dtype=np.int64)
+    out = np.zeros(pi.size)
+    P = ctypes.POINTER(_i64)
+    lib().oracle_pair_energies_subset(nvirt, naux, _p(B), _p(eo), _p(ev), pi.size, pi.ctypes.data_as(P),
+                                      pj.ctypes.data_as(P), _p(out))
+    return out
+
+
+def pair_V(B, i, j):
+    naux, nvirt, nocc = B.shape
+    B = _F(B)
+    V = np.zeros((nvirt, nvirt), order="F")
+    lib().oracle_pair_V(nvirt, naux, _p(B), i, j, _p(V))
+    return V
+
+
+def cholesky(V):
+    n = V.shape[0]
+    V = _F(V)
+    L = np.zeros((n, n), order="F")
+    st = lib().oracle_cholesky(n, _p(V), _p(L))
+    if st:
+        raise np.linalg.LinAlgError(f"not positive definite at pivot {st}")
+    return L
+
+
+def trsm_lower(L, X):
+    n = L.shape[0]
+    X2 = _F(X).reshape(n, -1, order="F")
+    B = np.zeros_like(X2, order="F")
+    lib().oracle_trsm_lower(n, X2.shape[1], _p(_F(L)), _p(X2), _p(B))
+    return B.reshape(X.shape, order="F")
+
+
+def metric_transform(V, X):
+    """B = L^{-1} X with V = L L^T (X: [naux, ...])."""
+    n = V.shape[0]
+    X2 = _F(X).reshape(n, -1, order="F")
+    B = np.zeros_like(X2, order="F")
+    st = lib().oracle_metric_transform(n, X2.shape[1], _p(_F(V)), _p(X2), _p(B))
+    if st:
+        raise np.linalg.LinAlgError(f"not positive definite at pivot {st}")
+    return B.reshape(X.shape, order="F")
+
+
+def num_threads():
+    return lib().oracle_num_threads()

@@ -1,0 +1,413 @@
+// C-ABI entry points: context lifetime, device-memory query, errors, timers,
+// and the blocking / device-resident RI-MP2 energy path.
+#include <cmath>
+#include <mutex>
+
+#include "rimp2_common.h"
+
+namespace rimp2 {
+
+static std::string g_last_error;
+static std::mutex g_err_mu;
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw Error{code, std::string(buf)};
+}
+
+void set_error(void* ctx, const std::string& msg) {
+  if (ctx) static_cast<Ctx*>(ctx)->last_error = msg;
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_last_error = msg;
+}
+
+Ctx* as_ctx(void* ctx) {
+  if (!ctx) fail(RIMP2_ERR_ARG, "null context");
+  Ctx* c = static_cast<Ctx*>(ctx);
+  RIMP2_HIP_CHECK(hipSetDevice(c->device));
+  return c;
+}
+
+void* DevBuf::ensure(size_t need) {
+  if (need <= bytes && ptr) return ptr;
+  release();
+  size_t want = need < 256 ? 256 : need;
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    bytes = 0;
+    (void)hipGetLastError();
+    fail(RIMP2_ERR_NOMEM, "device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+  }
+  bytes = want;
+  return ptr;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+static hipEvent_t get_event(Ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  RIMP2_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+ScopedTimer::ScopedTimer(Ctx* ctx, int k) : c(ctx), kid(k) {
+  if (!c->timing) return;
+  e0 = get_event(c);
+  e1 = get_event(c);
+  RIMP2_HIP_CHECK(hipEventRecord(e0, c->stream));
+}
+
+ScopedTimer::~ScopedTimer() {
+  if (!e0) return;
+  if (hipEventRecord(e1, c->stream) == hipSuccess) c->pending.push_back({kid, e0, e1});
+}
+
+static void resolve_timers(Ctx* c) {
+  auto& q = c->pending;
+  for (auto& t : q) {
+    RIMP2_HIP_CHECK(hipEventSynchronize(t.e1));
+    float ms = 0.f;
+    RIMP2_HIP_CHECK(hipEventElapsedTime(&ms, t.e0, t.e1));
+    c->timers[t.kid].total_ms += ms;
+    c->timers[t.kid].count += 1;
+    c->ev_pool.push_back(t.e0);
+    c->ev_pool.push_back(t.e1);
+  }
+  q.clear();
+}
+
+static void check_host_finite(const double* x, int64_t n, const char* what) {
+  for (int64_t k = 0; k < n; k++)
+    if (!std::isfinite(x[k])) fail(RIMP2_ERR_NUMERIC, "%s[%lld] is not finite", what, (long long)k);
+}
+
+static void check_host_denominators(const double* eo, int64_t nocc, const double* ev, int64_t nvirt) {
+  double mo = -INFINITY, mv = INFINITY;
+  for (int64_t i = 0; i < nocc; i++) mo = std::fmax(mo, eo[i]);
+  for (int64_t a = 0; a < nvirt; a++) mv = std::fmin(mv, ev[a]);
+  if (!(2.0 * mo - 2.0 * mv < 0.0))
+    fail(RIMP2_ERR_NUMERIC, "orbital-energy denominator e_i+e_j-e_a-e_b >= 0 (max e_occ=%g, min e_virt=%g)", mo, mv);
+}
+
+// Reads and clears the device status word; converts set bits to an error.
+static void check_status(Ctx* c) {
+  int st[2] = {0, 0};
+  int* d = static_cast<int*>(c->status.ensure(64));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(st, d, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (st[0]) RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+  if (st[0] & 2) fail(RIMP2_ERR_NOT_SPD, "metric is not positive definite (pivot %d <= 0)", st[1]);
+  if (st[0] & 4) fail(RIMP2_ERR_NUMERIC, "non-finite value in a device input");
+  if (st[0] & 8) fail(RIMP2_ERR_NUMERIC, "orbital-energy denominator e_i+e_j-e_a-e_b >= 0");
+}
+
+void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
+
+static void validate_mode(int mode) {
+  if (mode < 0 || mode >= RIMP2_NUM_MODES) fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
+}
+
+static void validate_dims(int64_t naux, int64_t nvirt, int64_t nocc) {
+  if (naux <= 0 || nvirt <= 0 || nocc <= 0)
+    fail(RIMP2_ERR_ARG, "dimensions must be positive (naux=%lld nvirt=%lld nocc=%lld)", (long long)naux,
+         (long long)nvirt, (long long)nocc);
+  if (naux > 65536 || nvirt > 65536 || nocc > 65536) fail(RIMP2_ERR_ARG, "dimension above 65536");
+}
+
+// transform + pack from a device X; the metric must already be factored.
+static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX, int mode) {
+  validate_mode(mode);
+  if (!c->have_metric) fail(RIMP2_ERR_STATE, "rimp2_metric must run before the transform");
+  if (mode_slices(mode) && round_up(c->naux, 32) > 16384)
+    fail(RIMP2_ERR_ARG, "exact-slice modes support naux <= 16384 (int32 accumulation bound)");
+  metric_transform(c, nvirt, nocc, dX);
+  c->tile = choose_tile(mode, nvirt, nocc);
+  PackGeom g = make_geom(mode, c->naux, nvirt, c->tile);
+  const size_t bytes = g.bytes_per_occ() * (size_t)nocc;
+  void* dst = c->packed.ensure(bytes);
+  double* sc = nullptr;
+  if (mode_slices(mode)) sc = static_cast<double*>(c->scales.ensure((size_t)(g.nvp * nocc) * sizeof(double)));
+  pack_B(c, c->B64.as<double>(), c->naux, nvirt, nocc, g, dst, sc, c->stream);
+  c->geom = g;
+  c->nvirt = nvirt;
+  c->nocc = nocc;
+  c->mode = mode;
+  c->have_B = true;
+}
+
+}  // namespace rimp2
+
+using namespace rimp2;
+
+extern "C" {
+
+int rimp2_ctx_create(int device, void** ctx) {
+  RIMP2_API_BEGIN
+  if (!ctx) fail(RIMP2_ERR_ARG, "null ctx pointer");
+  *ctx = nullptr;
+  int ndev = 0;
+  RIMP2_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) fail(RIMP2_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+  RIMP2_HIP_CHECK(hipSetDevice(device));
+  Ctx* c = new Ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    fail(RIMP2_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+  }
+  *ctx = c;
+  RIMP2_API_END(nullptr)
+}
+
+int rimp2_ctx_destroy(void* ctx) {
+  RIMP2_API_BEGIN
+  if (!ctx) return RIMP2_OK;
+  Ctx* c = as_ctx(ctx);
+  (void)hipStreamSynchronize(c->stream);
+  resolve_timers(c);
+  batch_destroy(c->batch);
+  mlp_destroy(c->mlp);
+  for (DevBuf* b : {&c->V, &c->Linv, &c->work, &c->X, &c->B64, &c->packed, &c->scales, &c->eps_o, &c->eps_v,
+                    &c->partials, &c->eps_pair, &c->etot, &c->status})
+    b->release();
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  RIMP2_API_END(nullptr)
+}
+
+int rimp2_get_dev_mem(size_t* total, size_t* free_bytes) {
+  RIMP2_API_BEGIN
+  if (!total || !free_bytes) fail(RIMP2_ERR_ARG, "null output pointer");
+  size_t f = 0, t = 0;
+  RIMP2_HIP_CHECK(hipMemGetInfo(&f, &t));
+  *total = t;
+  *free_bytes = f;
+  RIMP2_API_END(nullptr)
+}
+
+int rimp2_last_error(void* ctx, char* buf, size_t n) {
+  if (!buf || n == 0) return RIMP2_ERR_ARG;
+  std::string m;
+  if (ctx) m = static_cast<Ctx*>(ctx)->last_error;
+  else {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    m = g_last_error;
+  }
+  size_t k = m.size() < n - 1 ? m.size() : n - 1;
+  memcpy(buf, m.data(), k);
+  buf[k] = '\0';
+  return RIMP2_OK;
+}
+
+int rimp2_metric(void* ctx, int64_t naux, const double* V) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (naux <= 0 || naux > 65536) fail(RIMP2_ERR_ARG, "naux=%lld out of range", (long long)naux);
+  if (!V) fail(RIMP2_ERR_ARG, "null V");
+  check_host_finite(V, naux * naux, "V");
+  c->have_metric = false;
+  c->have_B = false;
+  double* d = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(d, V, (size_t)(naux * naux) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  metric_factor(c, naux, d);
+  check_status(c);
+  c->have_metric = true;
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, int mode) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_dims(naux, nvirt, nocc);
+  validate_mode(mode);
+  if (!Pia) fail(RIMP2_ERR_ARG, "null Pia");
+  if (!c->have_metric || c->naux != naux) fail(RIMP2_ERR_STATE, "rimp2_metric with naux=%lld must run first", (long long)naux);
+  const int64_t n = naux * nvirt * nocc;
+  check_host_finite(Pia, n, "Pia");
+  double* dX = static_cast<double*>(c->X.ensure((size_t)n * sizeof(double)));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dX, Pia, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  transform_device(c, nvirt, nocc, dX, mode);
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v, int mode,
+                        double* eps_pair, double* e_tot) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_mode(mode);
+  if (!eps_o || !eps_v || !e_tot) fail(RIMP2_ERR_ARG, "null eps_o / eps_v / e_tot");
+  if (!c->have_B) fail(RIMP2_ERR_STATE, "rimp2_transform must run before rimp2_pair_energies");
+  if (c->nocc != nocc || c->nvirt != nvirt || c->mode != mode)
+    fail(RIMP2_ERR_STATE, "pair energies requested for (nocc=%lld, nvirt=%lld, mode=%d) but the transform holds "
+         "(%lld, %lld, %d)", (long long)nocc, (long long)nvirt, mode, (long long)c->nocc, (long long)c->nvirt, c->mode);
+  check_host_finite(eps_o, nocc, "eps_o");
+  check_host_finite(eps_v, nvirt, "eps_v");
+  check_host_denominators(eps_o, nocc, eps_v, nvirt);
+  double* deo = static_cast<double*>(c->eps_o.ensure((size_t)(nocc + nvirt) * sizeof(double)));
+  double* dev = deo + nocc;
+  RIMP2_HIP_CHECK(hipMemcpyAsync(deo, eps_o, (size_t)nocc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dev, eps_v, (size_t)nvirt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  double* dpe = eps_pair ? static_cast<double*>(c->eps_pair.ensure((size_t)(nocc * nocc) * sizeof(double))) : nullptr;
+  double* det = static_cast<double*>(c->etot.ensure(sizeof(double)));
+  pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, deo, dev, dpe, det, nullptr, 0);
+  if (eps_pair)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(eps_pair, dpe, (size_t)(nocc * nocc) * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(e_tot, det, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, const double* V,
+                 const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot) {
+  int st = rimp2_metric(ctx, naux, V);
+  if (st) return st;
+  st = rimp2_transform(ctx, naux, nvirt, nocc, Pia, mode);
+  if (st) return st;
+  return rimp2_pair_energies(ctx, nocc, nvirt, eps_o, eps_v, mode, eps_pair, e_tot);
+}
+
+int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Pia, const double* d_V,
+                        const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair,
+                        double* d_e_tot) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_dims(naux, nvirt, nocc);
+  validate_mode(mode);
+  if (!d_Pia || !d_V || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
+  check_inputs_device(c, d_V, naux * naux, "V");
+  check_inputs_device(c, d_Pia, naux * nvirt * nocc, "Pia");
+  check_inputs_device(c, d_eps_o, nocc, "eps_o");
+  check_inputs_device(c, d_eps_v, nvirt, "eps_v");
+  denom_check_device(c, d_eps_o, nocc, d_eps_v, nvirt);
+  metric_factor(c, naux, d_V);
+  c->have_metric = true;
+  transform_device(c, nvirt, nocc, d_Pia, mode);
+  pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,
+                       d_e_tot, nullptr, 0);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_get_cholesky(void* ctx, double* L) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (!c->have_metric) fail(RIMP2_ERR_STATE, "no metric factored");
+  if (!L) fail(RIMP2_ERR_ARG, "null L");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(L, c->V.ptr, (size_t)(c->naux * c->naux) * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_get_metric_inverse(void* ctx, double* Linv) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (!c->have_metric) fail(RIMP2_ERR_STATE, "no metric factored");
+  if (!Linv) fail(RIMP2_ERR_ARG, "null Linv");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(Linv, c->Linv.ptr, (size_t)(c->naux * c->naux) * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_get_B(void* ctx, double* B) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (!c->have_B) fail(RIMP2_ERR_STATE, "no transform done");
+  if (!B) fail(RIMP2_ERR_ARG, "null B");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(B, c->B64.ptr, (size_t)(c->naux * c->nvirt * c->nocc) * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_device_alloc(void* ctx, size_t bytes, void** dptr) {
+  RIMP2_API_BEGIN
+  as_ctx(ctx);
+  if (!dptr) fail(RIMP2_ERR_ARG, "null dptr");
+  *dptr = nullptr;
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    fail(RIMP2_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_device_free(void* ctx, void* dptr) {
+  RIMP2_API_BEGIN
+  as_ctx(ctx);
+  if (dptr) RIMP2_HIP_CHECK(hipFree(dptr));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_copy_h2d(void* ctx, void* dst, const void* src, size_t bytes) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if ((!dst || !src) && bytes) fail(RIMP2_ERR_ARG, "null pointer");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_copy_d2h(void* ctx, void* dst, const void* src, size_t bytes) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if ((!dst || !src) && bytes) fail(RIMP2_ERR_ARG, "null pointer");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_sync(void* ctx) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  check_status(c);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_timing_enable(void* ctx, int on) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  c->timing = (on != 0);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_timing_get(void* ctx, int kid, double* total_ms, int64_t* count) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (kid < 0 || kid >= RIMP2_NUM_KERNELS) fail(RIMP2_ERR_ARG, "kernel id %d out of range", kid);
+  resolve_timers(c);
+  if (total_ms) *total_ms = c->timers[kid].total_ms;
+  if (count) *count = c->timers[kid].count;
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_timing_reset(void* ctx) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  resolve_timers(c);
+  for (auto& t : c->timers) t = Timer{};
+  RIMP2_API_END(ctx)
+}
+
+}  // extern "C"

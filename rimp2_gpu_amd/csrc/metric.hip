@@ -1,0 +1,253 @@
+// FP64 metric step on the GPU:  V = L L^T (blocked Cholesky),  L^{-1}
+// (blocked triangular inverse) and the metric transform B = L^{-1} X, all on
+// one FP64-MFMA GEMM kernel (v_mfma_f64_16x16x4f64) plus two small LDS
+// kernels for the 64x64 diagonal blocks.
+//
+// The reference receives the fitted tensor Calpha(NBA,nvirt,nocc) from its
+// parent code (RIMP2.F90:48-49, "alphaCD ... in the diagonal basis"); this file
+// builds that tensor from the raw (P|ia) and the metric V_PQ.
+#include "rimp2_common.h"
+
+namespace rimp2 {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int NB = 64;  // Cholesky / inverse panel width
+
+// ---------------------------------------------------------------------------
+// Generic column-major FP64 GEMM on MFMA:
+//   C = alpha * A * op(B) + beta * C,   A: M x K (lda),  op(B): K x N.
+// TRANSB: op(B)[k,n] = B[n + ldb*k], else B[k + ldb*n].
+// TRI_A : A is lower triangular (A[m,k] = 0 for k > m): the K loop of a row
+//         tile stops at its last row.
+// LOWER_C: only tiles touching the lower triangle (n <= m) are computed;
+//         tiles strictly above the diagonal return at once.
+// 256 threads = 4 waves in 2x2, wave tile (BM/2) x (BN/2) of 16x16 MFMA blocks.
+// MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
+// result d[r] = D[4r + l/16][l%16].
+// ---------------------------------------------------------------------------
+template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+                                                       const double* A, int64_t lda, const double* B, int64_t ldb,
+                                                       double beta, double* C, int64_t ldc) {
+  constexpr int BK = 16;
+  constexpr int PA = BM + 4, PB = BN + 4;  // LDS pitches (doubles)
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  __shared__ double As[BK * PA];
+  __shared__ double Bs[BK * PB];
+
+  const int64_t m0 = (int64_t)blockIdx.x * BM;
+  const int64_t n0 = (int64_t)blockIdx.y * BN;
+  if (LOWER_C && n0 > m0 + BM - 1) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+
+  v4d acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  int64_t kend = K;
+  if (TRI_A) kend = (m0 + BM < K) ? (m0 + BM) : K;
+
+  for (int64_t k0 = 0; k0 < kend; k0 += BK) {
+    // A tile: BM x BK, column-major source -> As[k][m]
+    for (int e = tid; e < BM * BK; e += 256) {
+      int k = e / BM, m = e % BM;
+      int64_t gm = m0 + m, gk = k0 + k;
+      As[k * PA + m] = (gm < M && gk < K) ? A[gm + lda * gk] : 0.0;
+    }
+    // op(B) tile: BK x BN -> Bs[k][n]
+    if (TRANSB) {
+      for (int e = tid; e < BK * BN; e += 256) {
+        int k = e / BN, n = e % BN;
+        int64_t gn = n0 + n, gk = k0 + k;
+        Bs[k * PB + n] = (gn < N && gk < K) ? B[gn + ldb * gk] : 0.0;
+      }
+    } else {
+      for (int e = tid; e < BK * BN; e += 256) {
+        int k = e % BK, n = e / BK;
+        int64_t gn = n0 + n, gk = k0 + k;
+        Bs[k * PB + n] = (gn < N && gk < K) ? B[gk + ldb * gn] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; i++) af[i] = As[(ks + lane / 16) * PA + wm + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; j++) bf[j] = Bs[(ks + lane / 16) * PB + wn + 16 * j + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // epilogue
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int64_t gm = m0 + wm + 16 * i + 4 * r + lane / 16;
+        int64_t gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          if (LOWER_C && gn > gm) continue;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * C[gm + ldc * gn];
+          C[gm + ldc * gn] = v;
+        }
+      }
+}
+
+template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
+void gemm_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN));
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TRANSB, TRI_A, LOWER_C>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda,
+                     B, ldb, beta, C, ldc);
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block: factor A[k0:k0+kb, k0:k0+kb] = L11 L11^T in LDS, write L11
+// back (zeros above the diagonal) and L11^{-1} into Linv at the same block.
+// status bit 1 is set on a non-positive pivot; pivot_at receives the global
+// index of the first failing pivot (1-based).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
+                                                         int64_t k0, int kb, int* status) {
+  __shared__ double L[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (int e = tid; e < NB * NB; e += 256) {
+    int r = e % NB, c = e / NB;
+    L[r][c] = (r < kb && c < kb && c <= r) ? A[(k0 + r) + lda * (k0 + c)] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < kb; c++) {
+    if (tid == 0) {
+      double d = L[c][c];
+      if (!(d > 0.0) && !bad) {
+        bad = 1;
+        atomicCAS(status + 1, 0, (int)(k0 + c + 1));
+        atomicOr(status, 2);
+      }
+      L[c][c] = (d > 0.0) ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double inv = 1.0 / L[c][c];
+    for (int r = c + 1 + tid; r < kb; r += 256) L[r][c] *= inv;
+    __syncthreads();
+    // trailing update of the lower triangle
+    const int nt = kb - c - 1;
+    for (int e = tid; e < nt * nt; e += 256) {
+      int r = c + 1 + e % nt, q = c + 1 + e / nt;
+      if (q <= r) L[r][q] -= L[r][c] * L[q][c];
+    }
+    __syncthreads();
+  }
+  // L11^{-1}: column j solved by thread j (forward substitution)
+  if (tid < kb) {
+    const int j = tid;
+    for (int r = 0; r < kb; r++) {
+      double v;
+      if (r < j) v = 0.0;
+      else {
+        double s = (r == j) ? 1.0 : 0.0;
+        for (int m = j; m < r; m++) s -= L[r][m] * X[m][j];
+        v = s / L[r][r];
+      }
+      X[r][j] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kb * kb; e += 256) {
+    int r = e % kb, c = e / kb;
+    A[(k0 + r) + lda * (k0 + c)] = (c <= r) ? L[r][c] : 0.0;
+    Linv[(k0 + r) + ldi * (k0 + c)] = X[r][c];
+  }
+}
+
+__global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  int64_t r = e % n, c = e / n;
+  if (c > r) A[r + lda * c] = 0.0;
+}
+
+}  // namespace
+
+// dV_in (device, naux x naux) -> c->V holds L (lower, zeros above),
+// c->Linv holds L^{-1}.  Sets RIMP2_ERR_NOT_SPD on a non-positive pivot.
+void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
+  hipStream_t s = c->stream;
+  double* L = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
+  double* Li = static_cast<double*>(c->Linv.ensure((size_t)(n * n) * sizeof(double)));
+  int* status = static_cast<int*>(c->status.ensure(64));
+  if (dV_in != L) RIMP2_HIP_CHECK(hipMemcpyAsync(L, dV_in, (size_t)(n * n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+  RIMP2_HIP_CHECK(hipMemsetAsync(Li, 0, (size_t)(n * n) * sizeof(double), s));
+  RIMP2_HIP_CHECK(hipMemsetAsync(status, 0, 64, s));
+  {
+    ScopedTimer t(c, RIMP2_K_CHOLESKY);
+    for (int64_t k0 = 0; k0 < n; k0 += NB) {
+      const int kb = (int)((n - k0) < NB ? (n - k0) : NB);
+      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, L, n, Li, n, k0, kb, status);
+      RIMP2_HIP_CHECK(hipGetLastError());
+      const int64_t r0 = k0 + kb, m = n - r0;
+      if (m <= 0) break;
+      // panel: L21 = A21 * L11^{-T}   (in place; one N tile covers kb <= 64)
+      gemm_f64<64, 64, true, false, false>(s, m, kb, kb, 1.0, L + r0 + n * k0, n, Li + k0 + n * k0, n, 0.0,
+                                           L + r0 + n * k0, n);
+      // trailing: A22 -= L21 L21^T  (lower tiles only)
+      gemm_f64<128, 128, true, false, true>(s, m, m, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
+                                            L + r0 + n * r0, n);
+    }
+    hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, s, L, n, n);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  {
+    // L^{-1} by recursive doubling over the 64-wide diagonal blocks whose
+    // inverses potrf_diag already wrote:  for [[P,0],[Q,R]],
+    // inv = [[P^-1, 0], [-R^-1 Q P^-1, R^-1]].
+    ScopedTimer t(c, RIMP2_K_TRTRI);
+    const int64_t nblk = ceil_div(n, NB);
+    double* T = static_cast<double*>(c->work.ensure((size_t)(n * n) * sizeof(double)));
+    for (int64_t w = 1; w < nblk; w *= 2) {
+      for (int64_t g = 0; g * 2 * w < nblk; g++) {
+        const int64_t lo = g * 2 * w * NB, mid = (g * 2 * w + w) * NB;
+        if (mid >= n) continue;
+        const int64_t hi = ((g * 2 * w + 2 * w) * NB < n) ? (g * 2 * w + 2 * w) * NB : n;
+        const int64_t sp = mid - lo, sr = hi - mid;
+        // T = Q * P^{-1}   (Q = L[mid:hi, lo:mid], P^{-1} = Linv[lo:mid, lo:mid])
+        gemm_f64<64, 64, false, false, false>(s, sr, sp, sp, 1.0, L + mid + n * lo, n, Li + lo + n * lo, n, 0.0, T,
+                                              sr);
+        // Linv[mid:hi, lo:mid] = -R^{-1} T   (R^{-1} lower triangular)
+        gemm_f64<64, 64, false, true, false>(s, sr, sp, sr, -1.0, Li + mid + n * mid, n, T, sr, 0.0,
+                                             Li + mid + n * lo, n);
+      }
+    }
+  }
+  c->naux = n;
+}
+
+// B64 = Linv * X  (X: naux x (nvirt*nocc) device, column-major).
+void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
+  const int64_t n = c->naux, ncol = nvirt * nocc;
+  double* B = static_cast<double*>(c->B64.ensure((size_t)(n * ncol) * sizeof(double)));
+  ScopedTimer t(c, RIMP2_K_TRANSFORM);
+  gemm_f64<128, 128, false, true, false>(c->stream, n, ncol, n, 1.0, c->Linv.as<double>(), n, dX, n, 0.0, B, n);
+}
+
+}  // namespace rimp2

@@ -1,0 +1,208 @@
+// Packing of the transformed 3-index tensor B[naux, nvirt, nocc] (FP64) into
+// the pair kernel's operand layout, one layout per precision mode.
+//
+// Layout ("MFMA-operand order"): for occupied i, k-step ks (kk auxiliary
+// values), row block rb (rb virtuals) and plane p, 64 lanes x lane_bytes are
+// stored in lane order, so that a wave's operand fragment is one contiguous
+// 64*lane_bytes chunk both in HBM and in LDS:
+//     packed[i][ks][rb][p][lane][lane_bytes]
+// The same fragment serves as the row (A) and the column (B) operand because
+// the MFMA A[m][k] and B[k][n] lane maps coincide (m <-> n):
+//   fp64 16x16x4 : lane l holds (row l%16, k l/16)              8 bytes
+//   fp32 32x32x2 : lane l holds (row l%32, k l/32)              4 bytes
+//   bf16 32x32x16: lane l holds (row l%32, k 8*(l/32)+e, e<8)  16 bytes
+//   i8   32x32x32: lane l holds (row l%32, k 16*(l/32)+e, e<16) 16 bytes
+// Padding rows (virtual a >= nvirt) and padding k (Q >= naux) are zero.
+//
+// Exact-slice (int8) modes: per row (a,i) the scale sigma = max_Q |B[Q,a,i]|,
+// x = 127 B / sigma in [-127, 127];  s_1 = rint(x), r_1 = x - s_1,
+// s_k = rint(254 r_{k-1}), r_k = 254 r_{k-1} - s_k  (|r| <= 1/2 keeps every
+// slice in [-127, 127]).  Then B = sigma/127 * sum_k s_k 254^{-(k-1)} + O(254^{-S}).
+#include "rimp2_common.h"
+
+namespace rimp2 {
+
+int mode_slices(int mode) {
+  switch (mode) {
+    case RIMP2_MODE_I8S4: return 4;
+    case RIMP2_MODE_I8S5: return 5;
+    default: return 0;
+  }
+}
+
+// Auxiliary padding quantum per mode: one LDS stage of the pair kernel.
+static int64_t q_quantum(int mode) {
+  switch (mode) {
+    case RIMP2_MODE_FP64: return 16;
+    case RIMP2_MODE_FP32: return 16;
+    case RIMP2_MODE_BF16X3: return 32;
+    default: return 32;
+  }
+}
+
+PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile) {
+  PackGeom g;
+  g.mode = mode;
+  switch (mode) {
+    case RIMP2_MODE_FP64: g.rb = 16; g.kk = 4; g.planes = 1; g.lane_bytes = 8; break;
+    case RIMP2_MODE_FP32: g.rb = 32; g.kk = 2; g.planes = 1; g.lane_bytes = 4; break;
+    case RIMP2_MODE_BF16X3: g.rb = 32; g.kk = 16; g.planes = 2; g.lane_bytes = 16; break;
+    case RIMP2_MODE_I8S4:
+    case RIMP2_MODE_I8S5: g.rb = 32; g.kk = 32; g.planes = mode_slices(mode); g.lane_bytes = 16; break;
+    default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
+  }
+  g.nvp = round_up(nvirt, tile);
+  g.nqp = round_up(naux, q_quantum(mode));
+  g.nks = g.nqp / g.kk;
+  g.nrb = g.nvp / g.rb;
+  return g;
+}
+
+namespace {
+
+// sigma[i*nvp + a] = max_Q |B[Q,a,i]| (0 for padding rows); one wave per row.
+__global__ __launch_bounds__(256) void rowmax_kernel(const double* __restrict__ B, int64_t naux, int64_t nvirt,
+                                                     int64_t nocc, int64_t nvp, double* __restrict__ sigma) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nvp * nocc) return;
+  const int64_t i = row / nvp, a = row % nvp;
+  double m = 0.0;
+  if (a < nvirt) {
+    const double* col = B + naux * (a + nvirt * i);
+    for (int64_t q = lane; q < naux; q += 64) m = fmax(m, fabs(col[q]));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if (lane == 0) sigma[row] = m;
+}
+
+struct PackArgs {
+  const double* B;
+  int64_t naux, nvirt, nocc;
+  int64_t nks, nrb;
+  const double* sigma;
+  void* dst;
+};
+
+__device__ inline double bget(const PackArgs& p, int64_t q, int64_t a, int64_t i) {
+  return (q < p.naux && a < p.nvirt) ? p.B[q + p.naux * (a + p.nvirt * i)] : 0.0;
+}
+
+// one thread per (i, ks, rb, lane)
+__global__ __launch_bounds__(256) void pack_fp64_kernel(PackArgs p) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t total = p.nocc * p.nks * p.nrb * 64;
+  if (t >= total) return;
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;  // fragment index = (i*nks + ks)*nrb + rb
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
+  static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
+}
+
+__global__ __launch_bounds__(256) void pack_fp32_kernel(PackArgs p) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t total = p.nocc * p.nks * p.nrb * 64;
+  if (t >= total) return;
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
+  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
+}
+
+// bf16 round-to-nearest-even of a float (no NaN inputs reach here).
+__device__ inline uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// planes: 0 = hi = bf16(b), 1 = lo = bf16(b - hi)
+__global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t total = p.nocc * p.nks * p.nrb * 64;
+  if (t >= total) return;
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
+  uint16_t hi[8], lo[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    float b = (float)bget(p, q0 + e, a, i);
+    hi[e] = f2bf(b);
+    lo[e] = f2bf(b - bf2f(hi[e]));
+  }
+  uint4 vh, vl;
+  vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
+  vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
+  vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
+  vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
+  uint4* dst = static_cast<uint4*>(p.dst);
+  dst[(f * 2 + 0) * 64 + lane] = vh;
+  dst[(f * 2 + 1) * 64 + lane] = vl;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void pack_i8_kernel(PackArgs p, int64_t nvp) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t total = p.nocc * p.nks * p.nrb * 64;
+  if (t >= total) return;
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
+  const double sg = p.sigma[i * nvp + a];
+  const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
+  int8_t sl[S][16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    double r = bget(p, q0 + e, a, i) * inv;
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      double s = rint(r);
+      s = fmin(127.0, fmax(-127.0, s));
+      sl[k][e] = (int8_t)s;
+      r = (r - s) * 254.0;
+    }
+  }
+  uint4* dst = static_cast<uint4*>(p.dst);
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    uint4 v;
+    memcpy(&v, sl[k], 16);
+    dst[(f * S + k) * 64 + lane] = v;
+  }
+}
+
+}  // namespace
+
+void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
+            double* dscales, hipStream_t s) {
+  ScopedTimer tm(c, RIMP2_K_PACK);
+  PackArgs p{dB, naux, nvirt, nocc, g.nks, g.nrb, dscales, dst};
+  const int64_t threads = nocc * g.nks * g.nrb * 64;
+  const unsigned blocks = (unsigned)ceil_div(threads, 256);
+  switch (g.mode) {
+    case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_fp64_kernel, dim3(blocks), dim3(256), 0, s, p); break;
+    case RIMP2_MODE_FP32: hipLaunchKernelGGL(pack_fp32_kernel, dim3(blocks), dim3(256), 0, s, p); break;
+    case RIMP2_MODE_BF16X3: hipLaunchKernelGGL(pack_bf16x3_kernel, dim3(blocks), dim3(256), 0, s, p); break;
+    case RIMP2_MODE_I8S4:
+    case RIMP2_MODE_I8S5: {
+      hipLaunchKernelGGL(rowmax_kernel, dim3((unsigned)ceil_div(g.nvp * nocc, 4)), dim3(256), 0, s, dB, naux, nvirt,
+                         nocc, g.nvp, dscales);
+      RIMP2_HIP_CHECK(hipGetLastError());
+      if (g.mode == RIMP2_MODE_I8S4)
+        hipLaunchKernelGGL(pack_i8_kernel<4>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      else
+        hipLaunchKernelGGL(pack_i8_kernel<5>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      break;
+    }
+    default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", g.mode);
+  }
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace rimp2

@@ -1,0 +1,153 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rimp2_hip.h"
+
+namespace rimp2 {
+
+// Thrown inside the library only; every extern "C" entry point catches it and
+// turns it into a status code + last-error text.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...);
+
+#define RIMP2_HIP_CHECK(expr)                                                                   \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) ::rimp2::fail(RIMP2_ERR_HIP, "%s failed: %s (%s:%d)", #expr,          \
+                                        hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* ensure(size_t need);
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Geometry of the packed ("MFMA-operand order") B for one precision mode.
+// Global order: [i][kstep][rowblock][plane][64 lanes x lane_bytes].
+struct PackGeom {
+  int mode = -1;
+  int rb = 0;          // rows per MFMA operand fragment (16 or 32)
+  int kk = 0;          // k values per fragment (one MFMA K step)
+  int planes = 0;      // 1 (fp64/fp32), 2 (bf16 hi/lo), S (int8 slices)
+  int lane_bytes = 0;  // bytes each lane holds per fragment
+  int64_t nvp = 0;     // padded nvirt (multiple of the pair tile)
+  int64_t nqp = 0;     // padded naux (multiple of kk * ksteps per LDS stage)
+  int64_t nks = 0;     // nqp / kk
+  int64_t nrb = 0;     // nvp / rb
+  size_t frag_bytes() const { return (size_t)64 * lane_bytes; }
+  size_t bytes_per_occ() const { return (size_t)nks * nrb * planes * frag_bytes(); }
+};
+
+struct Timer {
+  double total_ms = 0.0;
+  int64_t count = 0;
+};
+
+// event pair recorded around one launch, resolved lazily by rimp2_timing_get
+struct PendingTimer {
+  int kid;
+  hipEvent_t e0, e1;
+};
+
+struct BatchState;
+struct MlpState;
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // metric state
+  int64_t naux = 0;
+  bool have_metric = false;
+  DevBuf V, Linv, work;
+  // transform state
+  int64_t nvirt = 0, nocc = 0;
+  int mode = -1;
+  bool have_B = false;
+  DevBuf X, B64, packed, scales;
+  PackGeom geom;
+  int tile = 0;  // pair tile (virtual block) used for the packed layout
+  // pair state
+  DevBuf eps_o, eps_v, partials, eps_pair, etot, status;
+  // timing
+  bool timing = false;
+  Timer timers[RIMP2_NUM_KERNELS];
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<PendingTimer> pending;
+  // batch + surrogate
+  BatchState* batch = nullptr;
+  MlpState* mlp = nullptr;
+};
+
+// Records hipEvents around a launch on ctx->stream when timing is on.
+struct ScopedTimer {
+  Ctx* c;
+  int kid;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ScopedTimer(Ctx* ctx, int k);
+  ~ScopedTimer();
+};
+
+// --- metric (metric.hip) ---
+void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
+void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
+
+// --- packing (pack.hip) ---
+PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile);
+void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
+            double* dscales, hipStream_t s);
+int mode_slices(int mode);
+
+// --- pair kernel (pair.hip) ---
+int choose_tile(int mode, int64_t nvirt, int64_t nocc);
+void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const double* scales, int64_t nocc,
+                          int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
+                          double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
+void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
+
+// --- batch / mlp cleanup ---
+void batch_destroy(BatchState* b);
+void mlp_destroy(MlpState* m);
+
+// helpers
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+}  // namespace rimp2
+
+// Wraps an extern "C" body: converts rimp2::Error into a status code.
+#define RIMP2_API_BEGIN try {
+#define RIMP2_API_END(ctxp)                                         \
+  }                                                                 \
+  catch (const ::rimp2::Error& e) {                                 \
+    ::rimp2::set_error((ctxp), e.msg);                              \
+    return e.code;                                                  \
+  }                                                                 \
+  catch (...) {                                                     \
+    ::rimp2::set_error((ctxp), std::string("unknown C++ exception")); \
+    return RIMP2_ERR_HIP;                                           \
+  }                                                                 \
+  return RIMP2_OK;
+
+namespace rimp2 {
+void set_error(void* ctx, const std::string& msg);
+Ctx* as_ctx(void* ctx);
+}  // namespace rimp2

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+committed golden fixtures.
+
+Tolerances (stated per mode; E_ref is the oracle's FP64 energy):
+  fp64  : |dE| <= 1e-11 |E_ref|, pair energies rel 1e-10 (summation order only)
+  i8s5  : |dE| <  1 microhartree (1e-6 Eh) -- the exact-slice target
+  i8s4  : |dE| <  1 microhartree at these sizes
+  fp32  : |dE| <= 2e-7 |E_ref|   (FP32 inputs and accumulation)
+  bf16x3: |dE| <= 2e-5 |E_ref|   (2-way BF16 split, FP32 accumulation)
+Metric: Cholesky / inverse / B rel 1e-10 of the largest entry.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+TOL_REL = {"fp64": 1e-11, "fp32": 2e-7, "bf16x3": 2e-5}
+TOL_ABS = {"i8s5": 1e-6, "i8s4": 1e-6}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def check_E(mode, E, Eref):
+    if mode in TOL_ABS:
+        assert abs(E - Eref) < TOL_ABS[mode], (mode, E, Eref, E - Eref)
+    else:
+        assert abs(E - Eref) <= TOL_REL[mode] * abs(Eref), (mode, E, Eref, (E - Eref) / Eref)
+
+
+@pytest.mark.parametrize("naux", [1, 5, 64, 100, 130, 512])
+def test_metric_factor_and_inverse(ctx, oracle, rimp2, naux):
+    say("metric naux", naux)
+    _, V, _, _ = rimp2.gen_fragment(100 + naux, naux, 2, 1)
+    ctx.metric(V)
+    L = ctx.get_cholesky(naux)
+    Lref = oracle.cholesky(V)
+    assert np.max(np.abs(L - Lref)) <= 1e-10 * np.abs(Lref).max()
+    Li = ctx.get_metric_inverse(naux)
+    np.testing.assert_allclose(Li @ Lref, np.eye(naux), rtol=0, atol=1e-10)
+    assert np.all(np.triu(Li, 1) == 0) and np.all(np.triu(L, 1) == 0)
+
+
+@pytest.mark.parametrize("shape", [(64, 16, 3), (100, 17, 2), (130, 70, 3)])
+def test_transform_matches_oracle(ctx, oracle, rimp2, shape):
+    naux, nvirt, nocc = shape
+    say("transform", shape)
+    P, V, eo, ev = rimp2.gen_fragment(7, naux, nvirt, nocc)
+    ctx.metric(V)
+    ctx.transform(P, "fp64")
+    B = ctx.get_B(naux, nvirt, nocc)
+    Bref = oracle.metric_transform(V, P)
+    assert np.max(np.abs(B - Bref)) <= 1e-10 * np.abs(Bref).max()
+
+
+GOLDEN = sorted(f for f in os.listdir(GOLD) if f.startswith("frag_"))
+
+
+@pytest.mark.parametrize("fname", GOLDEN)
+def test_fp64_against_golden(ctx, rimp2, fname):
+    g = np.load(os.path.join(GOLD, fname))
+    say("golden", fname)
+    P, V, eo, ev = rimp2.gen_fragment(int(g["seed"]), int(g["naux"]), int(g["nvirt"]), int(g["nocc"]))
+    E, eps = ctx.energy(P, V, eo, ev, "fp64")
+    check_E("fp64", E, float(g["E"]))
+    np.testing.assert_allclose(eps, g["eps_pair"], rtol=1e-10, atol=1e-14)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5", "i8s4", "fp32", "bf16x3"])
+@pytest.mark.parametrize("fname", ["frag_4_512_128_16.npz", "frag_3_128_70_4.npz"])
+def test_modes_against_golden(ctx, rimp2, mode, fname):
+    g = np.load(os.path.join(GOLD, fname))
+    say("mode", mode, fname)
+    P, V, eo, ev = rimp2.gen_fragment(int(g["seed"]), int(g["naux"]), int(g["nvirt"]), int(g["nocc"]))
+    E, eps = ctx.energy(P, V, eo, ev, mode)
+    check_E(mode, E, float(g["E"]))
+    np.testing.assert_array_equal(eps, eps.T)
+    assert np.all(eps < 0)
+
+
+EDGE = [(16, 1, 1), (8, 3, 1), (20, 33, 2), (37, 65, 3), (64, 130, 5), (5, 2, 4), (48, 100, 1)]
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5", "fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", EDGE)
+def test_ragged_shapes(ctx, oracle, rimp2, mode, shape):
+    naux, nvirt, nocc = shape
+    say("edge", mode, shape)
+    P, V, eo, ev = rimp2.gen_fragment(1000 + naux + nvirt + nocc, naux, nvirt, nocc)
+    B = oracle.metric_transform(V, P)
+    Eref, epsref = oracle.pair_energies(B, eo, ev)
+    E, eps = ctx.energy(P, V, eo, ev, mode)
+    check_E(mode, E, Eref)
+    if mode == "fp64":
+        np.testing.assert_allclose(eps, epsref, rtol=1e-10, atol=1e-15)
+
+
+def test_device_resident_path_matches_host(ctx, rimp2):
+    naux, nvirt, nocc = 256, 96, 6
+    say("device path", naux, nvirt, nocc)
+    P, V, eo, ev = rimp2.gen_fragment(77, naux, nvirt, nocc)
+    for mode in ["fp64", "i8s5"]:
+        E_host, eps_host = ctx.energy(P, V, eo, ev, mode)
+        d = [ctx.to_device(x) for x in (P, V, eo, ev)]
+        dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+        ctx.energy_device(naux, nvirt, nocc, *d, mode, dpe, det)
+        ctx.sync()
+        E_dev = float(ctx.from_device(det, (1,))[0])
+        eps_dev = ctx.from_device(dpe, (nocc, nocc))
+        for p in d + [dpe, det]:
+            ctx.free(p)
+        assert E_dev == E_host
+        np.testing.assert_array_equal(eps_dev, eps_host)
+
+
+def test_deterministic(ctx, rimp2):
+    P, V, eo, ev = rimp2.gen_fragment(5, 192, 80, 7)
+    say("determinism")
+    for mode in ["fp64", "i8s5", "fp32"]:
+        E1, e1 = ctx.energy(P, V, eo, ev, mode)
+        E2, e2 = ctx.energy(P, V, eo, ev, mode)
+        assert E1 == E2
+        np.testing.assert_array_equal(e1, e2)
+
+
+def test_error_paths(ctx, rimp2):
+    say("error paths")
+    P, V, eo, ev = rimp2.gen_fragment(9, 32, 8, 2)
+    bad = V.copy(order="F")
+    bad[5, 5] = -10.0
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.metric(bad)
+    assert ei.value.code == rimp2.ERR_NOT_SPD
+    ctx.metric(V)
+    ctx.transform(P, "fp64")
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.pair_energies(eo, ev, "i8s5")  # transform holds fp64
+    assert ei.value.code == rimp2.ERR_STATE
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.pair_energies(eo, ev - 5.0, "fp64")  # denominators >= 0
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    Pn = P.copy(order="F")
+    Pn[3, 2, 1] = np.nan
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.transform(Pn, "fp64")
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    with pytest.raises(ValueError):
+        ctx.transform(P, "fp16")
+    # context is still usable after errors
+    ctx.metric(V)
+    E, _ = ctx.energy(P, V, eo, ev, "fp64")
+    assert E < 0
+
+
+def test_device_path_flags_bad_input(ctx, rimp2):
+    say("device error flags")
+    naux, nvirt, nocc = 32, 8, 2
+    P, V, eo, ev = rimp2.gen_fragment(9, naux, nvirt, nocc)
+    d = [ctx.to_device(x) for x in (P, V, eo, ev - 10.0)]
+    det = ctx.alloc(8)
+    ctx.energy_device(naux, nvirt, nocc, *d, "fp64", None, det)
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.sync()
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    for p in d + [det]:
+        ctx.free(p)
+    ctx.sync()  # status was cleared
+
+
+def test_c2_shape_exact_slice_within_1uEh(ctx, oracle, rimp2):
+    """64/512/2048 fragment: i8s5 vs fp64 energy within 1 uEh; a seeded subset
+    of pair energies checked against the oracle."""
+    naux, nvirt, nocc = 2048, 512, 64
+    say("c2 shape", naux, nvirt, nocc)
+    P, V, eo, ev = rimp2.gen_fragment(2, naux, nvirt, nocc)
+    E64, eps64 = ctx.energy(P, V, eo, ev, "fp64")
+    B = ctx.get_B(naux, nvirt, nocc)
+    rng = np.random.default_rng(0)
+    pi = rng.integers(0, nocc, 12)
+    pj = rng.integers(0, nocc, 12)
+    ref = oracle.pair_energies_subset(B, eo, ev, pi, pj)
+    np.testing.assert_allclose(eps64[pi, pj], ref, rtol=1e-11)
+    E8, eps8 = ctx.energy(P, V, eo, ev, "i8s5")
+    say("c2 E64", E64, "E_i8s5", E8, "dE uEh", (E8 - E64) * 1e6)
+    assert abs(E8 - E64) < 1e-6
+    np.testing.assert_array_equal(eps8, eps8.T)
+    assert np.all(eps64 < 0)

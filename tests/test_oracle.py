@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (CPU-only tests).
+
+Each check is chosen so that a plausible mistake in the restatement (wrong
+index order, transposed U, wrong sign of Delta, dropped exchange term) fails
+at least one of them:
+  * closed-form values of the reference's own fixtures (generate.F90:77-128
+    pushed through Major Steps 5-8, RIMP2.F90:67-170), abs tol 1e-10 as the
+    reference's check (RIMP2.F90:262);
+  * an independent numpy einsum of Steps 5-8 on random, non-constant inputs;
+  * pair-energy brute force, closed forms and invariants;
+  * LAPACK (scipy) for the metric step.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+
+# ---------------------------------------------------------------------------
+# reference fixtures (generate.F90:77-128), evaluated in the same order
+# ---------------------------------------------------------------------------
+def ref_fixture(nvirt, nocc, noccEOS, nvirtEOS, nba):
+    A = np.arange(1, nba + 1, dtype=np.float64)[:, None, None]
+    B = np.arange(1, nvirt + 1, dtype=np.float64)[None, :, None]
+    C = np.arange(1, nocc + 1, dtype=np.float64)[None, None, :]
+    Calpha = np.asfortranarray(A * 0.00124235636 / A + B * 0.01010101 / B + C * 0.014321 / C)
+    c = np.arange(1, nocc + 1, dtype=np.float64)
+    EVocc = c * 1.934321 / c
+    c = np.arange(1, nvirt + 1, dtype=np.float64)
+    EVvirt = c * 1.91321 / c
+
+    def U(n1, n2, x, y):
+        a = np.arange(1, n1 + 1, dtype=np.float64)[:, None]
+        b = np.arange(1, n2 + 1, dtype=np.float64)[None, :]
+        return np.asfortranarray(a * x / a + b * y / b)
+
+    UoccEOST = U(nocc, noccEOS, 0.030321, 0.101121)
+    UvirtT = U(nvirt, nvirt, 0.021321, 0.01321)
+    UoccT = U(nocc, nocc, 0.032321, 0.01521)
+    UvirtEOST = U(nvirt, nvirtEOS, 0.023321, 0.01421)
+    return Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST
+
+
+def closed_form(nvirt, nocc, nba):
+    c = 0.00124235636 + 0.01010101 + 0.014321
+    d = 2 * 1.934321 - 2 * 1.91321
+    uoE, uv, uo, uvE = 0.030321 + 0.101121, 0.021321 + 0.01321, 0.032321 + 0.01521, 0.023321 + 0.01421
+    tocc = nba * c * c / d * (nocc * uoE) ** 2 * (nvirt * uv) ** 2
+    tvirt = nba * c * c / d * (nvirt * uvE) ** 2 * (nocc * uo) ** 2
+    gocc = nba * (nvirt * uv * nocc * uoE * c) ** 2
+    gvirt = nba * (nvirt * uvE * nocc * uo * c) ** 2
+    return tocc, tvirt, gocc, gvirt
+
+
+# file1: generate.F90:41-46, file2: generate.F90:58-63; values tabulated from
+# the fixture formulas.
+FIXTURES = [
+    ((74, 10, 4, 15, 273), (48.0435406249666, 7.4213438296809, 2.02849437226733, 0.313343979176785)),
+    ((104, 20, 4, 15, 373), (518.614220438611, 80.1109659023915, 21.8969296153589, 3.38244520233076)),
+]
+
+
+@pytest.mark.parametrize("dims,expect", FIXTURES)
+def test_reference_fixture_closed_form(oracle, dims, expect):
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    cf = closed_form(nvirt, nocc, nba)
+    np.testing.assert_allclose(cf, expect, rtol=0, atol=1e-10)
+    outs = oracle.steps5to8(*ref_fixture(nvirt, nocc, noccEOS, nvirtEOS, nba))
+    shapes = [(nvirt, noccEOS, nvirt, noccEOS), (nvirtEOS, nocc, nvirtEOS, nocc),
+              (nvirt, noccEOS, nvirt, noccEOS), (nvirtEOS, nocc, nvirtEOS, nocc)]
+    for arr, shp, val in zip(outs, shapes, expect):
+        assert arr.shape == shp
+        assert np.all(np.isfinite(arr))
+        # the reference's own check: ABS(test - got) > 1e-10 -> ERROR3
+        assert np.max(np.abs(arr - val)) <= 1e-10
+
+
+def numpy_steps5to8(C, eo, ev, UoE, Uv, Uo, UvE):
+    """Independent restatement in einsum form (canonical basis -> local)."""
+    g = np.einsum("qai,qbj->aibj", C, C)  # g(a,i,b,j)
+    D = eo[None, :, None, None] + eo[None, None, None, :] - ev[:, None, None, None] - ev[None, None, :, None]
+    t = g / D
+    # toccEOS(Al,Il,Bl,Jl) = sum U_v(A,Al) U_oE(I,Il) U_v(B,Bl) U_oE(J,Jl) t(A,I,B,J)
+    toccEOS = np.einsum("aibj,aA,iI,bB,jJ->AIBJ", t, Uv, UoE, Uv, UoE)
+    # tvirtEOS(Al,Il,Bl,Jl) = sum U_vE(A,Al) U_o(I,Il) U_vE(B,Bl) U_o(J,Jl) t(A,I,B,J)
+    tvirtEOS = np.einsum("aibj,aA,iI,bB,jJ->AIBJ", t, UvE, Uo, UvE, Uo)
+    goccEOS = np.einsum("aibj,aA,iI,bB,jJ->AIBJ", g, Uv, UoE, Uv, UoE)
+    gvirtEOS = np.einsum("aibj,aA,iI,bB,jJ->AIBJ", g, UvE, Uo, UvE, Uo)
+    return toccEOS, tvirtEOS, goccEOS, gvirtEOS
+
+
+def test_steps5to8_random_inputs(oracle):
+    rng = np.random.default_rng(11)
+    nba, nvirt, nocc, noccEOS, nvirtEOS = 7, 6, 4, 3, 5
+    C = np.asfortranarray(rng.standard_normal((nba, nvirt, nocc)))
+    eo = rng.uniform(-2, -0.5, nocc)
+    ev = rng.uniform(0.2, 2, nvirt)
+    UoE = np.asfortranarray(rng.standard_normal((nocc, noccEOS)))
+    Uv = np.asfortranarray(rng.standard_normal((nvirt, nvirt)))
+    Uo = np.asfortranarray(rng.standard_normal((nocc, nocc)))
+    UvE = np.asfortranarray(rng.standard_normal((nvirt, nvirtEOS)))
+    got = oracle.steps5to8(C, eo, ev, UoE, Uv, Uo, UvE)
+    exp = numpy_steps5to8(C, eo, ev, UoE, Uv, Uo, UvE)
+    for g, e in zip(got, exp):
+        np.testing.assert_allclose(g, e, rtol=1e-12, atol=1e-12)
+
+
+def test_identity_U_reduces_to_canonical(oracle):
+    rng = np.random.default_rng(3)
+    nba, nvirt, nocc = 9, 5, 3
+    C = np.asfortranarray(rng.standard_normal((nba, nvirt, nocc)))
+    eo, ev = rng.uniform(-2, -0.5, nocc), rng.uniform(0.2, 2, nvirt)
+    t = oracle.toccA(C, eo, ev, np.eye(nocc))  # tocc(I,J,A,B)
+    g = oracle.gocc(C)                          # g(A,I,B,J)
+    D = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    np.testing.assert_allclose(t, np.einsum("aibj->ijab", g) / D, rtol=1e-13)
+    # gocc == gvirt when U = I and EOS = all orbitals
+    np.testing.assert_array_equal(oracle.gvirt(C), g)
+    # V^{ij}_{ab} = V^{ji}_{ba}
+    np.testing.assert_allclose(g, np.einsum("aibj->bjai", g), rtol=1e-14)
+    # tvirtA with U = I carries the same amplitudes in (I,J,A,B) order
+    tv = oracle.tvirtA(C, eo, ev, np.eye(nvirt))
+    np.testing.assert_allclose(tv, t, rtol=1e-13)
+
+
+def test_gen4dimfo_einsum(oracle):
+    rng = np.random.default_rng(5)
+    C3 = rng.standard_normal((6, 4, 3))
+    C4 = rng.standard_normal((6, 5, 2))
+    np.testing.assert_allclose(oracle.gen4dimfo(C3, C4), np.einsum("qab,qcd->abcd", C3, C4), rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# pair energies
+# ---------------------------------------------------------------------------
+def brute_energy(B, eo, ev):
+    nocc = B.shape[2]
+    eps = np.zeros((nocc, nocc))
+    for i in range(nocc):
+        for j in range(nocc):
+            V = B[:, :, i].T @ B[:, :, j]
+            D = eo[i] + eo[j] - ev[:, None] - ev[None, :]
+            terms = (V / D) * (2 * V - V.T)
+            eps[i, j] = math.fsum(terms.ravel())
+    return math.fsum(eps.ravel()), eps
+
+
+def rand_frag(rng, naux, nvirt, nocc):
+    B = np.asfortranarray(rng.standard_normal((naux, nvirt, nocc)) * 0.1)
+    return B, rng.uniform(-2, -0.3, nocc), rng.uniform(0.1, 3.0, nvirt)
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 3), (10, 6, 4), (33, 9, 5), (1, 1, 1), (16, 1, 3), (5, 7, 1)])
+def test_pair_energy_brute_force(oracle, shape):
+    rng = np.random.default_rng(sum(shape))
+    B, eo, ev = rand_frag(rng, *shape)
+    E, eps = oracle.pair_energies(B, eo, ev)
+    Eb, epsb = brute_energy(B, eo, ev)
+    np.testing.assert_allclose(eps, epsb, rtol=1e-13, atol=1e-16)
+    assert abs(E - Eb) <= 1e-13 * abs(Eb) + 1e-16
+    np.testing.assert_array_equal(eps, eps.T)
+
+
+def test_pair_energy_single_orbital_closed_form(oracle):
+    B = np.asfortranarray(np.array([0.3, -0.2, 0.5]).reshape(3, 1, 1))
+    eo, ev = np.array([-1.1]), np.array([0.7])
+    V = 0.09 + 0.04 + 0.25
+    E, _ = oracle.pair_energies(B, eo, ev)
+    assert abs(E - V * V / (2 * -1.1 - 2 * 0.7)) < 1e-16
+
+
+def test_pair_energy_invariants(oracle):
+    rng = np.random.default_rng(7)
+    B, eo, ev = rand_frag(rng, 20, 8, 5)
+    E, eps = oracle.pair_energies(B, eo, ev)
+    assert np.all(eps <= 0.0) and E < 0
+    s = 1.7
+    E2, _ = oracle.pair_energies(B * s, eo, ev)
+    assert abs(E2 - s ** 4 * E) <= 1e-12 * abs(E2)
+    po, pv = rng.permutation(5), rng.permutation(8)
+    E3, eps3 = oracle.pair_energies(np.asfortranarray(B[:, pv][:, :, po]), eo[po], ev[pv])
+    assert abs(E3 - E) <= 1e-12 * abs(E)
+    np.testing.assert_allclose(eps3, eps[np.ix_(po, po)], rtol=1e-12)
+    # E = sum_i eps_ii + 2 sum_{i<j} eps_ij
+    assert abs(E - (np.trace(eps) + 2 * np.sum(np.triu(eps, 1)))) <= 1e-12 * abs(E)
+
+
+def test_pair_energy_subset_matches_full(oracle):
+    rng = np.random.default_rng(9)
+    B, eo, ev = rand_frag(rng, 24, 10, 6)
+    _, eps = oracle.pair_energies(B, eo, ev)
+    pi, pj = np.array([0, 2, 5, 1]), np.array([0, 3, 5, 4])
+    np.testing.assert_array_equal(oracle.pair_energies_subset(B, eo, ev, pi, pj), eps[pi, pj])
+
+
+def test_exchange_term_matters(oracle):
+    """A restatement that dropped -V_ba (or used V_ab twice) fails here."""
+    rng = np.random.default_rng(13)
+    B, eo, ev = rand_frag(rng, 12, 6, 3)
+    E, _ = oracle.pair_energies(B, eo, ev)
+    nocc = 3
+    E_no_x = 0.0
+    for i in range(nocc):
+        for j in range(nocc):
+            V = B[:, :, i].T @ B[:, :, j]
+            D = eo[i] + eo[j] - ev[:, None] - ev[None, :]
+            E_no_x += np.sum(2 * V * V / D)
+    assert abs(E - E_no_x) > 1e-3 * abs(E)
+
+
+# ---------------------------------------------------------------------------
+# metric
+# ---------------------------------------------------------------------------
+def spd(rng, n):
+    A = rng.standard_normal((n, n))
+    return np.asfortranarray(A @ A.T / n + np.eye(n))
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 130])
+def test_cholesky_vs_lapack(oracle, n):
+    rng = np.random.default_rng(n)
+    V = spd(rng, n)
+    L = oracle.cholesky(V)
+    np.testing.assert_allclose(L, sl.cholesky(V, lower=True), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(L @ L.T, V, rtol=0, atol=1e-13 * n * np.abs(V).max())
+    assert np.all(np.triu(L, 1) == 0)
+
+
+def test_cholesky_rejects_indefinite(oracle):
+    V = np.asfortranarray(np.array([[1.0, 2.0], [2.0, 1.0]]))
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle.cholesky(V)
+
+
+def test_metric_transform_identities(oracle):
+    rng = np.random.default_rng(17)
+    naux, nvirt, nocc = 40, 6, 3
+    V = spd(rng, naux)
+    X = np.asfortranarray(rng.standard_normal((naux, nvirt, nocc)))
+    B = oracle.metric_transform(V, X)
+    X2, B2 = X.reshape(naux, -1, order="F"), B.reshape(naux, -1, order="F")
+    L = sl.cholesky(V, lower=True)
+    np.testing.assert_allclose(B2, sl.solve_triangular(L, X2, lower=True), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(B2.T @ B2, X2.T @ np.linalg.solve(V, X2), rtol=1e-10, atol=1e-11)
+    np.testing.assert_allclose(oracle.metric_transform(np.asfortranarray(np.eye(naux)), X), X, rtol=0, atol=0)
