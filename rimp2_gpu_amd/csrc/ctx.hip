@@ -117,6 +117,11 @@ static void check_status(Ctx* c) {
 
 void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
 
+// Clears the device status word at the start of a call sequence.
+static void reset_status(Ctx* c) {
+  RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, c->stream));
+}
+
 static void validate_mode(int mode) {
   if (mode < 0 || mode >= RIMP2_NUM_MODES) fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
 }
@@ -223,6 +228,7 @@ int rimp2_metric(void* ctx, int64_t naux, const double* V) {
   check_host_finite(V, naux * naux, "V");
   c->have_metric = false;
   c->have_B = false;
+  reset_status(c);
   double* d = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
   RIMP2_HIP_CHECK(hipMemcpyAsync(d, V, (size_t)(naux * naux) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   metric_factor(c, naux, d);
@@ -292,6 +298,7 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   validate_dims(naux, nvirt, nocc);
   validate_mode(mode);
   if (!d_Pia || !d_V || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
+  reset_status(c);
   check_inputs_device(c, d_V, naux * naux, "V");
   check_inputs_device(c, d_Pia, naux * nvirt * nocc, "Pia");
   check_inputs_device(c, d_eps_o, nocc, "eps_o");

@@ -190,7 +190,8 @@ __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
 }  // namespace
 
 // dV_in (device, naux x naux) -> c->V holds L (lower, zeros above),
-// c->Linv holds L^{-1}.  Sets RIMP2_ERR_NOT_SPD on a non-positive pivot.
+// c->Linv holds L^{-1}.  A non-positive pivot sets bit 2 of the context's
+// status word (cleared by the API entry points, read by check_status).
 void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
   hipStream_t s = c->stream;
   double* L = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
@@ -198,7 +199,6 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
   int* status = static_cast<int*>(c->status.ensure(64));
   if (dV_in != L) RIMP2_HIP_CHECK(hipMemcpyAsync(L, dV_in, (size_t)(n * n) * sizeof(double), hipMemcpyDeviceToDevice, s));
   RIMP2_HIP_CHECK(hipMemsetAsync(Li, 0, (size_t)(n * n) * sizeof(double), s));
-  RIMP2_HIP_CHECK(hipMemsetAsync(status, 0, 64, s));
   {
     ScopedTimer t(c, RIMP2_K_CHOLESKY);
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
