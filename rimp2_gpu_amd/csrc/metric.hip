@@ -26,17 +26,24 @@ constexpr int NB = 64;  // Cholesky / inverse panel width
 // 256 threads = 4 waves in 2x2, wave tile (BM/2) x (BN/2) of 16x16 MFMA blocks.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
 // result d[r] = D[4r + l/16][l%16].
+// Pipelining: the next K tile is fetched into registers (16-byte loads where
+// the leading dimensions allow) while the MFMAs consume the current one.
+// LDS: As[k][m] with pitch BM+16 doubles and Bs[n][k] with pitch 17 doubles,
+// so both fragment reads are free of bank conflicts.
 // ---------------------------------------------------------------------------
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
                                                        const double* A, int64_t lda, const double* B, int64_t ldb,
                                                        double beta, double* C, int64_t ldc) {
   constexpr int BK = 16;
-  constexpr int PA = BM + 4, PB = BN + 4;  // LDS pitches (doubles)
+  constexpr int PA = BM + 16;  // As[k][m]
+  constexpr int PB = BK + 1;   // Bs[n][k]
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int NA = BM * BK / 512;  // double2 loads of A per thread per K tile
+  constexpr int NBL = BN * BK / 512; // double2 loads of B per thread per K tile
   __shared__ double As[BK * PA];
-  __shared__ double Bs[BK * PB];
+  __shared__ double Bs[BN * PB];
 
   const int64_t m0 = (int64_t)blockIdx.x * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
@@ -44,6 +51,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+  const bool a_vec = ((lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  const bool b_vec = ((ldb & 1) == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0);
 
   v4d acc[FM][FN];
 #pragma unroll
@@ -54,35 +63,96 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
   int64_t kend = K;
   if (TRI_A) kend = (m0 + BM < K) ? (m0 + BM) : K;
 
+  double2 ra[NA], rb[NBL];
+  // A tile BM x BK: element pair (m, m+1) of column k; BM/2 pairs per column
+  auto load_a = [&](int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < NA; t++) {
+      const int e = tid + 256 * t;
+      const int k = e / (BM / 2), m = (e % (BM / 2)) * 2;
+      const int64_t gm = m0 + m, gk = k0 + k;
+      double2 v = {0.0, 0.0};
+      if (gk < K) {
+        if (a_vec && gm + 1 < M) v = *reinterpret_cast<const double2*>(A + gm + lda * gk);
+        else {
+          if (gm < M) v.x = A[gm + lda * gk];
+          if (gm + 1 < M) v.y = A[gm + 1 + lda * gk];
+        }
+      }
+      ra[t] = v;
+    }
+  };
+  // op(B) tile BK x BN
+  auto load_b = [&](int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < NBL; t++) {
+      const int e = tid + 256 * t;
+      double2 v = {0.0, 0.0};
+      if (TRANSB) {  // pairs along n: (n, n+1) of row k
+        const int k = e / (BN / 2), n = (e % (BN / 2)) * 2;
+        const int64_t gn = n0 + n, gk = k0 + k;
+        if (gk < K) {
+          if (b_vec && gn + 1 < N) v = *reinterpret_cast<const double2*>(B + gn + ldb * gk);
+          else {
+            if (gn < N) v.x = B[gn + ldb * gk];
+            if (gn + 1 < N) v.y = B[gn + 1 + ldb * gk];
+          }
+        }
+      } else {  // pairs along k: (k, k+1) of column n
+        const int n = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        const int64_t gn = n0 + n, gk = k0 + k;
+        if (gn < N) {
+          if (b_vec && gk + 1 < K) v = *reinterpret_cast<const double2*>(B + gk + ldb * gn);
+          else {
+            if (gk < K) v.x = B[gk + ldb * gn];
+            if (gk + 1 < K) v.y = B[gk + 1 + ldb * gn];
+          }
+        }
+      }
+      rb[t] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int t = 0; t < NA; t++) {
+      const int e = tid + 256 * t;
+      const int k = e / (BM / 2), m = (e % (BM / 2)) * 2;
+      As[k * PA + m] = ra[t].x;
+      As[k * PA + m + 1] = ra[t].y;
+    }
+#pragma unroll
+    for (int t = 0; t < NBL; t++) {
+      const int e = tid + 256 * t;
+      if (TRANSB) {
+        const int k = e / (BN / 2), n = (e % (BN / 2)) * 2;
+        Bs[n * PB + k] = rb[t].x;
+        Bs[(n + 1) * PB + k] = rb[t].y;
+      } else {
+        const int n = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        Bs[n * PB + k] = rb[t].x;
+        Bs[n * PB + k + 1] = rb[t].y;
+      }
+    }
+  };
+
+  if (kend > 0) {
+    load_a(0);
+    load_b(0);
+  }
   for (int64_t k0 = 0; k0 < kend; k0 += BK) {
-    // A tile: BM x BK, column-major source -> As[k][m]
-    for (int e = tid; e < BM * BK; e += 256) {
-      int k = e / BM, m = e % BM;
-      int64_t gm = m0 + m, gk = k0 + k;
-      As[k * PA + m] = (gm < M && gk < K) ? A[gm + lda * gk] : 0.0;
-    }
-    // op(B) tile: BK x BN -> Bs[k][n]
-    if (TRANSB) {
-      for (int e = tid; e < BK * BN; e += 256) {
-        int k = e / BN, n = e % BN;
-        int64_t gn = n0 + n, gk = k0 + k;
-        Bs[k * PB + n] = (gn < N && gk < K) ? B[gn + ldb * gk] : 0.0;
-      }
-    } else {
-      for (int e = tid; e < BK * BN; e += 256) {
-        int k = e % BK, n = e / BK;
-        int64_t gn = n0 + n, gk = k0 + k;
-        Bs[k * PB + n] = (gn < N && gk < K) ? B[gk + ldb * gn] : 0.0;
-      }
-    }
+    store();
     __syncthreads();
+    if (k0 + BK < kend) {
+      load_a(k0 + BK);
+      load_b(k0 + BK);
+    }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       double af[FM], bf[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) af[i] = As[(ks + lane / 16) * PA + wm + 16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < FN; j++) bf[j] = Bs[(ks + lane / 16) * PB + wn + 16 * j + (lane & 15)];
+      for (int j = 0; j < FN; j++) bf[j] = Bs[(wn + 16 * j + (lane & 15)) * PB + ks + lane / 16];
 #pragma unroll
       for (int i = 0; i < FM; i++)
 #pragma unroll
@@ -150,29 +220,34 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda,
     const double inv = 1.0 / L[c][c];
     for (int r = c + 1 + tid; r < kb; r += 256) L[r][c] *= inv;
     __syncthreads();
-    // trailing update of the lower triangle
-    const int nt = kb - c - 1;
-    for (int e = tid; e < nt * nt; e += 256) {
-      int r = c + 1 + e % nt, q = c + 1 + e / nt;
-      if (q <= r) L[r][q] -= L[r][c] * L[q][c];
+    // trailing update of the lower triangle: thread -> row tid%64, columns
+    // tid/64 + 4t (no integer division in the loop)
+    {
+      const int r = tid & 63;
+      if (r > c && r < kb) {
+        const double lrc = L[r][c];
+        for (int q = c + 1 + ((tid >> 6) - (c + 1) % 4 + 4) % 4; q <= r; q += 4) L[r][q] -= lrc * L[q][c];
+      }
     }
     __syncthreads();
   }
-  // L11^{-1}: column j solved by thread j (forward substitution)
-  if (tid < kb) {
-    const int j = tid;
-    for (int r = 0; r < kb; r++) {
-      double v;
-      if (r < j) v = 0.0;
-      else {
-        double s = (r == j) ? 1.0 : 0.0;
-        for (int m = j; m < r; m++) s -= L[r][m] * X[m][j];
-        v = s / L[r][r];
-      }
-      X[r][j] = v;
-    }
-  }
+  // L11^{-1} by a right-looking sweep over L11 X = I: row c of X is final
+  // after the division by L[c][c]; it is then eliminated from the rows below.
+  for (int e = tid; e < NB * NB; e += 256) X[e % NB][e / NB] = (e % NB == e / NB) ? 1.0 : 0.0;
   __syncthreads();
+  for (int c = 0; c < kb; c++) {
+    const double inv = 1.0 / L[c][c];
+    for (int j = tid; j <= c; j += 256) X[c][j] *= inv;
+    __syncthreads();
+    {
+      const int r = tid & 63;
+      if (r > c && r < kb) {
+        const double lrc = L[r][c];
+        for (int j = tid >> 6; j <= c; j += 4) X[r][j] -= lrc * X[c][j];
+      }
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < kb * kb; e += 256) {
     int r = e % kb, c = e / kb;
     A[(k0 + r) + lda * (k0 + c)] = (c <= r) ? L[r][c] : 0.0;

@@ -30,13 +30,14 @@ int mode_slices(int mode) {
   }
 }
 
-// Auxiliary padding quantum per mode: one LDS stage of the pair kernel.
+// Auxiliary padding quantum per mode: three LDS stages of the pair kernel
+// (its operand ring has three buffers and a break-free three-step body).
 static int64_t q_quantum(int mode) {
   switch (mode) {
-    case RIMP2_MODE_FP64: return 16;
-    case RIMP2_MODE_FP32: return 16;
-    case RIMP2_MODE_BF16X3: return 32;
-    default: return 32;
+    case RIMP2_MODE_FP64: return 3 * 16;
+    case RIMP2_MODE_FP32: return 3 * 16;
+    case RIMP2_MODE_BF16X3: return 3 * 16;
+    default: return 3 * 32;
   }
 }
 

@@ -38,7 +38,7 @@ struct PairParams {
 
 template <class Pol, int TM>
 __global__ __launch_bounds__(256) void pair_kernel(PairParams p) {
-  extern __shared__ __align__(16) unsigned char lds[];
+  RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   // XCD-aware mapping: consecutive workgroups land on different XCDs
   // (round-robin dispatch), so give each XCD slot a contiguous run of work
   // units: neighbouring units share an occupied pair and hit the same L2.
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void pair_kernel(PairParams p) {
   int64_t i, j, A, Bk;
   tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
   tri_decode(tp, A, Bk);  // A <= Bk
-  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, lds);
+  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2);
   if (threadIdx.x == 0) p.partials[wu] = e;
 }
 
@@ -125,15 +125,8 @@ __global__ void denom_check_kernel(const double* eo, int64_t nocc, const double*
 
 template <class Pol, int TM>
 void launch_pair(hipStream_t s, const PairParams& p) {
-  const size_t lds = pair_lds_bytes<Pol, TM>();
-  static bool attr_set = false;
-  if (!attr_set) {
-    RIMP2_HIP_CHECK(hipFuncSetAttribute((const void*)pair_kernel<Pol, TM>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
   const unsigned grid = (unsigned)(8 * p.chunk);
-  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3(grid), dim3(256), lds, s, p);
+  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3(grid), dim3(256), 0, s, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

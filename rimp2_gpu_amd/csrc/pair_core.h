@@ -142,24 +142,44 @@ struct FragView {
   int64_t nvp, nks, nrb;
 };
 
-// Bytes of dynamic LDS the tile function needs.
+// Bytes of LDS one stage of the operand pipeline occupies (4 operand slabs).
 template <class Pol, int TM>
-constexpr size_t pair_lds_bytes() {
-  return 4 * (size_t)Pol::KSTEPS * (TM / Pol::RB) * Pol::P * 64 * Pol::LB;
+constexpr int pair_stage_bytes() {
+  return 4 * Pol::KSTEPS * (TM / Pol::RB) * Pol::P * 64 * Pol::LB;
 }
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// s_waitcnt on the vector-memory counter only (expcnt / lgkmcnt left at their
+// maxima): returns once at most n of this wave's vector loads are in flight.
+#define RIMP2_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14)) | 0x0F70)
 
 // One work unit: occupied pair (i <= j), virtual blocks A <= Bk.  All 256
 // threads call it; the return value (the unit's energy partial) is valid in
-// thread 0.  Ends with a barrier, so `lds` may be reused at once.
+// thread 0.  Ends with a barrier, so the stage buffers may be reused at once.
+//
+// Operand pipeline: three LDS stage buffers; the slabs of stages s+1 and s+2
+// are in flight HBM/L2 -> LDS (global_load_lds_dwordx4, each wave instruction
+// moves 1 KB, no staging registers) while the MFMAs consume stage s.  The
+// buffers are separate LDS objects so the compiler can see that the DMA
+// target never aliases the buffer being read.  Because the packed
+// layout stores every fragment in lane order, the DMA destination of a wave is
+// one contiguous KB and the MFMA operand reads (ds_read of 64 x lane_bytes)
+// are contiguous and bank-conflict free.
 template <class Pol, int TM>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk,
-                            unsigned char* __restrict__ lds) {
+                            unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
+                            unsigned char* __restrict__ buf2) {
   constexpr int FPB = TM / Pol::RB;           // fragments per block side
   constexpr int WT = TM / 2;                  // wave tile side
   constexpr int FW = WT / Pol::RB;            // fragments per wave side
   constexpr int FB = 64 * Pol::LB;            // bytes per fragment
-  constexpr int OPB = Pol::KSTEPS * FPB * Pol::P * FB;  // bytes per operand per stage
+  constexpr int CHUNK = FPB * Pol::P * FB;    // contiguous bytes per operand per k-step
+  constexpr int OPB = Pol::KSTEPS * CHUNK;    // bytes per operand per stage
+  constexpr int SB = 4 * OPB;                 // bytes per stage
+  constexpr int NI = SB / 4096;               // 1-KB DMA instructions per wave per stage
   static_assert(FW >= 1, "wave tile smaller than one MFMA fragment");
+  static_assert(SB % 4096 == 0 && OPB % 1024 == 0, "stage must split into whole 1-KB wave transfers");
   const bool same_occ = (i == j), same_blk = (A == Bk);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -179,28 +199,31 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   const size_t per_ks = (size_t)f.nrb * Pol::P * FB;
   const size_t per_occ = (size_t)f.nks * per_ks;
   const unsigned char* src[4];
-  src[0] = f.packed + i * per_occ + (size_t)A * FPB * Pol::P * FB;
-  src[1] = f.packed + j * per_occ + (size_t)Bk * FPB * Pol::P * FB;
-  src[2] = f.packed + j * per_occ + (size_t)A * FPB * Pol::P * FB;
-  src[3] = f.packed + i * per_occ + (size_t)Bk * FPB * Pol::P * FB;
-  const int nops = same_occ ? 2 : 4;
-  constexpr int CHUNK = FPB * Pol::P * FB;  // contiguous bytes per operand per k-step
+  src[0] = f.packed + i * per_occ + (size_t)A * CHUNK;
+  src[1] = f.packed + j * per_occ + (size_t)Bk * CHUNK;
+  src[2] = f.packed + j * per_occ + (size_t)A * CHUNK;
+  src[3] = f.packed + i * per_occ + (size_t)Bk * CHUNK;
+  const int64_t nst = f.nks / Pol::KSTEPS;
 
-  for (int64_t ks0 = 0; ks0 < f.nks; ks0 += Pol::KSTEPS) {
-    // global -> LDS, 16 bytes per thread per iteration
-    for (int op = 0; op < nops; op++) {
-      const unsigned char* s = src[op] + (size_t)ks0 * per_ks;
-      unsigned char* d = lds + op * OPB;
-      for (int off = tid * 16; off < OPB; off += 256 * 16) {
-        const int ks = off / CHUNK, r = off % CHUNK;
-        *reinterpret_cast<uint4*>(d + off) = *reinterpret_cast<const uint4*>(s + (size_t)ks * per_ks + r);
-      }
+  // stage st -> LDS buffer dst (wave-uniform destinations, per-lane sources).
+  // Every wave issues exactly NI transfers per stage (all four slabs, also for
+  // i == j) so that the wait counts below are compile-time constants.
+  auto issue = [&](int64_t st, unsigned char* dst) {
+#pragma unroll
+    for (int n = 0; n < NI; n++) {
+      const int wb = (n * 4 + wave) * 1024;
+      const int op = wb / OPB;
+      const int rem = wb - op * OPB + lane * 16;
+      const int ks = rem / CHUNK, r = rem % CHUNK;
+      const unsigned char* g = src[op] + (size_t)(st * Pol::KSTEPS + ks) * per_ks + r;
+      __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)(dst + wb), 16, 0, 0);
     }
-    __syncthreads();
+  };
+  auto compute = [&](const unsigned char* stage) {
 #pragma unroll
     for (int ks = 0; ks < Pol::KSTEPS; ks++) {
       typename Pol::frag_t ra[FW][Pol::P], cb[FW][Pol::P];
-      const unsigned char* base = lds + ks * CHUNK;
+      const unsigned char* base = stage + ks * CHUNK;
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
@@ -230,7 +253,26 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int y = 0; y < FW; y++) Pol::mma(acc2[x][y], ra[x], cb[y]);
       }
     }
+  };
+
+  // step(st): wait for stage st (stage st+1 may stay in flight), barrier
+  // (stage st visible to all waves; the buffer of stage st-1 is free), issue
+  // stage st+2 into that buffer, consume stage st.
+  auto step = [&](int64_t st, const unsigned char* cur, unsigned char* next2) {
+    if (st + 1 < nst) RIMP2_WAIT_VMCNT(NI);
+    else RIMP2_WAIT_VMCNT(0);
     __syncthreads();
+    if (st + 2 < nst) issue(st + 2, next2);
+    compute(cur);
+  };
+  issue(0, buf0);
+  if (nst > 1) issue(1, buf1);
+  // nst is a multiple of 3 (the auxiliary padding guarantees it), so the
+  // three-step body needs no early exits.
+  for (int64_t st = 0; st < nst; st += 3) {
+    step(st, buf0, buf2);
+    step(st + 1, buf1, buf0);
+    step(st + 2, buf2, buf1);
   }
 
   // ---- epilogue: energy terms in FP64 ----
@@ -260,13 +302,22 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   double tot = s - c;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-  __shared__ double wsum[4];
+  // the wave sums go through buf0 once every wave is done with the stages
+  // (keeps the workgroup's LDS at exactly the stage buffers)
+  __syncthreads();
+  double* wsum = reinterpret_cast<double*>(buf0);
   if (lane == 0) wsum[wave] = tot;
   __syncthreads();
   const double res = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
   __syncthreads();
   return res;
 }
+
+// Declares the three stage buffers of a pair kernel instantiation.
+#define RIMP2_PAIR_STAGE_BUFFERS(Pol, TM)                                            \
+  __shared__ __align__(16) unsigned char rimp2_buf0[pair_stage_bytes<Pol, TM>()];  \
+  __shared__ __align__(16) unsigned char rimp2_buf1[pair_stage_bytes<Pol, TM>()];  \
+  __shared__ __align__(16) unsigned char rimp2_buf2[pair_stage_bytes<Pol, TM>()]
 
 }  // namespace pairk
 }  // namespace rimp2
