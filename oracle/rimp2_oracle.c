@@ -393,17 +393,33 @@ int oracle_cholesky(int64_t n, const double* V, double* L) {
   return 0;
 }
 
-/* B(:,c) = L^{-1} X(:,c) for each of ncols columns (forward substitution). */
+/* B(:,c) = L^{-1} X(:,c) for each of ncols columns (forward substitution,
+ * column-oriented: L is read down its columns, 16 right-hand sides at once). */
 void oracle_trsm_lower(int64_t n, int64_t ncols, const double* L, const double* X, double* B) {
-#pragma omp parallel for schedule(static)
-  for (int64_t c = 0; c < ncols; c++) {
-    const double* x = X + n * c;
-    double* b = B + n * c;
-    for (int64_t i = 0; i < n; i++) {
-      double s = x[i];
-      for (int64_t k = 0; k < i; k++) s -= L[i + n * k] * b[k];
-      b[i] = s / L[i + n * i];
+  const int64_t NC = 16;
+#pragma omp parallel
+  {
+    double* w = (double*)malloc(sizeof(double) * (size_t)(n * NC));
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t c0 = 0; c0 < ncols; c0 += NC) {
+      const int64_t nc = (ncols - c0 < NC) ? ncols - c0 : NC;
+      for (int64_t i = 0; i < n; i++)
+        for (int64_t c = 0; c < NC; c++) w[i * NC + c] = (c < nc) ? X[i + n * (c0 + c)] : 0.0;
+      for (int64_t k = 0; k < n; k++) {
+        const double* lk = L + n * k;
+        const double d = lk[k];
+        double* wk = w + k * NC;
+        for (int64_t c = 0; c < NC; c++) wk[c] /= d;
+        for (int64_t i = k + 1; i < n; i++) {
+          const double l = lk[i];
+          double* wi = w + i * NC;
+          for (int64_t c = 0; c < NC; c++) wi[c] -= l * wk[c];
+        }
+      }
+      for (int64_t c = 0; c < nc; c++)
+        for (int64_t i = 0; i < n; i++) B[i + n * (c0 + c)] = w[i * NC + c];
     }
+    free(w);
   }
 }
 

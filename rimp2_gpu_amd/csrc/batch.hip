@@ -1,22 +1,507 @@
-// Batched DEC fragments (filled in below the pair kernel work).
-#include "rimp2_common.h"
+// Batched DEC fragments: many fragments of uneven (nocc, nvirt, naux) through
+// ONE persistent pair launch captured in a hipGraph.
+//
+// The reference processes its fragments one after another (`do ifile=1,2`,
+// RIMP2.F90:36) with two OpenACC queues per fragment (RIMP2.GPU.F90:64-78).
+// Here every fragment is factored, transformed and packed once
+// (rimp2_batch_prepare); the pair work of all fragments is then one global
+// work list: work unit = (fragment, occupied pair i <= j, virtual tile pair
+// A <= B).  Fragments are ordered by the cost of their work units (longest
+// auxiliary loop first).  The list is cut into 8 contiguous ranges of equal
+// cost, one per XCD (workgroup b runs on XCD b % 8, so neighbouring units,
+// which share a fragment's operands, share that XCD's L2); inside its range a
+// persistent workgroup takes every (G/8)-th unit, largest first.  The loop
+// bounds are kernel arguments, so every wave leaves the loop together.  (A
+// per-workgroup atomic queue was not used: its divergent "thread 0 takes the
+// next unit" branch lets the compiler rotate the loop so that the other lanes
+// re-enter the barrier without the atomic.)  A deterministic reduction
+// then sums each fragment's partials per pair (Neumaier) and the pairs per
+// fragment (fixed tree order).
+//
+// Autotuner: the pair tile (virtual block width) is chosen per (mode, size
+// class of the batch) by timing the persistent launch for each candidate tile
+// on the batch itself; the winner is cached in the context.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+#include "pair_core.h"
 
 namespace rimp2 {
-struct BatchState {};
+
+struct BatchFrag {
+  pairk::FragView f;
+  int64_t nocc, nb, ntp, npairs;
+  int64_t wu_begin;  // first global work unit (launch order)
+  int64_t pair_off;  // offset of this fragment's [nocc, nocc] pair matrix
+  int64_t orig;      // index of the fragment in the caller's list
+};
+
+struct BatchState {
+  int mode = -1;
+  int tile = 0;
+  int64_t nfrag = 0, total_wu = 0, total_pairs_sq = 0;
+  std::vector<int64_t> nocc, nvirt, naux;
+  std::vector<size_t> b64_off;  // FP64 B of each fragment in the b64 arena (doubles)
+  DevBuf b64, packed, scales, eps_o, eps_v, partials, pairs, e_frag, frags;
+  int64_t range[9] = {0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ready = false;
+  std::map<std::pair<int, int>, int> tuned;  // (mode, size class) -> tile
+  double tune_ms[4] = {0, 0, 0, 0};
+  int tune_tiles[4] = {0, 0, 0, 0};
+  int ntune = 0;
+  ~BatchState() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags})
+      b->release();
+  }
+};
+
 void batch_destroy(BatchState* b) { delete b; }
+
+namespace {
+
+using namespace pairk;
+
+struct BatchParams {
+  const BatchFrag* frags;
+  int64_t nfrag, total;
+  int64_t range[9];  // XCD x owns work units [range[x], range[x+1])
+  double* partials;
+};
+
+template <class Pol, int TM>
+__global__ __launch_bounds__(256) void batch_pair_kernel(BatchParams p) {
+  RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+  const int64_t end = p.range[xcd + 1];
+  for (int64_t wu = p.range[xcd] + slot; wu < end; wu += nslot) {
+    // fragment holding wu: last fragment with wu_begin <= wu
+    int64_t lo = 0, hi = p.nfrag - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (p.frags[mid].wu_begin <= wu) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchFrag& fr = p.frags[lo];
+    const int64_t local = wu - fr.wu_begin;
+    const int64_t pidx = local / fr.ntp, tp = local % fr.ntp;
+    int64_t i, j, A, Bk;
+    tri_decode(pidx, i, j);
+    tri_decode(tp, A, Bk);
+    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2);
+    if (threadIdx.x == 0) p.partials[wu] = e;
+  }
+}
+
+// One workgroup per fragment: pair sums (Neumaier over its tile partials, in
+// order), the fragment's [nocc,nocc] pair matrix and its total energy.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchFrag* frags, const double* partials,
+                                                           double* pairs, double* e_frag) {
+  const BatchFrag& fr = frags[blockIdx.x];
+  __shared__ double ss[256], cc[256];
+  double s = 0.0, c = 0.0;
+  for (int64_t p = threadIdx.x; p < fr.npairs; p += 256) {
+    const double* q = partials + fr.wu_begin + p * fr.ntp;
+    double ps = 0.0, pc = 0.0;
+    for (int64_t t = 0; t < fr.ntp; t++) {
+      double x = q[t], u = ps + x;
+      pc += (fabs(ps) >= fabs(x)) ? (ps - u) + x : (x - u) + ps;
+      ps = u;
+    }
+    const double e = ps + pc;
+    int64_t i, j;
+    tri_decode(p, i, j);
+    pairs[fr.pair_off + i + fr.nocc * j] = e;
+    pairs[fr.pair_off + j + fr.nocc * i] = e;
+    const double x = (i == j) ? e : 2.0 * e;
+    const double u = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - u) + x : (x - u) + s;
+    s = u;
+  }
+  ss[threadIdx.x] = s;
+  cc[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double a = ss[threadIdx.x], b = ss[threadIdx.x + w], u = a + b;
+      double e = (fabs(a) >= fabs(b)) ? (a - u) + b : (b - u) + a;
+      ss[threadIdx.x] = u;
+      cc[threadIdx.x] += cc[threadIdx.x + w] + e;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) e_frag[fr.orig] = ss[0] + cc[0];
+}
+
+__global__ void pad_eps_kernel(const double* ev, int64_t nvirt, int64_t nvp, double* out) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < nvp) out[a] = (a < nvirt) ? ev[a] : 1.0;
+}
+
+template <class Pol, int TM>
+int occupancy_of() {
+  int blocks = 0;
+  RIMP2_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_pair_kernel<Pol, TM>, 256, 0));
+  return blocks < 1 ? 1 : blocks;
+}
+
+template <class Pol, int TM>
+void launch_batch_t(hipStream_t s, const BatchParams& p, int ncu) {
+  const int grid = 8 * ((ncu * occupancy_of<Pol, TM>() + 7) / 8);  // whole XCD slots
+  hipLaunchKernelGGL((batch_pair_kernel<Pol, TM>), dim3(grid), dim3(256), 0, s, p);
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch(int mode, int tile, hipStream_t s, const BatchParams& p, int ncu) {
+  switch (mode) {
+    case RIMP2_MODE_FP64:
+      if (tile == 32) return launch_batch_t<PolF64, 32>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolF64, 64>(s, p, ncu);
+      break;
+    case RIMP2_MODE_FP32:
+      if (tile == 64) return launch_batch_t<PolF32, 64>(s, p, ncu);
+      if (tile == 128) return launch_batch_t<PolF32, 128>(s, p, ncu);
+      break;
+    case RIMP2_MODE_BF16X3:
+      if (tile == 64) return launch_batch_t<PolBF16x3, 64>(s, p, ncu);
+      if (tile == 128) return launch_batch_t<PolBF16x3, 128>(s, p, ncu);
+      break;
+    case RIMP2_MODE_I8S4:
+      if (tile == 64) return launch_batch_t<PolI8<4, 4>, 64>(s, p, ncu);
+      break;
+    case RIMP2_MODE_I8S5:
+      if (tile == 64) return launch_batch_t<PolI8<5, 4>, 64>(s, p, ncu);
+      break;
+  }
+  fail(RIMP2_ERR_ARG, "no batch pair kernel for mode %d tile %d", mode, tile);
+}
+
+std::vector<int> tile_candidates(int mode) {
+  switch (mode) {
+    case RIMP2_MODE_FP64: return {32, 64};
+    case RIMP2_MODE_FP32:
+    case RIMP2_MODE_BF16X3: return {64, 128};
+    default: return {64};
+  }
+}
+
+int num_cus(int device) {
+  int n = 0;
+  RIMP2_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device));
+  return n > 0 ? n : 1;
+}
+
+// Packs every fragment's FP64 B (from the b64 arena) with pair tile `tile`,
+// builds the work-unit table in cost order and uploads it.
+void layout_batch(Ctx* c, BatchState* b, int tile) {
+  hipStream_t s = c->stream;
+  const int64_t nf = b->nfrag;
+  std::vector<PackGeom> g(nf);
+  size_t packed_bytes = 0, scale_n = 0, evp_n = 0;
+  std::vector<size_t> poff(nf), soff(nf), eoff(nf);
+  for (int64_t f = 0; f < nf; f++) {
+    g[f] = make_geom(b->mode, b->naux[f], b->nvirt[f], tile);
+    poff[f] = packed_bytes;
+    packed_bytes += round_up((int64_t)(g[f].bytes_per_occ() * b->nocc[f]), 256);
+    soff[f] = scale_n;
+    scale_n += (size_t)(g[f].nvp * b->nocc[f]);
+    eoff[f] = evp_n;
+    evp_n += (size_t)g[f].nvp;
+  }
+  unsigned char* pk = static_cast<unsigned char*>(b->packed.ensure(packed_bytes));
+  double* sc = mode_slices(b->mode) ? static_cast<double*>(b->scales.ensure(scale_n * sizeof(double))) : nullptr;
+  double* evp = static_cast<double*>(b->eps_v.ensure(evp_n * sizeof(double)));
+  // raw eps_v of fragment f sits in eps_o arena after all occupied energies
+  size_t occ_total = 0;
+  for (int64_t f = 0; f < nf; f++) occ_total += (size_t)b->nocc[f];
+  const double* eo = b->eps_o.as<double>();
+  const double* ev_raw = eo + occ_total;
+  std::vector<size_t> ooff(nf), voff(nf);
+  {
+    size_t o = 0, v = 0;
+    for (int64_t f = 0; f < nf; f++) {
+      ooff[f] = o;
+      voff[f] = v;
+      o += (size_t)b->nocc[f];
+      v += (size_t)b->nvirt[f];
+    }
+  }
+  for (int64_t f = 0; f < nf; f++) {
+    pack_B(c, b->b64.as<double>() + b->b64_off[f], b->naux[f], b->nvirt[f], b->nocc[f], g[f], pk + poff[f],
+           sc ? sc + soff[f] : nullptr, s);
+    hipLaunchKernelGGL(pad_eps_kernel, dim3((unsigned)ceil_div(g[f].nvp, 256)), dim3(256), 0, s, ev_raw + voff[f],
+                       b->nvirt[f], g[f].nvp, evp + eoff[f]);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  // launch order: most expensive work units (longest auxiliary loop) first
+  std::vector<int64_t> order(nf);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return g[x].nqp > g[y].nqp; });
+  std::vector<BatchFrag> fr(nf);
+  int64_t wu = 0, psq = 0;
+  std::vector<int64_t> pair_off(nf);
+  for (int64_t f = 0; f < nf; f++) {
+    pair_off[f] = psq;
+    psq += b->nocc[f] * b->nocc[f];
+  }
+  for (int64_t k = 0; k < nf; k++) {
+    const int64_t f = order[k];
+    BatchFrag& x = fr[k];
+    x.f.packed = pk + poff[f];
+    x.f.sigma = sc ? sc + soff[f] : nullptr;
+    x.f.eps_o = eo + ooff[f];
+    x.f.eps_v = evp + eoff[f];
+    x.f.nvp = g[f].nvp;
+    x.f.nks = g[f].nks;
+    x.f.nrb = g[f].nrb;
+    x.nocc = b->nocc[f];
+    x.nb = g[f].nvp / tile;
+    x.ntp = x.nb * (x.nb + 1) / 2;
+    x.npairs = x.nocc * (x.nocc + 1) / 2;
+    x.wu_begin = wu;
+    x.pair_off = pair_off[f];
+    x.orig = f;
+    wu += x.npairs * x.ntp;
+  }
+  b->total_wu = wu;
+  b->total_pairs_sq = psq;
+  BatchFrag* dfr = static_cast<BatchFrag*>(b->frags.ensure(sizeof(BatchFrag) * nf));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dfr, fr.data(), sizeof(BatchFrag) * nf, hipMemcpyHostToDevice, s));
+  b->partials.ensure((size_t)wu * sizeof(double));
+  b->pairs.ensure((size_t)psq * sizeof(double));
+  b->e_frag.ensure((size_t)nf * sizeof(double));
+  // 8 contiguous ranges of equal cost (a unit costs ~ its auxiliary stages)
+  {
+    std::vector<double> cstart(nf + 1, 0.0);
+    for (int64_t k = 0; k < nf; k++)
+      cstart[k + 1] = cstart[k] + (double)(fr[k].npairs * fr[k].ntp) * (double)fr[k].f.nks;
+    b->range[0] = 0;
+    int64_t k = 0;
+    for (int x = 1; x < 8; x++) {
+      const double target = cstart[nf] * x / 8.0;
+      while (k + 1 < nf && cstart[k + 1] <= target) k++;
+      const int64_t n = fr[k].npairs * fr[k].ntp;
+      int64_t u = (int64_t)std::ceil((target - cstart[k]) / (double)fr[k].f.nks);
+      u = std::min<int64_t>(std::max<int64_t>(u, 0), n);
+      b->range[x] = std::max<int64_t>(fr[k].wu_begin + u, b->range[x - 1]);
+    }
+    b->range[8] = wu;
+  }
+  b->tile = tile;
+  RIMP2_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void enqueue_pairs(Ctx* c, BatchState* b) {
+  hipStream_t s = c->stream;
+  BatchParams p;
+  p.frags = b->frags.as<BatchFrag>();
+  p.nfrag = b->nfrag;
+  p.total = b->total_wu;
+  for (int x = 0; x < 9; x++) p.range[x] = b->range[x];
+  p.partials = b->partials.as<double>();
+  launch_batch(b->mode, b->tile, s, p, num_cus(c->device));
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3((unsigned)b->nfrag), dim3(256), 0, s, p.frags, p.partials,
+                     b->pairs.as<double>(), b->e_frag.as<double>());
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void capture_graph(Ctx* c, BatchState* b) {
+  if (b->exec) RIMP2_HIP_CHECK(hipGraphExecDestroy(b->exec));
+  if (b->graph) RIMP2_HIP_CHECK(hipGraphDestroy(b->graph));
+  b->exec = nullptr;
+  b->graph = nullptr;
+  RIMP2_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_pairs(c, b);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  RIMP2_HIP_CHECK(hipStreamEndCapture(c->stream, &b->graph));
+  RIMP2_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+double time_once(Ctx* c, BatchState* b) {
+  hipEvent_t e0, e1;
+  RIMP2_HIP_CHECK(hipEventCreate(&e0));
+  RIMP2_HIP_CHECK(hipEventCreate(&e1));
+  RIMP2_HIP_CHECK(hipEventRecord(e0, c->stream));
+  enqueue_pairs(c, b);
+  RIMP2_HIP_CHECK(hipEventRecord(e1, c->stream));
+  RIMP2_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  RIMP2_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+// size class of a batch: log2 of the mean virtual dimension
+int size_class(const BatchState* b) {
+  double m = 0;
+  for (int64_t v : b->nvirt) m += (double)v;
+  m /= (double)(b->nfrag > 0 ? b->nfrag : 1);
+  int k = 0;
+  while ((1 << (k + 1)) <= (int)m) k++;
+  return k;
+}
+
+void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode) {
+  if (nfrag <= 0 || !frags) fail(RIMP2_ERR_ARG, "empty fragment list");
+  if (mode < 0 || mode >= RIMP2_NUM_MODES) fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
+  if (!c->batch) c->batch = new BatchState();
+  BatchState* b = c->batch;
+  b->ready = false;
+  b->mode = mode;
+  b->nfrag = nfrag;
+  b->nocc.resize(nfrag);
+  b->nvirt.resize(nfrag);
+  b->naux.resize(nfrag);
+  b->b64_off.resize(nfrag);
+  size_t b64n = 0, occ_total = 0, virt_total = 0;
+  for (int64_t f = 0; f < nfrag; f++) {
+    const rimp2_frag_desc& d = frags[f];
+    if (d.nocc <= 0 || d.nvirt <= 0 || d.naux <= 0 || d.nocc > 65536 || d.nvirt > 65536 || d.naux > 65536)
+      fail(RIMP2_ERR_ARG, "fragment %lld: bad dimensions", (long long)f);
+    if (!d.Pia || !d.V || !d.eps_o || !d.eps_v) fail(RIMP2_ERR_ARG, "fragment %lld: null array", (long long)f);
+    if (mode_slices(mode) && round_up(d.naux, 96) > 16384)
+      fail(RIMP2_ERR_ARG, "fragment %lld: exact-slice modes support naux <= 16384", (long long)f);
+    double mo = -INFINITY, mv = INFINITY;
+    for (int64_t i = 0; i < d.nocc; i++) mo = fmax(mo, d.eps_o[i]);
+    for (int64_t a = 0; a < d.nvirt; a++) mv = fmin(mv, d.eps_v[a]);
+    if (!(2.0 * mo - 2.0 * mv < 0.0) || !std::isfinite(mo) || !std::isfinite(mv))
+      fail(RIMP2_ERR_NUMERIC, "fragment %lld: orbital-energy denominator >= 0 or non-finite", (long long)f);
+    b->nocc[f] = d.nocc;
+    b->nvirt[f] = d.nvirt;
+    b->naux[f] = d.naux;
+    b->b64_off[f] = b64n;
+    b64n += (size_t)(d.naux * d.nvirt * d.nocc);
+    occ_total += (size_t)d.nocc;
+    virt_total += (size_t)d.nvirt;
+  }
+  double* b64 = static_cast<double*>(b->b64.ensure(b64n * sizeof(double)));
+  double* eo = static_cast<double*>(b->eps_o.ensure((occ_total + virt_total) * sizeof(double)));
+  double* ev = eo + occ_total;
+  hipStream_t s = c->stream;
+  size_t o = 0, v = 0;
+  for (int64_t f = 0; f < nfrag; f++) {
+    const rimp2_frag_desc& d = frags[f];
+    const int64_t n = d.naux;
+    double* dV = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
+    RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, s));
+    RIMP2_HIP_CHECK(hipMemcpyAsync(dV, d.V, (size_t)(n * n) * sizeof(double), hipMemcpyHostToDevice, s));
+    metric_factor(c, n, dV);
+    const size_t nx = (size_t)(d.naux * d.nvirt * d.nocc);
+    double* dX = static_cast<double*>(c->X.ensure(nx * sizeof(double)));
+    RIMP2_HIP_CHECK(hipMemcpyAsync(dX, d.Pia, nx * sizeof(double), hipMemcpyHostToDevice, s));
+    metric_transform(c, d.nvirt, d.nocc, dX);
+    RIMP2_HIP_CHECK(hipMemcpyAsync(b64 + b->b64_off[f], c->B64.ptr, nx * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RIMP2_HIP_CHECK(hipMemcpyAsync(eo + o, d.eps_o, (size_t)d.nocc * sizeof(double), hipMemcpyHostToDevice, s));
+    RIMP2_HIP_CHECK(hipMemcpyAsync(ev + v, d.eps_v, (size_t)d.nvirt * sizeof(double), hipMemcpyHostToDevice, s));
+    o += (size_t)d.nocc;
+    v += (size_t)d.nvirt;
+    int st[2] = {0, 0};
+    RIMP2_HIP_CHECK(hipMemcpyAsync(st, c->status.ptr, sizeof(st), hipMemcpyDeviceToHost, s));
+    RIMP2_HIP_CHECK(hipStreamSynchronize(s));
+    if (st[0] & 2) fail(RIMP2_ERR_NOT_SPD, "fragment %lld: metric not positive definite (pivot %d)", (long long)f, st[1]);
+  }
+  // the single-fragment state no longer matches the context buffers
+  c->have_metric = false;
+  c->have_B = false;
+  // autotune the pair tile for (mode, size class)
+  const std::pair<int, int> key(mode, size_class(b));
+  auto it = b->tuned.find(key);
+  b->ntune = 0;
+  int best = 0;
+  if (it != b->tuned.end()) best = it->second;
+  else {
+    double best_ms = 1e300;
+    for (int t : tile_candidates(mode)) {
+      layout_batch(c, b, t);
+      time_once(c, b);  // warm-up (first launch of the instantiation)
+      const double ms = time_once(c, b);
+      if (b->ntune < 4) {
+        b->tune_tiles[b->ntune] = t;
+        b->tune_ms[b->ntune] = ms;
+        b->ntune++;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    b->tuned[key] = best;
+  }
+  layout_batch(c, b, best);
+  capture_graph(c, b);
+  b->ready = true;
+}
+
+}  // namespace
 }  // namespace rimp2
 
 using namespace rimp2;
+
 extern "C" {
-int rimp2_batch(void* ctx, int64_t, const rimp2_frag_desc*, int, double*, double*) {
-  set_error(ctx, "rimp2_batch: not built yet");
-  return RIMP2_ERR_STATE;
+
+int rimp2_batch_prepare(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  prepare(c, nfrag, frags, mode);
+  RIMP2_API_END(ctx)
 }
-int rimp2_batch_prepare(void* ctx, int64_t, const rimp2_frag_desc*, int) {
-  set_error(ctx, "rimp2_batch_prepare: not built yet");
-  return RIMP2_ERR_STATE;
+
+int rimp2_batch_run(void* ctx, double* d_e_frag) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  BatchState* b = c->batch;
+  if (!b || !b->ready) fail(RIMP2_ERR_STATE, "rimp2_batch_prepare must run first");
+  ScopedTimer t(c, RIMP2_K_BATCH);
+  RIMP2_HIP_CHECK(hipGraphLaunch(b->exec, c->stream));
+  if (d_e_frag)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(d_e_frag, b->e_frag.ptr, (size_t)b->nfrag * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+  RIMP2_API_END(ctx)
 }
-int rimp2_batch_run(void* ctx, double*) { set_error(ctx, "not built yet"); return RIMP2_ERR_STATE; }
-int rimp2_batch_result(void* ctx, double*, double*) { set_error(ctx, "not built yet"); return RIMP2_ERR_STATE; }
-int rimp2_batch_tile(void* ctx, int64_t*) { set_error(ctx, "not built yet"); return RIMP2_ERR_STATE; }
+
+int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  BatchState* b = c->batch;
+  if (!b || !b->ready) fail(RIMP2_ERR_STATE, "rimp2_batch_prepare must run first");
+  if (!e_frag) fail(RIMP2_ERR_ARG, "null e_frag");
+  RIMP2_HIP_CHECK(hipMemcpyAsync(e_frag, b->e_frag.ptr, (size_t)b->nfrag * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+  if (eps_pair_packed)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(eps_pair_packed, b->pairs.ptr, (size_t)b->total_pairs_sq * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
 }
+
+int rimp2_batch(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode, double* e_frag,
+                double* eps_pair_packed) {
+  int st = rimp2_batch_prepare(ctx, nfrag, frags, mode);
+  if (st) return st;
+  st = rimp2_batch_run(ctx, nullptr);
+  if (st) return st;
+  return rimp2_batch_result(ctx, e_frag, eps_pair_packed);
+}
+
+int rimp2_batch_tile(void* ctx, int64_t* tile) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (!tile) fail(RIMP2_ERR_ARG, "null tile");
+  if (!c->batch || !c->batch->ready) fail(RIMP2_ERR_STATE, "rimp2_batch_prepare must run first");
+  *tile = c->batch->tile;
+  RIMP2_API_END(ctx)
+}
+
+}  // extern "C"

@@ -150,6 +150,19 @@ constexpr int pair_stage_bytes() {
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+// Workgroup barrier for the operand pipeline.  __syncthreads() carries a
+// workgroup release/acquire fence, and the release half makes the compiler wait
+// for every LDS DMA still in flight (vmcnt(0)), which would drain the ring at
+// each stage.  The stage data is instead covered by the explicit vmcnt waits
+// below (a wave's DMA writes are in LDS once its count drops), so only the
+// execution barrier is needed; the signal fences keep the compiler from moving
+// LDS accesses across it.
+__device__ inline void pipeline_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // s_waitcnt on the vector-memory counter only (expcnt / lgkmcnt left at their
 // maxima): returns once at most n of this wave's vector loads are in flight.
 #define RIMP2_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14)) | 0x0F70)
@@ -255,18 +268,20 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     }
   };
 
-  // step(st): wait for stage st (stage st+1 may stay in flight), barrier
-  // (stage st visible to all waves; the buffer of stage st-1 is free), issue
-  // stage st+2 into that buffer, consume stage st.
+  // step(st): wait for stage st (stage st+1 stays in flight), barrier (stage
+  // st visible to all waves; the buffer of stage st-1 is free), issue stage
+  // st+2 into that buffer, consume stage st.  The issue is unconditional (the
+  // last two steps re-fetch stage nst-1 into a free buffer): a branch around
+  // the DMA makes the compiler lose track of which buffer is in flight and
+  // wait for all DMA before every LDS read.
   auto step = [&](int64_t st, const unsigned char* cur, unsigned char* next2) {
-    if (st + 1 < nst) RIMP2_WAIT_VMCNT(NI);
-    else RIMP2_WAIT_VMCNT(0);
-    __syncthreads();
-    if (st + 2 < nst) issue(st + 2, next2);
+    RIMP2_WAIT_VMCNT(NI);
+    pipeline_barrier();
+    issue(st + 2 < nst ? st + 2 : nst - 1, next2);
     compute(cur);
   };
   issue(0, buf0);
-  if (nst > 1) issue(1, buf1);
+  issue(1, buf1);
   // nst is a multiple of 3 (the auxiliary padding guarantees it), so the
   // three-step body needs no early exits.
   for (int64_t st = 0; st < nst; st += 3) {
@@ -274,6 +289,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     step(st + 1, buf1, buf0);
     step(st + 2, buf2, buf1);
   }
+  RIMP2_WAIT_VMCNT(0);  // the trailing re-fetches have landed before LDS is reused
 
   // ---- epilogue: energy terms in FP64 ----
   const double ei = f.eps_o[i], ej = f.eps_o[j];

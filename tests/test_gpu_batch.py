@@ -1,0 +1,118 @@
+"""Batched DEC fragments through one persistent, graph-captured launch.
+
+Parity: every fragment's energy and pair matrix against the CPU oracle (mode
+tolerances as in test_gpu_parity.py).  The captured graph is replayed several
+times with eager work and synchronisations between replays; every replay
+must reproduce the first result bit for bit.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL = {"fp64": 1e-11, "fp32": 2e-7, "bf16x3": 2e-5}
+TOL_ABS = {"i8s5": 1e-6, "i8s4": 1e-6}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_batch(rimp2, seed, nfrag, scale=1):
+    o, v, q = rimp2.gen_batch_shapes(seed, nfrag)
+    frags = []
+    for f in range(nfrag):
+        nocc, nvirt, naux = int(o[f]) // scale or 1, int(v[f]) // scale or 1, int(q[f]) // scale or 1
+        frags.append(rimp2.gen_fragment(seed + f, naux, nvirt, nocc))
+    return frags
+
+
+def oracle_energies(oracle, frags):
+    out = []
+    for P, V, eo, ev in frags:
+        B = oracle.metric_transform(V, P)
+        out.append(oracle.pair_energies(B, eo, ev))
+    return out
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "i8s5", "bf16x3"])
+def test_batch_matches_oracle(ctx, oracle, rimp2, mode):
+    frags = make_batch(rimp2, 11, 9, scale=4)
+    say("batch", mode, [f[0].shape for f in frags])
+    ref = oracle_energies(oracle, frags)
+    ef, pp = ctx.batch(frags, mode)
+    off = 0
+    for k, (P, V, eo, ev) in enumerate(frags):
+        Eref, epsref = ref[k]
+        n = eo.size
+        eps = pp[off:off + n * n].reshape(n, n, order="F")
+        off += n * n
+        if mode in TOL_ABS:
+            assert abs(ef[k] - Eref) < TOL_ABS[mode], (k, ef[k], Eref)
+        else:
+            assert abs(ef[k] - Eref) <= TOL_REL[mode] * abs(Eref), (k, ef[k], Eref)
+        if mode == "fp64":
+            np.testing.assert_allclose(eps, epsref, rtol=1e-10, atol=1e-15)
+        np.testing.assert_array_equal(eps, eps.T)
+    assert ctx.batch_tile() in (32, 64, 128)
+
+
+def test_batch_graph_replays_are_identical(ctx, rimp2):
+    frags = make_batch(rimp2, 21, 12, scale=2)
+    say("graph replay", len(frags))
+    ctx.batch_prepare(frags, "fp32")
+    ctx.batch_run()
+    e0, p0 = ctx.batch_result()
+    P, V, eo, ev = frags[0]
+    for rep in range(3):
+        # eager work on the same context between replays
+        E_single, _ = ctx.energy(P, V, eo, ev, "fp64")
+        assert E_single < 0
+        ctx.sync()
+        ctx.batch_run()
+        e1, p1 = ctx.batch_result()
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_array_equal(p1, p0)
+
+
+def test_batch_equals_single_fragment_path(ctx, rimp2):
+    frags = make_batch(rimp2, 31, 6, scale=2)
+    say("batch vs single", len(frags))
+    ef, _ = ctx.batch(frags, "i8s5")
+    for k, (P, V, eo, ev) in enumerate(frags):
+        E, _ = ctx.energy(P, V, eo, ev, "i8s5")
+        assert abs(E - ef[k]) < 1e-9 * abs(E) + 1e-9
+
+
+def test_batch_errors(ctx, rimp2):
+    say("batch errors")
+    P, V, eo, ev = rimp2.gen_fragment(3, 40, 10, 3)
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.batch([(P, V, eo, ev - 10.0)], "fp64")
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.batch([], "fp64")
+    assert ei.value.code == rimp2.ERR_ARG
+    # still usable
+    ef, _ = ctx.batch([(P, V, eo, ev)], "fp64")
+    assert ef[0] < 0
+
+
+def test_dec_batch_256_fragments(ctx, oracle, rimp2):
+    """The 256-fragment DEC batch (nocc~U{4..32}, nvirt~U{32..256},
+    naux=4 nvirt): fp32 and i8s5 through the persistent launch; a seeded
+    subset of fragments is checked against the oracle."""
+    frags = make_batch(rimp2, 3, 256)
+    say("dec batch 256")
+    e32, _ = ctx.batch(frags, "fp32")
+    e8, _ = ctx.batch(frags, "i8s5")
+    assert np.all(e32 < 0) and np.all(e8 < 0)
+    rel = np.abs(e32 - e8) / np.abs(e8)
+    assert rel.max() < 2e-7
+    idx = [0, 17, 101, 255]
+    for k in idx:
+        P, V, eo, ev = frags[k]
+        B = oracle.metric_transform(V, P)
+        Eref, _ = oracle.pair_energies(B, eo, ev)
+        say("frag", k, P.shape, "dE_i8s5 uEh", (e8[k] - Eref) * 1e6)
+        assert abs(e8[k] - Eref) < 1e-6
