@@ -91,6 +91,26 @@ def cpu_baseline(r, P, V, eo, ev, budget_s=25.0):
                 sample=f"{len(idx)} of {npairs} unique pairs (extrapolated)", seconds=dt), None
 
 
+def copy_probe(r, mib):
+    """One hipMemcpy device-to-device of `mib` MiB: its FETCH_SIZE / WRITE_SIZE
+    counters give the scale between counter KiB and bytes moved."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    ctx = r.Context(0)
+    n = mib << 20
+    src, dst = ctx.alloc(n), ctx.alloc(n)
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    for _ in range(2):
+        st = hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(n), 3)
+        if st != 0:
+            raise RuntimeError(f"hipMemcpy failed: {st}")
+    ctx.sync()
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 2}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +123,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-memory (PCIe-inclusive) run")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (calibrates the HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {a.gpus} is not supported (the pair and fragment "
@@ -110,6 +133,9 @@ def main():
         sys.exit(2)
 
     import rimp2_gpu_amd as r
+    if a.copy_probe:
+        copy_probe(r, a.copy_probe)
+        return
     naux, nvirt, nocc = a.naux, a.nvirt, a.nocc
     log(f"[bench] generating fragment nocc={nocc} nvirt={nvirt} naux={naux} seed={a.seed}")
     P, V, eo, ev = r.gen_fragment(a.seed, naux, nvirt, nocc)
@@ -132,9 +158,11 @@ def main():
         log(f"[bench] fp64: {t64 * 1e3:.3f} ms/step E={E64:.10f}")
 
     # time-to-energy from host memory (PCIe-inclusive), one run
-    t0 = time.perf_counter()
-    E_host, _ = ctx.energy(P, V, eo, ev, a.mode)
-    tte_host = time.perf_counter() - t0
+    tte_host = None
+    if not a.no_host:
+        t0 = time.perf_counter()
+        ctx.energy(P, V, eo, ev, a.mode)
+        tte_host = time.perf_counter() - t0
 
     pair_ms = k_mode["pair"][0] / max(1, k_mode["pair"][1])
     achieved = Fp / (pair_ms * 1e-3) / 1e12

@@ -135,9 +135,14 @@ int rimp2_batch_tile(void* ctx, int64_t* tile);
 int rimp2_pair_features(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v,
                         float* feats);
 int rimp2_mlp_init(void* ctx, uint64_t seed);
-/* One Adam step on a minibatch (host arrays); loss receives the mean squared
- * error of the normalised target before the step. */
+/* Standardisation used by every MLP call: feature f enters as
+ * (x_f - mean[f]) / stdev[f]; the network predicts (log(-eps_ij) - tmean)/tstd. */
+int rimp2_mlp_set_norm(void* ctx, const float* mean, const float* stdev, float tmean, float tstd);
+/* One Adam step on a minibatch (host arrays: raw features [n,16], raw targets
+ * log(-eps_ij)); loss receives the mean squared error of the standardised
+ * target before the step. */
 int rimp2_mlp_train_step(void* ctx, int64_t n, const float* feats, const float* target, float lr, float* loss);
+/* pred[k] = predicted log(-eps) of sample k (raw units). */
 int rimp2_mlp_predict(void* ctx, int64_t n, const float* feats, float* pred);
 int rimp2_mlp_get_params(void* ctx, float* params, int64_t n);
 int rimp2_mlp_set_params(void* ctx, const float* params, int64_t n);
@@ -168,6 +173,16 @@ int rimp2_timing_reset(void* ctx);
  * eps_o ~ U[-2,-0.3], eps_v ~ U[0.1,3.0]; any output pointer may be NULL. */
 int rimp2_gen_fragment(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double* Pia, double* V,
                        double* eps_o, double* eps_v);
+/* Locality-decay variant (lam > 0): orbitals on a 1-D chain, (P|ia) scaled by
+ * exp(-(|x_Q-x_i| + |x_a-x_i|)/lam), A[p,q] by exp(-|x_p-x_q|/lam), so pair
+ * energies decay with |i-j|.  lam <= 0 gives rimp2_gen_fragment exactly. */
+int rimp2_gen_fragment_local(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double lam, double* Pia,
+                             double* V, double* eps_o, double* eps_v);
+/* The same generator evaluated on the GPU into device buffers (any may be
+ * NULL), asynchronous on the context stream; agrees with the host generator to
+ * a few ulps (device math library). */
+int rimp2_gen_fragment_device(void* ctx, uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double lam,
+                              double* d_Pia, double* d_V, double* d_eps_o, double* d_eps_v);
 /* nocc ~ U{4..32}, nvirt ~ U{32..256}, naux = 4 nvirt per fragment. */
 int rimp2_gen_batch_shapes(uint64_t seed, int64_t nfrag, int64_t* nocc, int64_t* nvirt, int64_t* naux);
 

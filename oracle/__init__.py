@@ -82,11 +82,40 @@ def lib():
 
 
 MLP_SIG = {
-    "oracle_mlp_forward": (None, [_i64, _dp, _dp, _dp]),
-    "oracle_mlp_loss_grad": (ctypes.c_double, [_i64, _dp, _dp, _dp, _dp]),
+    "oracle_mlp_loss_grad": (ctypes.c_double, [_i64, _fp, _fp, _fp, _fp, ctypes.c_int, _dp, _dp]),
     "oracle_gelu": (ctypes.c_double, [ctypes.c_double]),
+    "oracle_gelu_d": (ctypes.c_double, [ctypes.c_double]),
+    "oracle_bf16": (ctypes.c_float, [ctypes.c_float]),
     "oracle_mlp_num_params": (_i64, []),
 }
+
+
+def mlp_loss_grad(feats, target, params, norm, emulate_bf16=False, want_grad=True):
+    """FP64 loss, gradient and raw predictions of the screening MLP."""
+    F = np.ascontiguousarray(feats, dtype=np.float32)
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    N = np.ascontiguousarray(norm, dtype=np.float32)
+    n = F.shape[0]
+    g = np.zeros(lib().oracle_mlp_num_params()) if want_grad else None
+    pred = np.zeros(n)
+    fp = ctypes.POINTER(ctypes.c_float)
+    loss = lib().oracle_mlp_loss_grad(n, F.ctypes.data_as(fp), None if t is None else t.ctypes.data_as(fp),
+                                      P.ctypes.data_as(fp), N.ctypes.data_as(fp), 1 if emulate_bf16 else 0,
+                                      None if g is None else g.ctypes.data_as(_dp), pred.ctypes.data_as(_dp))
+    return loss, g, pred
+
+
+def gelu(x):
+    return lib().oracle_gelu(x)
+
+
+def gelu_d(x):
+    return lib().oracle_gelu_d(x)
+
+
+def bf16(x):
+    return lib().oracle_bf16(x)
 
 
 def _p(a):

@@ -63,6 +63,7 @@ SIGNATURES = {
     "rimp2_batch_tile": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
     "rimp2_pair_features": (ctypes.c_int, [_vp, _i64, _i64, _dp, _dp, _fp]),
     "rimp2_mlp_init": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "rimp2_mlp_set_norm": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_float, ctypes.c_float]),
     "rimp2_mlp_train_step": (ctypes.c_int, [_vp, _i64, _fp, _fp, ctypes.c_float, _fp]),
     "rimp2_mlp_predict": (ctypes.c_int, [_vp, _i64, _fp, _fp]),
     "rimp2_mlp_get_params": (ctypes.c_int, [_vp, _fp, _i64]),
@@ -80,6 +81,10 @@ SIGNATURES = {
     "rimp2_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(_i64)]),
     "rimp2_timing_reset": (ctypes.c_int, [_vp]),
     "rimp2_gen_fragment": (ctypes.c_int, [ctypes.c_uint64, _i64, _i64, _i64, _dp, _dp, _dp, _dp]),
+    "rimp2_gen_fragment_local": (ctypes.c_int, [ctypes.c_uint64, _i64, _i64, _i64, ctypes.c_double, _dp, _dp, _dp,
+                                                _dp]),
+    "rimp2_gen_fragment_device": (ctypes.c_int, [_vp, ctypes.c_uint64, _i64, _i64, _i64, ctypes.c_double, _vp, _vp,
+                                                 _vp, _vp]),
     "rimp2_gen_batch_shapes": (ctypes.c_int, [ctypes.c_uint64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                               ctypes.POINTER(_i64)]),
 }
@@ -137,6 +142,16 @@ def gen_fragment(seed, naux, nvirt, nocc, want_V=True):
     eo = np.empty(nocc)
     ev = np.empty(nvirt)
     check(lib().rimp2_gen_fragment(seed, naux, nvirt, nocc, _ptr(Pia), _ptr(V), _ptr(eo), _ptr(ev)))
+    return Pia, V, eo, ev
+
+
+def gen_fragment_local(seed, naux, nvirt, nocc, lam):
+    """Locality-decay variant of gen_fragment (lam <= 0 reproduces it exactly)."""
+    Pia = np.empty((naux, nvirt, nocc), dtype=np.float64, order="F")
+    V = np.empty((naux, naux), dtype=np.float64, order="F")
+    eo = np.empty(nocc)
+    ev = np.empty(nvirt)
+    check(lib().rimp2_gen_fragment_local(seed, naux, nvirt, nocc, lam, _ptr(Pia), _ptr(V), _ptr(eo), _ptr(ev)))
     return Pia, V, eo, ev
 
 
@@ -251,6 +266,9 @@ class Context:
     def sync(self):
         self._chk(lib().rimp2_sync(self.h))
 
+    def gen_device(self, seed, naux, nvirt, nocc, dPia, dV, deo, dev, lam=0.0):
+        self._chk(lib().rimp2_gen_fragment_device(self.h, seed, naux, nvirt, nocc, lam, dPia, dV, deo, dev))
+
     def energy_device(self, naux, nvirt, nocc, dPia, dV, deo, dev, mode, dpairs, detot):
         self._chk(lib().rimp2_energy_device(self.h, naux, nvirt, nocc, dPia, dV, deo, dev, _mode(mode), dpairs, detot))
 
@@ -316,6 +334,12 @@ class Context:
 
     def mlp_init(self, seed=0):
         self._chk(lib().rimp2_mlp_init(self.h, seed))
+
+    def mlp_set_norm(self, mean, std, tmean, tstd):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        sd = np.ascontiguousarray(std, dtype=np.float32)
+        self._chk(lib().rimp2_mlp_set_norm(self.h, _ptr(m, ctypes.c_float), _ptr(sd, ctypes.c_float), float(tmean),
+                                           float(tstd)))
 
     def mlp_train_step(self, feats, target, lr):
         F = np.ascontiguousarray(feats, dtype=np.float32)

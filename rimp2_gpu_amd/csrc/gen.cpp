@@ -10,6 +10,11 @@
 //                        sin for odd n
 // Streams: 0 = (P|ia), 1 = metric factor A, 2 = occupied energies,
 //          3 = virtual energies, 4 = batch fragment shapes.
+// Locality-decay variant (lam > 0, rimp2_gen_fragment_local): orbitals sit on
+// a 1-D chain, occupied i at x_i = i, auxiliary Q at x_Q = (Q+1/2) nocc/naux,
+// virtual a at x_a = (a+1/2) nocc/nvirt;  (P|ia) is multiplied by
+// exp(-(|x_Q - x_i| + |x_a - x_i|)/lam) and A[p,q] by exp(-|x_p - x_q|/lam),
+// so pair couplings decay with |i - j| as in a spatially extended fragment.
 // Distributions (virtual index a is 0-based):
 //   (P|ia)[Q,a,i] = normal * 0.1 / sqrt(1 + a/16),  layout [naux, nvirt, nocc]
 //   V = A A^T / naux + I,  A[p,q] = normal,          layout [naux, naux]
@@ -49,27 +54,40 @@ inline double normal(uint64_t key, uint64_t n) {
 
 extern "C" {
 
-int rimp2_gen_fragment(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double* Pia, double* V,
-                       double* eps_o, double* eps_v) {
+int rimp2_gen_fragment_local(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double lam, double* Pia,
+                             double* V, double* eps_o, double* eps_v) {
   if (naux <= 0 || nvirt <= 0 || nocc <= 0) return RIMP2_ERR_ARG;
+  if (!std::isfinite(lam)) return RIMP2_ERR_ARG;
+  const bool local = lam > 0.0;
   if (Pia) {
     const uint64_t kx = stream_key(seed, 0);
     const int64_t ncol = nvirt * nocc;
 #pragma omp parallel for schedule(static)
     for (int64_t c = 0; c < ncol; c++) {
-      const int64_t a = c % nvirt;
-      const double s = 0.1 / std::sqrt(1.0 + (double)a / 16.0);
+      const int64_t a = c % nvirt, i = c / nvirt;
+      double s = 0.1 / std::sqrt(1.0 + (double)a / 16.0);
+      const double xa = ((double)a + 0.5) * (double)nocc / (double)nvirt;
+      if (local) s *= std::exp(-std::fabs(xa - (double)i) / lam);
       double* col = Pia + naux * c;
-      for (int64_t q = 0; q < naux; q++) col[q] = normal(kx, (uint64_t)(q + naux * c)) * s;
+      for (int64_t q = 0; q < naux; q++) {
+        double v = normal(kx, (uint64_t)(q + naux * c)) * s;
+        if (local) v *= std::exp(-std::fabs(((double)q + 0.5) * (double)nocc / (double)naux - (double)i) / lam);
+        col[q] = v;
+      }
     }
   }
   if (V) {
     const uint64_t ka = stream_key(seed, 1);
     // A stored transposed (row p contiguous) so that V_pq = sum_k A_pk A_qk streams.
     std::vector<double> At((size_t)(naux * naux));
+    const double h = (double)nocc / (double)naux;
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < naux; p++)
-      for (int64_t k = 0; k < naux; k++) At[(size_t)(k + naux * p)] = normal(ka, (uint64_t)(p + naux * k));
+      for (int64_t k = 0; k < naux; k++) {
+        double v = normal(ka, (uint64_t)(p + naux * k));
+        if (local) v *= std::exp(-std::fabs((double)(p - k)) * h / lam);
+        At[(size_t)(k + naux * p)] = v;
+      }
     const double inv = 1.0 / (double)naux;
 #pragma omp parallel for schedule(dynamic, 8)
     for (int64_t q = 0; q < naux; q++) {
@@ -100,6 +118,11 @@ int rimp2_gen_fragment(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc,
     for (int64_t a = 0; a < nvirt; a++) eps_v[a] = 0.1 + 2.9 * uniform53(kv, (uint64_t)a);
   }
   return RIMP2_OK;
+}
+
+int rimp2_gen_fragment(uint64_t seed, int64_t naux, int64_t nvirt, int64_t nocc, double* Pia, double* V,
+                       double* eps_o, double* eps_v) {
+  return rimp2_gen_fragment_local(seed, naux, nvirt, nocc, 0.0, Pia, V, eps_o, eps_v);
 }
 
 // Shapes of a batch of DEC fragments: nocc ~ U{4..32}, nvirt ~ U{32..256},

@@ -317,6 +317,11 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
   c->naux = n;
 }
 
+void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  gemm_f64<128, 128, true, false, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 // B64 = Linv * X  (X: naux x (nvirt*nocc) device, column-major).
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   const int64_t n = c->naux, ncol = nvirt * nocc;

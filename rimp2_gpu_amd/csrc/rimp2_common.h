@@ -109,6 +109,9 @@ struct ScopedTimer {
 // --- metric (metric.hip) ---
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
+// C = alpha * A * B^T + beta * C (column-major FP64 MFMA GEMM)
+void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
 
 // --- packing (pack.hip) ---
 PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile);

@@ -244,3 +244,71 @@ def test_metric_transform_identities(oracle):
     np.testing.assert_allclose(B2, sl.solve_triangular(L, X2, lower=True), rtol=1e-11, atol=1e-12)
     np.testing.assert_allclose(B2.T @ B2, X2.T @ np.linalg.solve(V, X2), rtol=1e-10, atol=1e-11)
     np.testing.assert_allclose(oracle.metric_transform(np.asfortranarray(np.eye(naux)), X), X, rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# screening MLP oracle
+# ---------------------------------------------------------------------------
+def mlp_case(seed, n=37):
+    rng = np.random.default_rng(seed)
+    import oracle as o
+    npar = o.lib().oracle_mlp_num_params()
+    assert npar == 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    P = (rng.standard_normal(npar) * 0.2).astype(np.float32)
+    F = rng.standard_normal((n, 16)).astype(np.float32) * 2 + 1
+    T = rng.standard_normal(n).astype(np.float32) + 3
+    norm = np.concatenate([np.full(16, 1.0), np.full(16, 2.0), [3.0, 1.5]]).astype(np.float32)
+    return P, F, T, norm
+
+
+def test_mlp_oracle_finite_differences(oracle):
+    P, F, T, norm = mlp_case(1)
+    loss, g, _ = oracle.mlp_loss_grad(F, T, P, norm)
+    rng = np.random.default_rng(2)
+    idx = np.concatenate([rng.choice(1024, 6), 1024 + rng.choice(64, 3), 1088 + rng.choice(4096, 8),
+                          5184 + rng.choice(64, 3), 5248 + rng.choice(64, 3), [5312]])
+    for k in idx:
+        h = 1e-3 * max(1.0, abs(float(P[k])))
+        Pp, Pm = P.astype(np.float64).copy(), P.astype(np.float64).copy()
+        # parameters are float32 at the interface: step by an exactly representable amount
+        Pp[k] = np.float32(P[k] + h)
+        Pm[k] = np.float32(P[k] - h)
+        lp, _, _ = oracle.mlp_loss_grad(F, T, Pp.astype(np.float32), norm, want_grad=False)
+        lm, _, _ = oracle.mlp_loss_grad(F, T, Pm.astype(np.float32), norm, want_grad=False)
+        fd = (lp - lm) / (float(Pp[k]) - float(Pm[k]))
+        assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(g[k]), (k, fd, g[k])
+    assert loss > 0
+
+
+def test_gelu_and_bf16_closed_forms(oracle):
+    import math as m
+    assert oracle.gelu(0.0) == 0.0
+    assert abs(oracle.gelu(1.0) - 0.5 * (1 + m.erf(1 / m.sqrt(2)))) < 1e-16
+    for x in [-3.0, -0.7, 0.2, 2.5]:
+        assert abs(oracle.gelu(x) - oracle.gelu(-x) - x) < 1e-15  # GELU(x) - GELU(-x) = x
+        h = 1e-6
+        assert abs((oracle.gelu(x + h) - oracle.gelu(x - h)) / (2 * h) - oracle.gelu_d(x)) < 1e-8
+    assert oracle.gelu_d(0.0) == 0.5
+    assert oracle.bf16(1.0 + 2.0 ** -8) == 1.0            # tie -> even
+    assert oracle.bf16(1.0 + 3 * 2.0 ** -9) == 1.0 + 2.0 ** -7
+    assert oracle.bf16(1.0 + 2.0 ** -7 + 2.0 ** -8) == 1.0 + 2.0 ** -6  # tie -> even (up)
+    assert oracle.bf16(-2.5) == -2.5
+
+
+def test_mlp_bf16_emulation_is_close_to_fp64(oracle):
+    P, F, T, norm = mlp_case(3, n=64)
+    l64, g64, p64 = oracle.mlp_loss_grad(F, T, P, norm)
+    lbf, gbf, pbf = oracle.mlp_loss_grad(F, T, P, norm, emulate_bf16=True)
+    assert np.linalg.norm(gbf - g64) <= 5e-2 * np.linalg.norm(g64)
+    assert np.max(np.abs(pbf - p64)) < 0.1 * np.max(np.abs(p64))
+
+
+def test_generator_locality_variant(rimp2):
+    P0, V0, eo0, ev0 = rimp2.gen_fragment(8, 64, 16, 6)
+    P1, V1, eo1, ev1 = rimp2.gen_fragment_local(8, 64, 16, 6, 0.0)
+    np.testing.assert_array_equal(P0, P1)
+    np.testing.assert_array_equal(V0, V1)
+    P2, V2, _, _ = rimp2.gen_fragment_local(8, 64, 16, 6, 1.0)
+    # occupied 0 lives at the start of the chain: far auxiliary functions are damped
+    assert np.abs(P2[-8:, :, 0]).max() < 1e-2 * np.abs(P2[:8, :, 0]).max()
+    assert np.linalg.eigvalsh(V2).min() > 0.99
