@@ -187,7 +187,8 @@ int rimp2_ctx_destroy(void* ctx) {
   resolve_timers(c);
   batch_destroy(c->batch);
   mlp_destroy(c->mlp);
-  for (DevBuf* b : {&c->V, &c->Linv, &c->work, &c->X, &c->B64, &c->packed, &c->scales, &c->eps_o, &c->eps_v,
+  for (DevBuf* b : {&c->V, &c->Linv, &c->work, &c->linv_packed, &c->X, &c->B64, &c->packed, &c->scales, &c->x_packed,
+                    &c->eps_o, &c->eps_v,
                     &c->partials, &c->eps_pair, &c->etot, &c->status})
     b->release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

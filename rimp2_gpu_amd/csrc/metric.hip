@@ -6,10 +6,13 @@
 // The reference receives the fitted tensor Calpha(NBA,nvirt,nocc) from its
 // parent code (RIMP2.F90:48-49, "alphaCD ... in the diagonal basis"); this file
 // builds that tensor from the raw (P|ia) and the metric V_PQ.
-#include "rimp2_common.h"
+#include "pair_core.h"
 
 namespace rimp2 {
 namespace {
+
+using pairk::lds_ptr_t;
+using pairk::pipeline_barrier;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -191,68 +194,193 @@ void gemm_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, cons
 // ---------------------------------------------------------------------------
 // Diagonal block: factor A[k0:k0+kb, k0:k0+kb] = L11 L11^T in LDS, write L11
 // back (zeros above the diagonal) and L11^{-1} into Linv at the same block.
-// status bit 1 is set on a non-positive pivot; pivot_at receives the global
+// status bit 1 is set on a non-positive pivot; status[1] receives the global
 // index of the first failing pivot (1-based).
+// One barrier per column: the factorisation runs as an unscaled LDL^T-style
+// right-looking update, A[r][q] -= A[r][c] A[q][c] / d_c, with the column
+// scaling by 1/sqrt(d_c) deferred to the end; the inverse sweep scales row c
+// of X on the fly in the same way.  Thread t owns row t%64 and columns
+// t/64 + 4k of each update.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
                                                          int64_t k0, int kb, int* status) {
   __shared__ double L[NB][NB + 1];
   __shared__ double X[NB][NB + 1];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
-  if (tid == 0) bad = 0;
+  __shared__ double dg[NB];
+  const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
   for (int e = tid; e < NB * NB; e += 256) {
-    int r = e % NB, c = e / NB;
-    L[r][c] = (r < kb && c < kb && c <= r) ? A[(k0 + r) + lda * (k0 + c)] : 0.0;
+    const int rr = e % NB, cc = e / NB;
+    L[rr][cc] = (rr < kb && cc < kb && cc <= rr) ? A[(k0 + rr) + lda * (k0 + cc)] : 0.0;
   }
   __syncthreads();
+  bool bad = false;
   for (int c = 0; c < kb; c++) {
-    if (tid == 0) {
-      double d = L[c][c];
-      if (!(d > 0.0) && !bad) {
-        bad = 1;
+    double d = L[c][c];  // final: every update of column c's pivot is done
+    if (!(d > 0.0)) {
+      if (tid == 0 && !bad) {
         atomicCAS(status + 1, 0, (int)(k0 + c + 1));
         atomicOr(status, 2);
       }
-      L[c][c] = (d > 0.0) ? sqrt(d) : 1.0;
+      bad = true;
+      d = 1.0;
     }
-    __syncthreads();
-    const double inv = 1.0 / L[c][c];
-    for (int r = c + 1 + tid; r < kb; r += 256) L[r][c] *= inv;
-    __syncthreads();
-    // trailing update of the lower triangle: thread -> row tid%64, columns
-    // tid/64 + 4t (no integer division in the loop)
-    {
-      const int r = tid & 63;
-      if (r > c && r < kb) {
-        const double lrc = L[r][c];
-        for (int q = c + 1 + ((tid >> 6) - (c + 1) % 4 + 4) % 4; q <= r; q += 4) L[r][q] -= lrc * L[q][c];
-      }
+    const double inv_d = 1.0 / d;
+    if (r > c && r < kb) {
+      const double lrc = L[r][c] * inv_d;
+      for (int q = c + 1 + ((g - (c + 1) % 4 + 4) % 4); q <= r; q += 4) L[r][q] -= lrc * L[q][c];
     }
+    if (tid == 0) dg[c] = d;
     __syncthreads();
   }
-  // L11^{-1} by a right-looking sweep over L11 X = I: row c of X is final
-  // after the division by L[c][c]; it is then eliminated from the rows below.
+  // L[r][c] = A_rc / sqrt(d_c) below the diagonal, sqrt(d_c) on it
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int rr = e % NB, cc = e / NB;
+    if (rr < kb && cc < kb && cc <= rr) L[rr][cc] = (rr == cc) ? sqrt(dg[cc]) : L[rr][cc] / sqrt(dg[cc]);
+  }
   for (int e = tid; e < NB * NB; e += 256) X[e % NB][e / NB] = (e % NB == e / NB) ? 1.0 : 0.0;
   __syncthreads();
+  // L11 X = I by a right-looking sweep: at step c, row c of X (scaled by
+  // 1/L[c][c] on the fly) is eliminated from the rows below; rows are scaled
+  // at the end
   for (int c = 0; c < kb; c++) {
     const double inv = 1.0 / L[c][c];
-    for (int j = tid; j <= c; j += 256) X[c][j] *= inv;
-    __syncthreads();
-    {
-      const int r = tid & 63;
-      if (r > c && r < kb) {
-        const double lrc = L[r][c];
-        for (int j = tid >> 6; j <= c; j += 4) X[r][j] -= lrc * X[c][j];
-      }
+    if (r > c && r < kb) {
+      const double lrc = L[r][c] * inv;
+      for (int j = g; j <= c; j += 4) X[r][j] -= lrc * X[c][j];
     }
     __syncthreads();
   }
   for (int e = tid; e < kb * kb; e += 256) {
-    int r = e % kb, c = e / kb;
-    A[(k0 + r) + lda * (k0 + c)] = (c <= r) ? L[r][c] : 0.0;
-    Linv[(k0 + r) + ldi * (k0 + c)] = X[r][c];
+    const int rr = e % kb, cc = e / kb;
+    A[(k0 + rr) + lda * (k0 + cc)] = (cc <= rr) ? L[rr][cc] : 0.0;
+    Linv[(k0 + rr) + ldi * (k0 + cc)] = (cc <= rr) ? X[rr][cc] / L[rr][rr] : 0.0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Metric transform on packed operands.
+//   Lp[ks][mb][lane]: A fragments of L^{-1}, lane l <-> (m = 16 mb + l%16,
+//                     k = 4 ks + l/16), zero above the diagonal and in padding;
+//   Xp[i][ks][rb][lane]: B fragments of (P|ia), lane l <-> (q = 4 ks + l/16,
+//                     a = 16 rb + l%16)  (the FP64 operand layout of pack.hip);
+// so every 512-byte fragment is contiguous in HBM and in LDS.  One workgroup
+// computes a 64 (Q rows) x 128 (virtuals of one occupied) tile of B = L^{-1} X
+// with a three-stage LDS ring fed by global_load_lds_dwordx4 (24 KB per
+// 16-wide K stage); the K loop of a row tile stops at its last row (L^{-1} is
+// lower triangular).  Waves 2 x 2, wave tile 32 x 64 = 2 x 4 MFMA blocks.
+// ---------------------------------------------------------------------------
+constexpr int TR_TM = 64, TR_TN = 128, TR_KQ = 48;  // K padding: three 16-wide stages
+
+__global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64_t MB, int64_t KS,
+                                 double* __restrict__ Lp) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= KS * MB * 64) return;
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6, mb = f % MB, ks = f / MB;
+  const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
+  Lp[t] = (m < n && k < n && k <= m) ? Li[m + n * k] : 0.0;
+}
+
+struct TransformArgs {
+  const unsigned char* Lp;  // packed L^{-1}
+  const unsigned char* Xp;  // packed X
+  double* B;                // [naux, nvirt, nocc]
+  int64_t naux, nvirt, nocc;
+  int64_t MB, KS;           // L^{-1} row blocks (16) and k-steps (4)
+  int64_t NRB;              // X row blocks (16 virtuals) per occupied
+  int64_t MT, NT;           // row tiles, column tiles per occupied
+  int64_t nwu, chunk;
+};
+
+__global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
+  constexpr int FB = 512;                     // bytes per fragment
+  constexpr int A_STAGE = 4 * 4 * FB;         // 4 k-steps x 4 row blocks
+  constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks
+  constexpr int SB = A_STAGE + B_STAGE;       // 24 KB
+  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage
+  __shared__ __align__(16) unsigned char buf0[SB];
+  __shared__ __align__(16) unsigned char buf1[SB];
+  __shared__ __align__(16) unsigned char buf2[SB];
+  const int64_t bid = blockIdx.x;
+  const int64_t wu = (bid & 7) * p.chunk + (bid >> 3);
+  if (wu >= p.nwu) return;
+  const int64_t mt = wu % p.MT, col = wu / p.MT, i = col / p.NT, nt = col % p.NT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t m0 = mt * TR_TM;
+  int64_t kend = m0 + TR_TM;
+  if (kend > p.KS * 4) kend = p.KS * 4;
+  kend = ((kend + TR_KQ - 1) / TR_KQ) * TR_KQ;  // multiple of three stages
+  if (kend > p.KS * 4) kend = p.KS * 4;
+  const int64_t nst = kend / 16;  // KS*4 is a multiple of 48, so nst % 3 == 0
+  const unsigned char* abase = p.Lp + (size_t)(mt * 4) * FB;
+  const size_t a_ks = (size_t)p.MB * FB;
+  const unsigned char* bbase = p.Xp + ((size_t)i * p.KS * p.NRB + (size_t)nt * 8) * FB;
+  const size_t b_ks = (size_t)p.NRB * FB;
+
+  v4d acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  auto issue = [&](int64_t st, unsigned char* dst) {
+#pragma unroll
+    for (int n = 0; n < NI; n++) {
+      const int wb = (n * 4 + wave) * 1024;
+      const unsigned char* g;
+      if (wb < A_STAGE) {
+        const int ks = wb / (4 * FB), off = wb % (4 * FB) + lane * 16;
+        g = abase + (size_t)(st * 4 + ks) * a_ks + off;
+      } else {
+        const int w2 = wb - A_STAGE;
+        const int ks = w2 / (8 * FB), off = w2 % (8 * FB) + lane * 16;
+        g = bbase + (size_t)(st * 4 + ks) * b_ks + off;
+      }
+      __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)(dst + wb), 16, 0, 0);
+    }
+  };
+  auto compute = [&](const unsigned char* stg) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+      double af[2], bf[4];
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+        af[x] = *reinterpret_cast<const double*>(stg + (ks * 4 + wr * 2 + x) * FB + lane * 8);
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+        bf[y] = *reinterpret_cast<const double*>(stg + A_STAGE + (ks * 8 + wc * 4 + y) * FB + lane * 8);
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+    }
+  };
+  auto step = [&](int64_t st, const unsigned char* cur, unsigned char* next2) {
+    RIMP2_WAIT_VMCNT(NI);
+    pipeline_barrier();
+    issue(st + 2 < nst ? st + 2 : nst - 1, next2);
+    compute(cur);
+  };
+  issue(0, buf0);
+  issue(1, buf1);
+  for (int64_t st = 0; st < nst; st += 3) {
+    step(st, buf0, buf2);
+    step(st + 1, buf1, buf0);
+    step(st + 2, buf2, buf1);
+  }
+  RIMP2_WAIT_VMCNT(0);
+  // epilogue: D layout d[r] = D[4r + l/16][l%16]
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t q = m0 + wr * 32 + x * 16 + 4 * r + (lane >> 4);
+        const int64_t a = nt * TR_TN + wc * 64 + y * 16 + (lane & 15);
+        if (q < p.naux && a < p.nvirt) p.B[q + p.naux * (a + p.nvirt * i)] = acc[x][y][r];
+      }
 }
 
 __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
@@ -314,6 +442,14 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
       }
     }
   }
+  {
+    // L^{-1} in A-fragment order for the transform GEMM
+    const int64_t MB = round_up(n, TR_TM) / 16, KS = round_up(n, TR_KQ) / 4;
+    double* Lp = static_cast<double*>(c->linv_packed.ensure((size_t)(KS * MB * 64) * sizeof(double)));
+    hipLaunchKernelGGL(pack_linv_kernel, dim3((unsigned)ceil_div(KS * MB * 64, 256)), dim3(256), 0, s, Li, n, MB, KS,
+                       Lp);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
   c->naux = n;
 }
 
@@ -322,12 +458,40 @@ void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
   gemm_f64<128, 128, true, false, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-// B64 = Linv * X  (X: naux x (nvirt*nocc) device, column-major).
+// B64 = Linv * X  (X: naux x (nvirt*nocc) device, column-major): X is packed
+// into B-fragment order, then one DMA-pipelined FP64 MFMA GEMM.
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   const int64_t n = c->naux, ncol = nvirt * nocc;
   double* B = static_cast<double*>(c->B64.ensure((size_t)(n * ncol) * sizeof(double)));
+  TransformArgs p;
+  p.naux = n;
+  p.nvirt = nvirt;
+  p.nocc = nocc;
+  p.MB = round_up(n, TR_TM) / 16;
+  p.KS = round_up(n, TR_KQ) / 4;
+  p.NRB = round_up(nvirt, TR_TN) / 16;
+  p.MT = round_up(n, TR_TM) / TR_TM;
+  p.NT = p.NRB / 8;
+  p.nwu = p.MT * p.NT * nocc;
+  p.chunk = ceil_div(p.nwu, 8);
+  PackGeom gx;
+  gx.mode = RIMP2_MODE_FP64;
+  gx.rb = 16;
+  gx.kk = 4;
+  gx.planes = 1;
+  gx.lane_bytes = 8;
+  gx.nvp = p.NRB * 16;
+  gx.nqp = p.KS * 4;
+  gx.nks = p.KS;
+  gx.nrb = p.NRB;
+  void* Xp = c->x_packed.ensure(gx.bytes_per_occ() * (size_t)nocc);
   ScopedTimer t(c, RIMP2_K_TRANSFORM);
-  gemm_f64<128, 128, false, true, false>(c->stream, n, ncol, n, 1.0, c->Linv.as<double>(), n, dX, n, 0.0, B, n);
+  pack_B(c, dX, n, nvirt, nocc, gx, Xp, nullptr, c->stream);
+  p.Lp = c->linv_packed.as<unsigned char>();
+  p.Xp = static_cast<const unsigned char*>(Xp);
+  p.B = B;
+  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * p.chunk)), dim3(256), 0, c->stream, p);
+  RIMP2_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace rimp2

@@ -77,12 +77,12 @@ struct Ctx {
   // metric state
   int64_t naux = 0;
   bool have_metric = false;
-  DevBuf V, Linv, work;
+  DevBuf V, Linv, work, linv_packed;
   // transform state
   int64_t nvirt = 0, nocc = 0;
   int mode = -1;
   bool have_B = false;
-  DevBuf X, B64, packed, scales;
+  DevBuf X, B64, packed, scales, x_packed;
   PackGeom geom;
   int tile = 0;  // pair tile (virtual block) used for the packed layout
   // pair state
