@@ -18,6 +18,8 @@
 // Per-lane Kahan sums are reduced by wave shuffles and LDS to one FP64 partial
 // per workgroup; the partials are summed per pair and then over pairs in a
 // fixed order with Neumaier compensation (deterministic).
+#include <algorithm>
+
 #include "pair_core.h"
 
 namespace rimp2 {
@@ -219,7 +221,8 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what) {
   (void)what;
   int* status = static_cast<int*>(c->status.ensure(64));
-  hipLaunchKernelGGL(finite_check_kernel, dim3(256), dim3(256), 0, c->stream, p, n, status);
+  const unsigned blocks = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(1, ceil_div(n, 256 * 8)));
+  hipLaunchKernelGGL(finite_check_kernel, dim3(blocks), dim3(256), 0, c->stream, p, n, status);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
