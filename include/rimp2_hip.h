@@ -98,6 +98,22 @@ int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const dou
 int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Pia, const double* d_V,
                         const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair, double* d_e_tot);
 
+/* ---- the same path from already-fitted coefficients (no metric) ---- */
+
+/* Loads the fitted 3-index tensor Calpha(NBA,nvirt,nocc) exactly as the
+ * reference reads it (RIMP2.F90:47-49, read(fileLu) Calpha) and packs it for
+ * `mode`; B = Calpha, no metric step.  rimp2_pair_energies then runs on it. */
+int rimp2_load_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha, int mode);
+/* rimp2_load_fitted + rimp2_pair_energies: the reference's per-fragment call
+ * of RIMP2_calc_toccA / RIMP2_calc_gocc (RIMP2.F90:99, 207) reduced to the
+ * canonical energy, from host arrays. */
+int rimp2_energy_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha,
+                        const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot);
+/* Device-pointer, asynchronous variant of rimp2_energy_fitted. */
+int rimp2_energy_fitted_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Calpha,
+                               const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair,
+                               double* d_e_tot);
+
 /* ---- inspection of intermediate results (host pointers, blocking) ---- */
 int rimp2_get_cholesky(void* ctx, double* L);      /* L [naux,naux], lower, zeros above */
 int rimp2_get_metric_inverse(void* ctx, double* Linv); /* L^{-1} [naux,naux] */
@@ -107,7 +123,7 @@ int rimp2_get_B(void* ctx, double* B);             /* FP64 B [naux,nvirt,nocc] *
 typedef struct rimp2_frag_desc {
   int64_t nocc, nvirt, naux;
   const double* Pia;   /* [naux, nvirt, nocc] */
-  const double* V;     /* [naux, naux] */
+  const double* V;     /* [naux, naux]; NULL: Pia is already fitted (B = Pia) */
   const double* eps_o; /* [nocc] */
   const double* eps_v; /* [nvirt] */
 } rimp2_frag_desc;

@@ -53,6 +53,9 @@ SIGNATURES = {
     "rimp2_pair_energies": (ctypes.c_int, [_vp, _i64, _i64, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "rimp2_load_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, ctypes.c_int]),
+    "rimp2_energy_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
+    "rimp2_energy_fitted_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "rimp2_get_cholesky": (ctypes.c_int, [_vp, _dp]),
     "rimp2_get_metric_inverse": (ctypes.c_int, [_vp, _dp]),
     "rimp2_get_B": (ctypes.c_int, [_vp, _dp]),
@@ -211,6 +214,24 @@ class Context:
         naux, nvirt, nocc = Pia.shape
         self._chk(lib().rimp2_transform(self.h, naux, nvirt, nocc, _ptr(Pia), _mode(mode)))
 
+    def load_fitted(self, Calpha, mode="fp64"):
+        """Already-fitted Calpha [naux, nvirt, nocc] becomes B (no metric)."""
+        Calpha = _f64(Calpha)
+        naux, nvirt, nocc = Calpha.shape
+        self._chk(lib().rimp2_load_fitted(self.h, naux, nvirt, nocc, _ptr(Calpha), _mode(mode)))
+
+    def energy_fitted(self, Calpha, eps_o, eps_v, mode="fp64", want_pairs=True):
+        Calpha, eo, ev = _f64(Calpha), _f64(eps_o), _f64(eps_v)
+        naux, nvirt, nocc = Calpha.shape
+        ep = np.zeros((nocc, nocc), order="F") if want_pairs else None
+        et = np.zeros(1)
+        self._chk(lib().rimp2_energy_fitted(self.h, naux, nvirt, nocc, _ptr(Calpha), _ptr(eo), _ptr(ev), _mode(mode),
+                                            _ptr(ep), _ptr(et)))
+        return float(et[0]), ep
+
+    def energy_fitted_device(self, naux, nvirt, nocc, dB, deo, dev, mode, dpairs, detot):
+        self._chk(lib().rimp2_energy_fitted_device(self.h, naux, nvirt, nocc, dB, deo, dev, _mode(mode), dpairs, detot))
+
     def pair_energies(self, eps_o, eps_v, mode="fp64", want_pairs=True):
         eo, ev = _f64(eps_o), _f64(eps_v)
         nocc, nvirt = eo.size, ev.size
@@ -290,7 +311,8 @@ class Context:
         arr = (FragDesc * len(frags))()
         keep = []
         for k, (Pia, V, eo, ev) in enumerate(frags):
-            Pia, V, eo, ev = _f64(Pia), _f64(V), _f64(eo), _f64(ev)
+            # V None: Pia is already fitted (B = Pia), no metric
+            Pia, V, eo, ev = _f64(Pia), (None if V is None else _f64(V)), _f64(eo), _f64(ev)
             keep += [Pia, V, eo, ev]
             naux, nvirt, nocc = Pia.shape
             arr[k] = FragDesc(nocc, nvirt, naux, _ptr(Pia), _ptr(V), _ptr(eo), _ptr(ev))

@@ -370,7 +370,7 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode) {
     const rimp2_frag_desc& d = frags[f];
     if (d.nocc <= 0 || d.nvirt <= 0 || d.naux <= 0 || d.nocc > 65536 || d.nvirt > 65536 || d.naux > 65536)
       fail(RIMP2_ERR_ARG, "fragment %lld: bad dimensions", (long long)f);
-    if (!d.Pia || !d.V || !d.eps_o || !d.eps_v) fail(RIMP2_ERR_ARG, "fragment %lld: null array", (long long)f);
+    if (!d.Pia || !d.eps_o || !d.eps_v) fail(RIMP2_ERR_ARG, "fragment %lld: null array", (long long)f);
     if (mode_slices(mode) && round_up(d.naux, 96) > 16384)
       fail(RIMP2_ERR_ARG, "fragment %lld: exact-slice modes support naux <= 16384", (long long)f);
     double mo = -INFINITY, mv = INFINITY;
@@ -394,15 +394,20 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode) {
   for (int64_t f = 0; f < nfrag; f++) {
     const rimp2_frag_desc& d = frags[f];
     const int64_t n = d.naux;
-    double* dV = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
-    RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, s));
-    RIMP2_HIP_CHECK(hipMemcpyAsync(dV, d.V, (size_t)(n * n) * sizeof(double), hipMemcpyHostToDevice, s));
-    metric_factor(c, n, dV);
     const size_t nx = (size_t)(d.naux * d.nvirt * d.nocc);
-    double* dX = static_cast<double*>(c->X.ensure(nx * sizeof(double)));
-    RIMP2_HIP_CHECK(hipMemcpyAsync(dX, d.Pia, nx * sizeof(double), hipMemcpyHostToDevice, s));
-    metric_transform(c, d.nvirt, d.nocc, dX);
-    RIMP2_HIP_CHECK(hipMemcpyAsync(b64 + b->b64_off[f], c->B64.ptr, nx * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, s));
+    if (d.V) {
+      double* dV = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
+      RIMP2_HIP_CHECK(hipMemcpyAsync(dV, d.V, (size_t)(n * n) * sizeof(double), hipMemcpyHostToDevice, s));
+      metric_factor(c, n, dV);
+      double* dX = static_cast<double*>(c->X.ensure(nx * sizeof(double)));
+      RIMP2_HIP_CHECK(hipMemcpyAsync(dX, d.Pia, nx * sizeof(double), hipMemcpyHostToDevice, s));
+      metric_transform(c, d.nvirt, d.nocc, dX);
+      RIMP2_HIP_CHECK(hipMemcpyAsync(b64 + b->b64_off[f], c->B64.ptr, nx * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      // already fitted: Pia is B
+      RIMP2_HIP_CHECK(hipMemcpyAsync(b64 + b->b64_off[f], d.Pia, nx * sizeof(double), hipMemcpyHostToDevice, s));
+    }
     RIMP2_HIP_CHECK(hipMemcpyAsync(eo + o, d.eps_o, (size_t)d.nocc * sizeof(double), hipMemcpyHostToDevice, s));
     RIMP2_HIP_CHECK(hipMemcpyAsync(ev + v, d.eps_v, (size_t)d.nvirt * sizeof(double), hipMemcpyHostToDevice, s));
     o += (size_t)d.nocc;

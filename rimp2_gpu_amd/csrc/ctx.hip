@@ -134,12 +134,13 @@ static void validate_dims(int64_t naux, int64_t nvirt, int64_t nocc) {
 }
 
 // transform + pack from a device X; the metric must already be factored.
-static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX, int mode) {
-  validate_mode(mode);
-  if (!c->have_metric) fail(RIMP2_ERR_STATE, "rimp2_metric must run before the transform");
-  if (mode_slices(mode) && round_up(c->naux, 32) > 16384)
+static void check_slice_bound(int64_t naux, int mode) {
+  if (mode_slices(mode) && round_up(naux, 32) > 16384)
     fail(RIMP2_ERR_ARG, "exact-slice modes support naux <= 16384 (int32 accumulation bound)");
-  metric_transform(c, nvirt, nocc, dX);
+}
+
+// Packs the FP64 B held in c->B64 ([c->naux, nvirt, nocc]) for the pair kernel.
+static void pack_current_B(Ctx* c, int64_t nvirt, int64_t nocc, int mode) {
   c->tile = choose_tile(mode, nvirt, nocc);
   PackGeom g = make_geom(mode, c->naux, nvirt, c->tile);
   const size_t bytes = g.bytes_per_occ() * (size_t)nocc;
@@ -152,6 +153,28 @@ static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* 
   c->nocc = nocc;
   c->mode = mode;
   c->have_B = true;
+}
+
+static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX, int mode) {
+  validate_mode(mode);
+  if (!c->have_metric) fail(RIMP2_ERR_STATE, "rimp2_metric must run before the transform");
+  check_slice_bound(c->naux, mode);
+  metric_transform(c, nvirt, nocc, dX);
+  pack_current_B(c, nvirt, nocc, mode);
+}
+
+// Already-fitted coefficients (B = Calpha, no metric) from a device array.
+// A factored metric of another naux no longer matches the context.
+static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc, const double* dB, int mode,
+                               hipMemcpyKind kind) {
+  check_slice_bound(naux, mode);
+  c->have_B = false;
+  if (naux != c->naux) c->have_metric = false;
+  c->naux = naux;
+  const size_t bytes = (size_t)(naux * nvirt * nocc) * sizeof(double);
+  double* B = static_cast<double*>(c->B64.ensure(bytes));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(B, dB, bytes, kind, c->stream));
+  pack_current_B(c, nvirt, nocc, mode);
 }
 
 }  // namespace rimp2
@@ -254,13 +277,51 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
   RIMP2_API_END(ctx)
 }
 
+int rimp2_load_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha, int mode) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_dims(naux, nvirt, nocc);
+  validate_mode(mode);
+  if (!Calpha) fail(RIMP2_ERR_ARG, "null Calpha");
+  check_host_finite(Calpha, naux * nvirt * nocc, "Calpha");
+  load_fitted_device(c, naux, nvirt, nocc, Calpha, mode, hipMemcpyHostToDevice);
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_energy_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha,
+                        const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot) {
+  int st = rimp2_load_fitted(ctx, naux, nvirt, nocc, Calpha, mode);
+  if (st) return st;
+  return rimp2_pair_energies(ctx, nocc, nvirt, eps_o, eps_v, mode, eps_pair, e_tot);
+}
+
+int rimp2_energy_fitted_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Calpha,
+                               const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair,
+                               double* d_e_tot) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_dims(naux, nvirt, nocc);
+  validate_mode(mode);
+  if (!d_Calpha || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
+  reset_status(c);
+  check_inputs_device(c, d_Calpha, naux * nvirt * nocc, "Calpha");
+  check_inputs_device(c, d_eps_o, nocc, "eps_o");
+  check_inputs_device(c, d_eps_v, nvirt, "eps_v");
+  denom_check_device(c, d_eps_o, nocc, d_eps_v, nvirt);
+  load_fitted_device(c, naux, nvirt, nocc, d_Calpha, mode, hipMemcpyDeviceToDevice);
+  pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,
+                       d_e_tot, nullptr, 0);
+  RIMP2_API_END(ctx)
+}
+
 int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v, int mode,
                         double* eps_pair, double* e_tot) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
   validate_mode(mode);
   if (!eps_o || !eps_v || !e_tot) fail(RIMP2_ERR_ARG, "null eps_o / eps_v / e_tot");
-  if (!c->have_B) fail(RIMP2_ERR_STATE, "rimp2_transform must run before rimp2_pair_energies");
+  if (!c->have_B) fail(RIMP2_ERR_STATE, "rimp2_transform or rimp2_load_fitted must run before rimp2_pair_energies");
   if (c->nocc != nocc || c->nvirt != nvirt || c->mode != mode)
     fail(RIMP2_ERR_STATE, "pair energies requested for (nocc=%lld, nvirt=%lld, mode=%d) but the transform holds "
          "(%lld, %lld, %d)", (long long)nocc, (long long)nvirt, mode, (long long)c->nocc, (long long)c->nvirt, c->mode);

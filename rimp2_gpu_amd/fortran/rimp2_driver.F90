@@ -8,8 +8,8 @@
 !>   rimp2_driver.x file <Calpha.data> <mode>
 !>       fragment file in the reference's record order (RIMP2.F90:45-63):
 !>       header (nvirt,nocc,noccEOS,nvirtEOS,NBA,nocctot), Calpha(NBA,nvirt,nocc),
-!>       EVocc(nocc), EVvirt(nvirt).  Calpha is already fitted, so the metric is
-!>       the identity (B = Calpha).
+!>       EVocc(nocc), EVvirt(nvirt).  Calpha is already fitted (B = Calpha) and
+!>       goes through rimp2_energy_fitted / rimp2_load_fitted, no metric.
 !> mode: 0 fp64, 1 fp32, 2 bf16x3, 3 i8s4, 4 i8s5.
 !> Prints "E_MP2 <mode> <E>" and "dE_uEh <mode-vs-fp64>"; stops with
 !> 'ERROR3' when the energy is not finite (the reference's failure word,
@@ -19,14 +19,16 @@ program rimp2_driver
   use rimp2_hip_interfaces
   implicit none
   character(len=512) :: arg, fname
-  integer(C_INT64_T) :: seed, naux, nvirt, nocc, i
+  integer(C_INT64_T) :: seed, naux, nvirt, nocc
   integer :: hdr(6), lu, ios, imode
   integer(C_INT) :: mode, stat
   type(C_PTR) :: ctx
   real(C_DOUBLE), allocatable :: Pia(:,:,:), V(:,:), eo(:), ev(:), eps(:,:), eps64(:,:)
   real(C_DOUBLE) :: E, E64
   integer(C_SIZE_T) :: total, free
+  logical :: fitted
 
+  fitted = .false.
   call get_command_argument(1, arg)
   if (trim(arg) == 'gen') then
      call get_command_argument(2, arg); read(arg, *) seed
@@ -50,7 +52,8 @@ program rimp2_driver
      if (ios /= 0) error stop 'ERROR3'
      nvirt = hdr(1); nocc = hdr(2); naux = hdr(5)
      print *, 'nvirt,nocc,noccEOS,nvirtEOS,NBA,nocctot', hdr
-     allocate(Pia(naux, nvirt, nocc), V(naux, naux), eo(nocc), ev(nvirt))
+     allocate(Pia(naux, nvirt, nocc), eo(nocc), ev(nvirt))
+     fitted = .true.
      read(lu, iostat=ios) Pia
      if (ios /= 0) error stop 'ERROR3'
      read(lu, iostat=ios) eo
@@ -58,10 +61,6 @@ program rimp2_driver
      read(lu, iostat=ios) ev
      if (ios /= 0) error stop 'ERROR3'
      close(lu)
-     V = 0.0_C_DOUBLE
-     do i = 1, naux
-        V(i, i) = 1.0_C_DOUBLE
-     enddo
   else
      print *, 'usage: rimp2_driver.x gen <seed> <naux> <nvirt> <nocc> <mode> | file <path> <mode>'
      error stop 'ERROR3'
@@ -75,11 +74,18 @@ program rimp2_driver
   print '(A,I0,A,I0)', 'device memory total ', total, ' free ', free
 
   allocate(eps(nocc, nocc), eps64(nocc, nocc))
-  stat = rimp2_energy(ctx, naux, nvirt, nocc, Pia, V, eo, ev, RIMP2_MODE_FP64, eps64, E64)
-  call rimp2_check(stat, ctx, 'rimp2_energy(fp64)')
-  ! the metric is factored once; the other mode reuses it
-  stat = rimp2_transform(ctx, naux, nvirt, nocc, Pia, mode)
-  call rimp2_check(stat, ctx, 'rimp2_transform')
+  if (fitted) then
+     stat = rimp2_energy_fitted(ctx, naux, nvirt, nocc, Pia, eo, ev, RIMP2_MODE_FP64, eps64, E64)
+     call rimp2_check(stat, ctx, 'rimp2_energy_fitted(fp64)')
+     stat = rimp2_load_fitted(ctx, naux, nvirt, nocc, Pia, mode)
+     call rimp2_check(stat, ctx, 'rimp2_load_fitted')
+  else
+     stat = rimp2_energy(ctx, naux, nvirt, nocc, Pia, V, eo, ev, RIMP2_MODE_FP64, eps64, E64)
+     call rimp2_check(stat, ctx, 'rimp2_energy(fp64)')
+     ! the metric is factored once; the other mode reuses it
+     stat = rimp2_transform(ctx, naux, nvirt, nocc, Pia, mode)
+     call rimp2_check(stat, ctx, 'rimp2_transform')
+  endif
   stat = rimp2_pair_energies(ctx, nocc, nvirt, eo, ev, mode, eps, E)
   call rimp2_check(stat, ctx, 'rimp2_pair_energies')
   if (E /= E .or. E64 /= E64) error stop 'ERROR3'

@@ -81,6 +81,24 @@ module rimp2_hip_interfaces
        real(C_DOUBLE) :: eps_pair(*), e_tot
      end function rimp2_energy
 
+     integer(C_INT) function rimp2_load_fitted(ctx, naux, nvirt, nocc, Calpha, mode) bind(C, name="rimp2_load_fitted")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: naux, nvirt, nocc
+       real(C_DOUBLE) :: Calpha(*)
+       integer(C_INT), value :: mode
+     end function rimp2_load_fitted
+
+     integer(C_INT) function rimp2_energy_fitted(ctx, naux, nvirt, nocc, Calpha, eps_o, eps_v, mode, eps_pair, e_tot) &
+          & bind(C, name="rimp2_energy_fitted")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: naux, nvirt, nocc
+       real(C_DOUBLE) :: Calpha(*), eps_o(*), eps_v(*)
+       integer(C_INT), value :: mode
+       real(C_DOUBLE) :: eps_pair(*), e_tot
+     end function rimp2_energy_fitted
+
      integer(C_INT) function rimp2_batch(ctx, nfrag, frags, mode, e_frag, eps_pair_packed) bind(C, name="rimp2_batch")
        import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE, rimp2_frag_desc
        type(C_PTR), value :: ctx

@@ -116,3 +116,26 @@ def test_dec_batch_256_fragments(ctx, oracle, rimp2):
         Eref, _ = oracle.pair_energies(B, eo, ev)
         say("frag", k, P.shape, "dE_i8s5 uEh", (e8[k] - Eref) * 1e6)
         assert abs(e8[k] - Eref) < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5"])
+def test_batch_mixes_fitted_and_metric_fragments(ctx, oracle, rimp2, mode):
+    """A descriptor with V = NULL carries fitted coefficients (B = Pia)."""
+    frags = make_batch(rimp2, 23, 6, scale=4)
+    rng = np.random.default_rng(5)
+    mixed, ref = [], []
+    for k, (P, V, eo, ev) in enumerate(frags):
+        if k % 2:
+            C = np.asfortranarray(rng.standard_normal(P.shape) * 0.05)
+            mixed.append((C, None, eo, ev))
+            ref.append(oracle.pair_energies(C, eo, ev)[0])
+        else:
+            mixed.append((P, V, eo, ev))
+            ref.append(oracle.pair_energies(oracle.metric_transform(V, P), eo, ev)[0])
+    say("batch mixed", mode, [f[0].shape for f in mixed])
+    ef, _ = ctx.batch(mixed, mode, want_pairs=False)
+    for k in range(len(mixed)):
+        if mode in TOL_ABS:
+            assert abs(ef[k] - ref[k]) < TOL_ABS[mode], (k, ef[k], ref[k])
+        else:
+            assert abs(ef[k] - ref[k]) <= TOL_REL[mode] * abs(ref[k]), (k, ef[k], ref[k])

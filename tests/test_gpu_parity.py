@@ -190,3 +190,74 @@ def test_c2_shape_exact_slice_within_1uEh(ctx, oracle, rimp2):
     assert abs(E8 - E64) < 1e-6
     np.testing.assert_array_equal(eps8, eps8.T)
     assert np.all(eps64 < 0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5", "fp32", "bf16x3"])
+def test_fitted_coefficients_match_oracle(ctx, oracle, rimp2, mode):
+    """Calpha read as the reference reads it (already fitted, B = Calpha)."""
+    rng = np.random.default_rng(61)
+    naux, nvirt, nocc = 200, 45, 7
+    say("fitted", mode, naux, nvirt, nocc)
+    C = np.asfortranarray(rng.standard_normal((naux, nvirt, nocc)) * 0.05)
+    eo = rng.uniform(-2.0, -0.3, nocc)
+    ev = rng.uniform(0.1, 3.0, nvirt)
+    Eref, epsref = oracle.pair_energies(C, eo, ev)
+    E, eps = ctx.energy_fitted(C, eo, ev, mode)
+    check_E(mode, E, Eref)
+    if mode == "fp64":
+        np.testing.assert_allclose(eps, epsref, rtol=1e-10, atol=1e-15)
+        np.testing.assert_array_equal(ctx.get_B(naux, nvirt, nocc), C)
+
+
+def test_fitted_device_path_matches_host(ctx, rimp2):
+    rng = np.random.default_rng(62)
+    naux, nvirt, nocc = 160, 70, 5
+    C = np.asfortranarray(rng.standard_normal((naux, nvirt, nocc)) * 0.05)
+    eo = rng.uniform(-2.0, -0.3, nocc)
+    ev = rng.uniform(0.1, 3.0, nvirt)
+    say("fitted device")
+    for mode in ["fp64", "i8s5"]:
+        E_host, eps_host = ctx.energy_fitted(C, eo, ev, mode)
+        d = [ctx.to_device(x) for x in (C, eo, ev)]
+        dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+        ctx.energy_fitted_device(naux, nvirt, nocc, *d, mode, dpe, det)
+        ctx.sync()
+        E_dev = float(ctx.from_device(det, (1,))[0])
+        eps_dev = ctx.from_device(dpe, (nocc, nocc))
+        for p in d + [dpe, det]:
+            ctx.free(p)
+        assert E_dev == E_host
+        np.testing.assert_array_equal(eps_dev, eps_host)
+
+
+def test_fitted_equals_metric_path_on_transformed_B(ctx, rimp2):
+    """Loading the B that the metric path produced gives the same energies."""
+    naux, nvirt, nocc = 192, 64, 6
+    P, V, eo, ev = rimp2.gen_fragment(63, naux, nvirt, nocc)
+    say("fitted vs metric path")
+    for mode in ["fp64", "i8s5"]:
+        E1, e1 = ctx.energy(P, V, eo, ev, mode)
+        B = ctx.get_B(naux, nvirt, nocc)
+        E2, e2 = ctx.energy_fitted(B, eo, ev, mode)
+        assert E1 == E2
+        np.testing.assert_array_equal(e1, e2)
+
+
+def test_fitted_state_errors(ctx, rimp2):
+    say("fitted state errors")
+    P, V, eo, ev = rimp2.gen_fragment(64, 48, 10, 3)
+    ctx.metric(V)
+    ctx.load_fitted(P[:40], "fp64")  # another naux: the factored metric no longer applies
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.transform(P, "fp64")
+    assert ei.value.code == rimp2.ERR_STATE
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.pair_energies(eo, ev, "i8s5")  # loaded for fp64
+    assert ei.value.code == rimp2.ERR_STATE
+    Pn = P.copy(order="F")
+    Pn[0, 0, 0] = np.inf
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.load_fitted(Pn, "fp64")
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    E, _ = ctx.energy_fitted(P, eo, ev, "fp64")
+    assert E < 0
