@@ -69,6 +69,11 @@ int rimp2_ctx_destroy(void* ctx);
 /* Replaces get_dev_mem (cuda_api.cu:4-9, gpu_interfaces.F90:47-53); same
  * argument order (total, free), built on hipMemGetInfo for the current device. */
 int rimp2_get_dev_mem(size_t* total, size_t* free_bytes);
+/* The reference's own symbol, same name and ABI (cuda_api.cu:4-9 declares
+ * `void get_dev_mem(size_t& total, size_t& free)` under extern "C"; bound by
+ * gpu_interfaces.F90:47-53), so that interface links against this library
+ * unchanged.  Both outputs are 0 when the query fails. */
+void get_dev_mem(size_t* total, size_t* free_bytes);
 /* Copies the last error text of ctx (or the library-global one when ctx is
  * NULL) into buf (NUL-terminated, at most n bytes). */
 int rimp2_last_error(void* ctx, char* buf, size_t n);
@@ -104,9 +109,9 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
  * reference reads it (RIMP2.F90:47-49, read(fileLu) Calpha) and packs it for
  * `mode`; B = Calpha, no metric step.  rimp2_pair_energies then runs on it. */
 int rimp2_load_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha, int mode);
-/* rimp2_load_fitted + rimp2_pair_energies: the reference's per-fragment call
- * of RIMP2_calc_toccA / RIMP2_calc_gocc (RIMP2.F90:99, 207) reduced to the
- * canonical energy, from host arrays. */
+/* rimp2_load_fitted + rimp2_pair_energies: the reference's per-fragment calls
+ * of RIMP2_calc_toccA and RIMP2_calc_gocc (RIMP2.F90:73, 146; GPU build
+ * RIMP2.GPU.F90:109, 228) reduced to the canonical energy, from host arrays. */
 int rimp2_energy_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Calpha,
                         const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot);
 /* Device-pointer, asynchronous variant of rimp2_energy_fitted. */

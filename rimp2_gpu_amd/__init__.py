@@ -53,6 +53,7 @@ SIGNATURES = {
     "rimp2_pair_energies": (ctypes.c_int, [_vp, _i64, _i64, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "get_dev_mem": (None, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "rimp2_load_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, ctypes.c_int]),
     "rimp2_energy_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy_fitted_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),

@@ -230,6 +230,13 @@ int rimp2_get_dev_mem(size_t* total, size_t* free_bytes) {
   RIMP2_API_END(nullptr)
 }
 
+void get_dev_mem(size_t* total, size_t* free_bytes) {
+  size_t t = 0, f = 0;
+  if (rimp2_get_dev_mem(&t, &f) != RIMP2_OK) t = f = 0;
+  if (total) *total = t;
+  if (free_bytes) *free_bytes = f;
+}
+
 int rimp2_last_error(void* ctx, char* buf, size_t n) {
   if (!buf || n == 0) return RIMP2_ERR_ARG;
   std::string m;

@@ -138,7 +138,7 @@ contains
        msg(k:k) = buf(k)
     enddo
     print '(A,A,A,I0,A,A)', 'rimp2 error in ', what, ' (status ', stat, '): ', trim(msg)
-    error stop 'RIMP2 ERROR'
+    error stop 'ERROR3'
   end subroutine rimp2_check
 
 end module rimp2_hip_interfaces

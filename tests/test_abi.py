@@ -5,6 +5,7 @@ import ctypes
 import hashlib
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -15,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     txt = open(os.path.join(ROOT, "include", "rimp2_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(rimp2_[A-Za-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(rimp2_[A-Za-z0-9_]+|get_dev_mem)\s*\(", txt)))
 
 
 def test_header_symbols_exported(rimp2):
@@ -31,7 +32,8 @@ def test_header_symbols_exported(rimp2):
 def test_fortran_interfaces_cover_core_api():
     f = open(os.path.join(ROOT, "rimp2_gpu_amd", "fortran", "rimp2_hip_interfaces.F90")).read()
     for s in ["rimp2_ctx_create", "rimp2_ctx_destroy", "rimp2_get_dev_mem", "rimp2_metric", "rimp2_transform",
-              "rimp2_pair_energies", "rimp2_energy", "rimp2_last_error", "rimp2_batch"]:
+              "rimp2_pair_energies", "rimp2_energy", "rimp2_last_error", "rimp2_batch", "rimp2_load_fitted",
+              "rimp2_energy_fitted"]:
         assert f'name="{s}"' in f, s
 
 
@@ -97,3 +99,27 @@ def test_flop_counts(rimp2):
     # unique-pair count of the pair contraction at the 64/512/2048 fragment
     assert rimp2.pair_flops(64, 512, 2048) == pytest.approx(2.233e12, rel=1e-3)
     assert rimp2.pair_flops(256, 2048, 6144) == pytest.approx(1.695e15, rel=1e-3)
+
+
+REF_IFACE = "/root/reference/gpu_interfaces.F90"
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_IFACE) and os.path.exists("/opt/rocm/bin/amdflang")),
+                    reason="reference interface module or amdflang not present")
+def test_reference_interface_module_links_get_dev_mem(tmp_path):
+    """The reference's own Fortran interface module (compiled where it lies,
+    output only into a temporary directory) binds get_dev_mem to this library
+    unchanged.  Without a GPU the query reports 0 / 0."""
+    prog = tmp_path / "call_get_dev_mem.F90"
+    prog.write_text("program p\n  use iso_c_binding\n  use gpu_interfaces\n  implicit none\n"
+                    "  integer(C_SIZE_T) :: total, free\n  total = 7\n  free = 7\n"
+                    "  call get_dev_mem(total, free)\n  print *, 'mem', total, free, total >= free\nend program p\n")
+    exe = tmp_path / "p.x"
+    libdir = os.path.join(ROOT, "rimp2_gpu_amd")
+    r = subprocess.run(["/opt/rocm/bin/amdflang", "-J", str(tmp_path), "-o", str(exe), REF_IFACE, str(prog),
+                        "-L" + libdir, "-lrimp2_hip", "-Wl,-rpath," + libdir], cwd=tmp_path,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120).stdout
+    w = out.split()
+    assert w[0] == "mem" and w[3] == "T", out

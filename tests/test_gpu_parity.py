@@ -261,3 +261,13 @@ def test_fitted_state_errors(ctx, rimp2):
     assert ei.value.code == rimp2.ERR_NUMERIC
     E, _ = ctx.energy_fitted(P, eo, ev, "fp64")
     assert E < 0
+
+
+def test_get_dev_mem_both_names(ctx, rimp2):
+    import ctypes
+    say("get_dev_mem")
+    t, f = rimp2.get_dev_mem()
+    assert t > 200 * 2**30 and 0 < f <= t  # 288 GB of HBM3E
+    t2, f2 = ctypes.c_size_t(), ctypes.c_size_t()
+    rimp2.lib().get_dev_mem(ctypes.byref(t2), ctypes.byref(f2))
+    assert t2.value == t and 0 < f2.value <= t
