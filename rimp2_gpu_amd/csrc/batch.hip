@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void batch_pair_kernel(BatchParams p) {
     int64_t i, j, A, Bk;
     tri_decode(pidx, i, j);
     tri_decode(tp, A, Bk);
-    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2);
+    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
     if (threadIdx.x == 0) p.partials[wu] = e;
   }
 }
@@ -371,7 +371,7 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode) {
     if (d.nocc <= 0 || d.nvirt <= 0 || d.naux <= 0 || d.nocc > 65536 || d.nvirt > 65536 || d.naux > 65536)
       fail(RIMP2_ERR_ARG, "fragment %lld: bad dimensions", (long long)f);
     if (!d.Pia || !d.eps_o || !d.eps_v) fail(RIMP2_ERR_ARG, "fragment %lld: null array", (long long)f);
-    if (mode_slices(mode) && round_up(d.naux, 96) > 16384)
+    if (mode_slices(mode) && d.naux > 16384)
       fail(RIMP2_ERR_ARG, "fragment %lld: exact-slice modes support naux <= 16384", (long long)f);
     double mo = -INFINITY, mv = INFINITY;
     for (int64_t i = 0; i < d.nocc; i++) mo = fmax(mo, d.eps_o[i]);

@@ -135,7 +135,8 @@ static void validate_dims(int64_t naux, int64_t nvirt, int64_t nocc) {
 
 // transform + pack from a device X; the metric must already be factored.
 static void check_slice_bound(int64_t naux, int mode) {
-  if (mode_slices(mode) && round_up(naux, 32) > 16384)
+  // padded to at most 16511 auxiliary rows: 5 * 127^2 * 16511 < 2^31
+  if (mode_slices(mode) && naux > 16384)
     fail(RIMP2_ERR_ARG, "exact-slice modes support naux <= 16384 (int32 accumulation bound)");
 }
 

@@ -30,14 +30,14 @@ int mode_slices(int mode) {
   }
 }
 
-// Auxiliary padding quantum per mode: three LDS stages of the pair kernel
-// (its operand ring has three buffers and a break-free three-step body).
+// Auxiliary padding quantum per mode: four LDS stages of the pair kernel (its
+// break-free loop body runs four stage buffers against two register sets).
 static int64_t q_quantum(int mode) {
   switch (mode) {
-    case RIMP2_MODE_FP64: return 3 * 16;
-    case RIMP2_MODE_FP32: return 3 * 16;
-    case RIMP2_MODE_BF16X3: return 3 * 16;
-    default: return 3 * 32;
+    case RIMP2_MODE_FP64: return 4 * 16;
+    case RIMP2_MODE_FP32: return 4 * 16;
+    case RIMP2_MODE_BF16X3: return 4 * 16;
+    default: return 4 * 32;
   }
 }
 
@@ -56,6 +56,10 @@ PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile) {
   g.nqp = round_up(naux, q_quantum(mode));
   g.nks = g.nqp / g.kk;
   g.nrb = g.nvp / g.rb;
+  // the pair kernel addresses one occupied orbital's block with 32-bit offsets
+  if (g.bytes_per_occ() > 0xFFFFFFFFull)
+    fail(RIMP2_ERR_ARG, "packed block per occupied orbital (%zu bytes) exceeds 4 GiB (naux=%lld nvirt=%lld)",
+         g.bytes_per_occ(), (long long)naux, (long long)nvirt);
   return g;
 }
 

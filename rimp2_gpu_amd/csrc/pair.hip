@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void pair_kernel(PairParams p) {
   int64_t i, j, A, Bk;
   tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
   tri_decode(tp, A, Bk);  // A <= Bk
-  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2);
+  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
   if (threadIdx.x == 0) p.partials[wu] = e;
 }
 

@@ -36,7 +36,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 // LB bytes per lane per fragment, ND accumulators per output fragment.
 // ---------------------------------------------------------------------------
 struct PolF64 {
-  static constexpr int RB = 16, KK = 4, P = 1, LB = 8, ND = 1, NE = 4, KSTEPS = 4;
+  static constexpr int RB = 16, KK = 4, P = 1, LB = 8, ND = 1, NE = 4, KSTEPS = 4, MMA_PER_CALL = 1;
   using frag_t = double;
   using acc_t = v4d;
   __device__ static void zero(acc_t& a) { a = v4d{0, 0, 0, 0}; }
@@ -49,7 +49,7 @@ struct PolF64 {
 };
 
 struct PolF32 {
-  static constexpr int RB = 32, KK = 2, P = 1, LB = 4, ND = 1, NE = 16, KSTEPS = 8;
+  static constexpr int RB = 32, KK = 2, P = 1, LB = 4, ND = 1, NE = 16, KSTEPS = 8, MMA_PER_CALL = 1;
   using frag_t = float;
   using acc_t = v16f;
   __device__ static void zero(acc_t& a) {
@@ -65,7 +65,7 @@ struct PolF32 {
 };
 
 struct PolBF16x3 {
-  static constexpr int RB = 32, KK = 16, P = 2, LB = 16, ND = 1, NE = 16, KSTEPS = 1;
+  static constexpr int RB = 32, KK = 16, P = 2, LB = 16, ND = 1, NE = 16, KSTEPS = 1, MMA_PER_CALL = 3;
   using frag_t = v8bf;
   using acc_t = v16f;
   __device__ static void zero(acc_t& a) { PolF32::zero(a); }
@@ -85,6 +85,8 @@ struct PolBF16x3 {
 template <int S, int DMAX>
 struct PolI8 {
   static constexpr int RB = 32, KK = 32, P = S, LB = 16, ND = DMAX + 1, NE = 16, KSTEPS = 1;
+  // products s + t = d <= DMAX with 0 <= s, t < S
+  static constexpr int MMA_PER_CALL = (DMAX + 1) * (DMAX + 2) / 2 - (DMAX >= S ? (DMAX - S + 1) * (DMAX - S + 2) : 0);
   using frag_t = v4i;
   using acc_t = v16i;
   __device__ static void zero(acc_t& a) {
@@ -163,26 +165,48 @@ __device__ inline void pipeline_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Raw buffer resource (stride 0) over [base, base + bytes); gfx9 word 3.
+__device__ inline __amdgpu_buffer_rsrc_t slab_rsrc(const unsigned char* base, size_t bytes) {
+  const int n = bytes > 0xFFFFFFFFull ? -1 : (int)(uint32_t)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, n, 0x00020000);
+}
+
 // s_waitcnt on the vector-memory counter only (expcnt / lgkmcnt left at their
 // maxima): returns once at most n of this wave's vector loads are in flight.
 #define RIMP2_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14)) | 0x0F70)
+// Same, and additionally every LDS read of this wave has returned (lgkmcnt 0).
+#define RIMP2_WAIT_VMCNT_LGKM0(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14)) | 0x0070)
 
 // One work unit: occupied pair (i <= j), virtual blocks A <= Bk.  All 256
 // threads call it; the return value (the unit's energy partial) is valid in
 // thread 0.  Ends with a barrier, so the stage buffers may be reused at once.
 //
-// Operand pipeline: three LDS stage buffers; the slabs of stages s+1 and s+2
-// are in flight HBM/L2 -> LDS (global_load_lds_dwordx4, each wave instruction
-// moves 1 KB, no staging registers) while the MFMAs consume stage s.  The
-// buffers are separate LDS objects so the compiler can see that the DMA
-// target never aliases the buffer being read.  Because the packed
-// layout stores every fragment in lane order, the DMA destination of a wave is
-// one contiguous KB and the MFMA operand reads (ds_read of 64 x lane_bytes)
-// are contiguous and bank-conflict free.
+// Operand pipeline: four LDS stage buffers fed by buffer_load_dwordx4 ... lds
+// (each wave instruction moves 1 KB, no staging registers) and two register
+// sets per wave.  Step st:
+//   wait   own DMA of stage st+1 landed; own LDS reads of stage st finished
+//   barrier  stage st+1 is in LDS for every wave; buffer st%4 is free
+//   issue  stage st+4 -> buffer st%4           (three steps of latency cover:
+//          loaded L2-miss latency exceeds two steps of MFMA work)
+//   read   stage st+1 -> register set (st+1)&1 (LDS latency hidden ...)
+//   MFMA   stage st from register set st&1      (... behind these)
+// The body is unrolled four times (4 buffers, 2 register sets) so every
+// buffer and register set is named statically; the auxiliary padding makes
+// the stage count a multiple of four.  The issue is unconditional (the last
+// steps re-fetch stage nst-1 into a free buffer, the last read fetches a
+// valid but unused buffer): a branch around the DMA makes the compiler lose
+// track of which buffer is in flight and wait for all DMA before every LDS
+// read.  Because the packed layout stores every fragment in lane order, a
+// wave's DMA destination is one contiguous KB and the MFMA operand reads
+// (ds_read of 64 x lane_bytes) are contiguous and bank-conflict free.
+//
+// DMA addressing: the slab resource and the stage offset of each transfer are
+// wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
+// offset, so each transfer is one buffer load plus the M0 write.
 template <class Pol, int TM>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
-                            unsigned char* __restrict__ buf2) {
+                            unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3) {
   constexpr int FPB = TM / Pol::RB;           // fragments per block side
   constexpr int WT = TM / 2;                  // wave tile side
   constexpr int FW = WT / Pol::RB;            // fragments per wave side
@@ -191,10 +215,13 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   constexpr int OPB = Pol::KSTEPS * CHUNK;    // bytes per operand per stage
   constexpr int SB = 4 * OPB;                 // bytes per stage
   constexpr int NI = SB / 4096;               // 1-KB DMA instructions per wave per stage
+  constexpr int KS = Pol::KSTEPS;
+  using frag_t = typename Pol::frag_t;
   static_assert(FW >= 1, "wave tile smaller than one MFMA fragment");
   static_assert(SB % 4096 == 0 && OPB % 1024 == 0, "stage must split into whole 1-KB wave transfers");
   const bool same_occ = (i == j), same_blk = (A == Bk);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
 
   typename Pol::acc_t acc1[FW][FW][Pol::ND], acc2[FW][FW][Pol::ND];
@@ -211,85 +238,116 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // operand slabs: 0 = B_i[A] (rows), 1 = B_j[B] (cols), 2 = B_j[A], 3 = B_i[B]
   const size_t per_ks = (size_t)f.nrb * Pol::P * FB;
   const size_t per_occ = (size_t)f.nks * per_ks;
-  const unsigned char* src[4];
-  src[0] = f.packed + i * per_occ + (size_t)A * CHUNK;
-  src[1] = f.packed + j * per_occ + (size_t)Bk * CHUNK;
-  src[2] = f.packed + j * per_occ + (size_t)A * CHUNK;
-  src[3] = f.packed + i * per_occ + (size_t)Bk * CHUNK;
-  const int64_t nst = f.nks / Pol::KSTEPS;
+  const size_t stage_stride = (size_t)KS * per_ks;
+  const unsigned char* s0 = f.packed + i * per_occ + (size_t)A * CHUNK;
+  const unsigned char* s1 = f.packed + j * per_occ + (size_t)Bk * CHUNK;
+  const unsigned char* s2 = f.packed + j * per_occ + (size_t)A * CHUNK;
+  const unsigned char* s3 = f.packed + i * per_occ + (size_t)Bk * CHUNK;
+  const int64_t nst = f.nks / KS;
 
-  // stage st -> LDS buffer dst (wave-uniform destinations, per-lane sources).
-  // Every wave issues exactly NI transfers per stage (all four slabs, also for
-  // i == j) so that the wait counts below are compile-time constants.
+  // One buffer resource per slab, covering the rest of that occupied
+  // orbital's packed block (per_occ < 4 GiB is checked on the host; a transfer
+  // outside the record range would read zeros, never other memory).
+  // Transfer n of this wave: LDS bytes [(4n + wave) KB, +1 KB) of the stage,
+  // from slab op (resource rs[n]) at per-lane offset voff[n] plus the
+  // wave-uniform stage offset.
+  const __amdgpu_buffer_rsrc_t r0 = slab_rsrc(s0, per_occ - (size_t)A * CHUNK);
+  const __amdgpu_buffer_rsrc_t r1 = slab_rsrc(s1, per_occ - (size_t)Bk * CHUNK);
+  const __amdgpu_buffer_rsrc_t r2 = slab_rsrc(s2, per_occ - (size_t)A * CHUNK);
+  const __amdgpu_buffer_rsrc_t r3 = slab_rsrc(s3, per_occ - (size_t)Bk * CHUNK);
+  __amdgpu_buffer_rsrc_t rs[NI];
+  uint32_t voff[NI];
+#pragma unroll
+  for (int n = 0; n < NI; n++) {
+    const int wb = (n * 4 + wave) * 1024;
+    const int op = wb / OPB;
+    const int rem = wb - op * OPB + lane * 16;
+    voff[n] = (uint32_t)((rem / CHUNK) * per_ks + (rem % CHUNK));
+    rs[n] = op == 0 ? r0 : op == 1 ? r1 : op == 2 ? r2 : r3;
+  }
   auto issue = [&](int64_t st, unsigned char* dst) {
+    const uint32_t so = (uint32_t)((size_t)st * stage_stride);
 #pragma unroll
-    for (int n = 0; n < NI; n++) {
-      const int wb = (n * 4 + wave) * 1024;
-      const int op = wb / OPB;
-      const int rem = wb - op * OPB + lane * 16;
-      const int ks = rem / CHUNK, r = rem % CHUNK;
-      const unsigned char* g = src[op] + (size_t)(st * Pol::KSTEPS + ks) * per_ks + r;
-      __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)(dst + wb), 16, 0, 0);
-    }
+    for (int n = 0; n < NI; n++)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[n], (lds_ptr_t)(dst + (n * 4 + wave) * 1024), 16, voff[n], so, 0,
+                                               0);
   };
-  auto compute = [&](const unsigned char* stage) {
+
+  struct Ops {
+    frag_t r1[KS][FW][Pol::P], c1[KS][FW][Pol::P], r2[KS][FW][Pol::P], c2[KS][FW][Pol::P];
+  };
+  auto frag = [&](const unsigned char* stage, int ks, int slab, int blk, int pl) {
+    return *reinterpret_cast<const frag_t*>(stage + ks * CHUNK + slab * OPB + (blk * Pol::P + pl) * FB +
+                                            lane * Pol::LB);
+  };
+  // all four slabs are read also for i == j (uniform wait counts; 1/nocc of
+  // the units)
+  auto load = [&](const unsigned char* stage, Ops& o) {
 #pragma unroll
-    for (int ks = 0; ks < Pol::KSTEPS; ks++) {
-      typename Pol::frag_t ra[FW][Pol::P], cb[FW][Pol::P];
-      const unsigned char* base = stage + ks * CHUNK;
+    for (int ks = 0; ks < KS; ks++)
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
         for (int pl = 0; pl < Pol::P; pl++) {
-          ra[x][pl] = *reinterpret_cast<const typename Pol::frag_t*>(
-              base + 0 * OPB + ((wr * FW + x) * Pol::P + pl) * FB + lane * Pol::LB);
-          cb[x][pl] = *reinterpret_cast<const typename Pol::frag_t*>(
-              base + 1 * OPB + ((wc * FW + x) * Pol::P + pl) * FB + lane * Pol::LB);
+          o.r1[ks][x][pl] = frag(stage, ks, 0, wr * FW + x, pl);
+          o.c1[ks][x][pl] = frag(stage, ks, 1, wc * FW + x, pl);
+          o.r2[ks][x][pl] = frag(stage, ks, 2, wr * FW + x, pl);
+          o.c2[ks][x][pl] = frag(stage, ks, 3, wc * FW + x, pl);
         }
+  };
+  // Tile 2 is formed also for i == j (where the epilogue uses tile 1 twice):
+  // a branch would split the step into two scheduling regions and 1/nocc of
+  // the units does not pay for that.
+  auto mma = [&](const Ops& o) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
-        for (int y = 0; y < FW; y++) Pol::mma(acc1[x][y], ra[x], cb[y]);
-      if (!same_occ) {
+        for (int y = 0; y < FW; y++) Pol::mma(acc1[x][y], o.r1[ks][x], o.c1[ks][y]);
 #pragma unroll
-        for (int x = 0; x < FW; x++)
+      for (int x = 0; x < FW; x++)
 #pragma unroll
-          for (int pl = 0; pl < Pol::P; pl++) {
-            ra[x][pl] = *reinterpret_cast<const typename Pol::frag_t*>(
-                base + 2 * OPB + ((wr * FW + x) * Pol::P + pl) * FB + lane * Pol::LB);
-            cb[x][pl] = *reinterpret_cast<const typename Pol::frag_t*>(
-                base + 3 * OPB + ((wc * FW + x) * Pol::P + pl) * FB + lane * Pol::LB);
-          }
+        for (int y = 0; y < FW; y++) Pol::mma(acc2[x][y], o.r2[ks][x], o.c2[ks][y]);
+    }
+  };
+  // Issue is in order per wave: a DMA or LDS read that waits for a full queue
+  // also holds back every MFMA behind it.  Interleaving one DMA and a share of
+  // the LDS reads between groups of MFMAs keeps both queues short and the
+  // matrix pipe fed while the stage transfer proceeds.
+  constexpr int NLDS = 4 * KS * FW * Pol::P;
+  constexpr int NMMA = 2 * KS * FW * FW * Pol::MMA_PER_CALL;
+  constexpr int GL = (NLDS + NI - 1) / NI, GM = (NMMA + NI - 1) / NI;
+  auto step = [&](int64_t st, Ops& cur, Ops& nxt, unsigned char* freed, const unsigned char* next) {
+    RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
+    pipeline_barrier();
+    issue(st + 4 < nst ? st + 4 : nst - 1, freed);
+    load(next, nxt);
+    mma(cur);
 #pragma unroll
-        for (int x = 0; x < FW; x++)
-#pragma unroll
-          for (int y = 0; y < FW; y++) Pol::mma(acc2[x][y], ra[x], cb[y]);
-      }
+    for (int g = 0; g < NI; g++) {
+      __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);   // VMEM read (the DMA)
+      __builtin_amdgcn_sched_group_barrier(0x0100, GL, 0);  // LDS reads
+      __builtin_amdgcn_sched_group_barrier(0x0008, GM, 0);  // MFMA
     }
   };
 
-  // step(st): wait for stage st (stage st+1 stays in flight), barrier (stage
-  // st visible to all waves; the buffer of stage st-1 is free), issue stage
-  // st+2 into that buffer, consume stage st.  The issue is unconditional (the
-  // last two steps re-fetch stage nst-1 into a free buffer): a branch around
-  // the DMA makes the compiler lose track of which buffer is in flight and
-  // wait for all DMA before every LDS read.
-  auto step = [&](int64_t st, const unsigned char* cur, unsigned char* next2) {
-    RIMP2_WAIT_VMCNT(NI);
-    pipeline_barrier();
-    issue(st + 2 < nst ? st + 2 : nst - 1, next2);
-    compute(cur);
-  };
+  Ops ra, rb;
   issue(0, buf0);
   issue(1, buf1);
-  // nst is a multiple of 3 (the auxiliary padding guarantees it), so the
-  // three-step body needs no early exits.
-  for (int64_t st = 0; st < nst; st += 3) {
-    step(st, buf0, buf2);
-    step(st + 1, buf1, buf0);
-    step(st + 2, buf2, buf1);
+  issue(2, buf2);
+  issue(3, buf3);
+  RIMP2_WAIT_VMCNT(3 * NI);
+  pipeline_barrier();
+  load(buf0, ra);
+  // nst is a multiple of 4 (the auxiliary padding guarantees it)
+  for (int64_t st = 0; st < nst; st += 4) {
+    step(st, ra, rb, buf0, buf1);
+    step(st + 1, rb, ra, buf1, buf2);
+    step(st + 2, ra, rb, buf2, buf3);
+    step(st + 3, rb, ra, buf3, buf0);
   }
-  RIMP2_WAIT_VMCNT(0);  // the trailing re-fetches have landed before LDS is reused
+  RIMP2_WAIT_VMCNT_LGKM0(0);  // trailing re-fetches and the unused last read are done before LDS is reused
 
   // ---- epilogue: energy terms in FP64 ----
   const double ei = f.eps_o[i], ej = f.eps_o[j];
@@ -329,11 +387,12 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   return res;
 }
 
-// Declares the three stage buffers of a pair kernel instantiation.
+// Declares the four stage buffers of a pair kernel instantiation.
 #define RIMP2_PAIR_STAGE_BUFFERS(Pol, TM)                                            \
   __shared__ __align__(16) unsigned char rimp2_buf0[pair_stage_bytes<Pol, TM>()];  \
   __shared__ __align__(16) unsigned char rimp2_buf1[pair_stage_bytes<Pol, TM>()];  \
-  __shared__ __align__(16) unsigned char rimp2_buf2[pair_stage_bytes<Pol, TM>()]
+  __shared__ __align__(16) unsigned char rimp2_buf2[pair_stage_bytes<Pol, TM>()];  \
+  __shared__ __align__(16) unsigned char rimp2_buf3[pair_stage_bytes<Pol, TM>()]
 
 }  // namespace pairk
 }  // namespace rimp2
