@@ -192,68 +192,103 @@ void gemm_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, cons
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block: factor A[k0:k0+kb, k0:k0+kb] = L11 L11^T in LDS, write L11
-// back (zeros above the diagonal) and L11^{-1} into Linv at the same block.
+// Orders this wave's LDS stores before its following LDS loads as seen by
+// the other lanes (no instruction at wavefront scope; constrains the compiler).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block: factor A[k0:k0+kb, k0:k0+kb] = L11 L11^T, write L11 back
+// (zeros above the diagonal) and L11^{-1} into Linv at the same block.
 // status bit 1 is set on a non-positive pivot; status[1] receives the global
 // index of the first failing pivot (1-based).
-// One barrier per column: the factorisation runs as an unscaled LDL^T-style
-// right-looking update, A[r][q] -= A[r][c] A[q][c] / d_c, with the column
-// scaling by 1/sqrt(d_c) deferred to the end; the inverse sweep scales row c
-// of X on the fly in the same way.  Thread t owns row t%64 and columns
-// t/64 + 4k of each update.
+//
+// The block is padded to 64 x 64 with the identity, so there is no ragged
+// case.  Factorisation: one wave, lane r holds row r in registers, the column
+// loop is unrolled (static register indices) and runs the unscaled update
+//     a_r[q] -= (a_r[c] / d_c) * a_q[c],   q > c,   d_c = a_c[c];
+// column c is published through LDS (lane r writes a_r[c], every lane then
+// reads the entries it needs with broadcast loads, two per instruction).  A
+// wave's LDS accesses complete in order, so a wavefront-scope fence (no
+// instruction) is all the ordering needed: the whole dependency chain lives
+// in one wave.  The scaling by 1/sqrt(d_c) is applied once at the end.
+// Inverse, same wave: lane j forward-substitutes column j of X = L11^{-1},
+//     x_j[c] = (delta_cj - sum_{k<c} L[c][k] x_j[k]) / L[c][c],
+// with the row L[c][0..c) read by broadcast loads (every lane needs the same
+// entries), so the arithmetic is lane-local; X goes through LDS to be stored
+// with coalesced writes by all four waves.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
                                                          int64_t k0, int kb, int* status) {
-  __shared__ double L[NB][NB + 1];
-  __shared__ double X[NB][NB + 1];
-  __shared__ double dg[NB];
-  const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int rr = e % NB, cc = e / NB;
-    L[rr][cc] = (rr < kb && cc < kb && cc <= rr) ? A[(k0 + rr) + lda * (k0 + cc)] : 0.0;
-  }
-  __syncthreads();
-  bool bad = false;
-  for (int c = 0; c < kb; c++) {
-    double d = L[c][c];  // final: every update of column c's pivot is done
-    if (!(d > 0.0)) {
-      if (tid == 0 && !bad) {
-        atomicCAS(status + 1, 0, (int)(k0 + c + 1));
-        atomicOr(status, 2);
+  __shared__ double Ls[NB][NB + 1];
+  __shared__ double Xs[NB][NB + 1];
+  __shared__ __align__(16) double colb[2][NB];
+  __shared__ __align__(16) double invd[NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (wave == 0) {
+    const int r = lane;
+    double a[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      a[q] = (r < kb && q < kb) ? (q <= r ? A[(k0 + r) + lda * (k0 + q)] : 0.0) : (r == q ? 1.0 : 0.0);
+    double piv = 1.0;
+    int bad_at = 0;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+      double* col = colb[c & 1];
+      col[r] = a[c];
+      wave_lds_sync();
+      double d = col[c];
+      if (!(d > 0.0)) {
+        if (bad_at == 0) bad_at = c + 1;
+        d = 1.0;
       }
-      bad = true;
-      d = 1.0;
+      if (r == c) piv = d;
+      const double lrc = a[c] / d;
+#pragma unroll
+      for (int q = c + 1; q < NB; q++) a[q] = fma(-lrc, col[q], a[q]);
     }
-    const double inv_d = 1.0 / d;
-    if (r > c && r < kb) {
-      const double lrc = L[r][c] * inv_d;
-      for (int q = c + 1 + ((g - (c + 1) % 4 + 4) % 4); q <= r; q += 4) L[r][q] -= lrc * L[q][c];
+    const double sp = sqrt(piv);
+    invd[r] = 1.0 / sp;
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < NB; q++) Ls[r][q] = q < r ? a[q] * invd[q] : (q == r ? sp : 0.0);
+    wave_lds_sync();
+    if (bad_at && r == 0) {
+      atomicCAS(status + 1, 0, (int)(k0 + bad_at));
+      atomicOr(status, 2);
     }
-    if (tid == 0) dg[c] = d;
-    __syncthreads();
+    // lane j: column j of L11^{-1}
+    const int j = lane;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+      double s0 = (c == j) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+      for (int k = 0; k + 1 < c; k += 2) {
+        s0 = fma(-Ls[c][k], x[k], s0);
+        s1 = fma(-Ls[c][k + 1], x[k + 1], s1);
+      }
+      if (c & 1) s0 = fma(-Ls[c][c - 1], x[c - 1], s0);
+      x[c] = (s0 + s1) * invd[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; c++) Xs[c][j] = x[c];
   }
-  // L[r][c] = A_rc / sqrt(d_c) below the diagonal, sqrt(d_c) on it
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int rr = e % NB, cc = e / NB;
-    if (rr < kb && cc < kb && cc <= rr) L[rr][cc] = (rr == cc) ? sqrt(dg[cc]) : L[rr][cc] / sqrt(dg[cc]);
-  }
-  for (int e = tid; e < NB * NB; e += 256) X[e % NB][e / NB] = (e % NB == e / NB) ? 1.0 : 0.0;
   __syncthreads();
-  // L11 X = I by a right-looking sweep: at step c, row c of X (scaled by
-  // 1/L[c][c] on the fly) is eliminated from the rows below; rows are scaled
-  // at the end
-  for (int c = 0; c < kb; c++) {
-    const double inv = 1.0 / L[c][c];
-    if (r > c && r < kb) {
-      const double lrc = L[r][c] * inv;
-      for (int j = g; j <= c; j += 4) X[r][j] -= lrc * X[c][j];
+  {
+    const int r = lane;
+#pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+      const int q = wave + 4 * t;
+      if (r < kb && q < kb) {
+        Linv[(k0 + r) + ldi * (k0 + q)] = q <= r ? Xs[r][q] : 0.0;
+        A[(k0 + r) + lda * (k0 + q)] = Ls[r][q];
+      }
     }
-    __syncthreads();
-  }
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int rr = e % kb, cc = e / kb;
-    A[(k0 + rr) + lda * (k0 + cc)] = (cc <= rr) ? L[rr][cc] : 0.0;
-    Linv[(k0 + rr) + ldi * (k0 + cc)] = (cc <= rr) ? X[rr][cc] / L[rr][rr] : 0.0;
   }
 }
 
