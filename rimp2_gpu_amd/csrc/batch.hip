@@ -92,8 +92,8 @@ __global__ __launch_bounds__(256) void batch_pair_kernel(BatchParams p) {
     const int64_t pidx = local / fr.ntp, tp = local % fr.ntp;
     int64_t i, j, A, Bk;
     tri_decode(pidx, i, j);
-    tri_decode(tp, A, Bk);
-    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+    const int kind = unit_decode(tp, fr.nb, packs_diagonal<Pol, TM>(), A, Bk);
+    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
     if (threadIdx.x == 0) p.partials[wu] = e;
   }
 }
@@ -261,7 +261,7 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
     x.f.nrb = g[f].nrb;
     x.nocc = b->nocc[f];
     x.nb = g[f].nvp / tile;
-    x.ntp = x.nb * (x.nb + 1) / 2;
+    x.ntp = pair_units(x.nb, pair_packs_diagonal(b->mode, tile));
     x.npairs = x.nocc * (x.nocc + 1) / 2;
     x.wu_begin = wu;
     x.pair_off = pair_off[f];

@@ -32,7 +32,7 @@ struct PairParams {
   double* partials;             // [npairs * ntp]
   const int64_t* pair_list;     // optional list of unique pair indices (j*(j+1)/2+i)
   int64_t nocc;
-  int64_t nb, ntp;              // virtual blocks, tile pairs per occupied pair
+  int64_t nb, ntp;              // virtual blocks, work units per occupied pair
   int64_t npairs;               // pairs processed (list length or all)
   int64_t nwu;                  // npairs * ntp
   int64_t chunk;                // work units per XCD slot
@@ -50,8 +50,8 @@ __global__ __launch_bounds__(256) void pair_kernel(PairParams p) {
   const int64_t pidx = wu / p.ntp, tp = wu % p.ntp;
   int64_t i, j, A, Bk;
   tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
-  tri_decode(tp, A, Bk);  // A <= Bk
-  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+  const int kind = unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk);
+  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
   if (threadIdx.x == 0) p.partials[wu] = e;
 }
 
@@ -146,6 +146,17 @@ int choose_tile(int mode, int64_t nvirt, int64_t nocc) {
   }
 }
 
+// Whether the pair kernel of (mode, tile) packs two diagonal blocks per unit.
+bool pair_packs_diagonal(int mode, int tile) {
+  switch (mode) {
+    case RIMP2_MODE_FP64: return tile == 32 ? packs_diagonal<PolF64, 32>() : packs_diagonal<PolF64, 64>();
+    case RIMP2_MODE_FP32: return tile == 64 ? packs_diagonal<PolF32, 64>() : packs_diagonal<PolF32, 128>();
+    case RIMP2_MODE_BF16X3: return tile == 64 ? packs_diagonal<PolBF16x3, 64>() : packs_diagonal<PolBF16x3, 128>();
+    case RIMP2_MODE_I8S4: return packs_diagonal<PolI8<4, 4>, 64>();
+    default: return packs_diagonal<PolI8<5, 4>, 64>();
+  }
+}
+
 static void dispatch_pair(int mode, int tile, hipStream_t s, const PairParams& p) {
   switch (mode) {
     case RIMP2_MODE_FP64:
@@ -178,7 +189,7 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
   hipStream_t s = c->stream;
   const int tile = c->tile;
   const int64_t nb = g.nvp / tile;
-  const int64_t ntp = nb * (nb + 1) / 2;
+  const int64_t ntp = pair_units(nb, pair_packs_diagonal(g.mode, tile));
   const int64_t npairs = d_pair_list ? npairs_listed : nocc * (nocc + 1) / 2;
   double* evp = static_cast<double*>(c->eps_v.ensure((size_t)g.nvp * sizeof(double)));
   hipLaunchKernelGGL(pad_eps_v_kernel, dim3((unsigned)ceil_div(g.nvp, 256)), dim3(256), 0, s, d_eps_v, nvirt, g.nvp,

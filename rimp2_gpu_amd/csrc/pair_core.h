@@ -150,6 +150,46 @@ constexpr int pair_stage_bytes() {
   return 4 * Pol::KSTEPS * (TM / Pol::RB) * Pol::P * 64 * Pol::LB;
 }
 
+// A diagonal block's exchange tile V^{ji}[A,A] is the transpose of V^{ij}[A,A],
+// so where the FP64 tile fits one stage buffer (for the epilogue's LDS
+// transpose) two diagonal blocks share one work unit and no MFMA forms the
+// transpose.  Units per occupied pair over nb virtual blocks: the nb(nb-1)/2
+// off-diagonal tile pairs A < B, then ceil(nb/2) units of two diagonal
+// blocks; without packing, the nb(nb+1)/2 tile pairs A <= B.
+template <class Pol, int TM>
+constexpr bool packs_diagonal() {
+  return TM * TM * 8 <= pair_stage_bytes<Pol, TM>();
+}
+
+__host__ __device__ inline int64_t pair_units(int64_t nb, bool packed) {
+  return packed ? nb * (nb - 1) / 2 + (nb + 1) / 2 : nb * (nb + 1) / 2;
+}
+
+enum UnitKind { UNIT_TILE_PAIR = 0, UNIT_TWO_DIAG = 1, UNIT_ONE_DIAG = 2 };
+
+// Unit u of an occupied pair -> blocks (A, B) and its kind: a tile pair
+// (A < B, or A <= B without packing), two diagonal blocks A and B = A + 1, or
+// the last diagonal block A alone (odd nb; B = A, its second tile unused).
+__device__ inline int unit_decode(int64_t u, int64_t nb, bool packed, int64_t& A, int64_t& B) {
+  if (!packed) {
+    tri_decode(u, A, B);
+    return UNIT_TILE_PAIR;
+  }
+  const int64_t noff = nb * (nb - 1) / 2;
+  if (u < noff) {
+    tri_decode(u, A, B);
+    B += 1;
+    return UNIT_TILE_PAIR;
+  }
+  A = 2 * (u - noff);
+  B = A + 1;
+  if (B >= nb) {
+    B = A;
+    return UNIT_ONE_DIAG;
+  }
+  return UNIT_TWO_DIAG;
+}
+
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 // Workgroup barrier for the operand pipeline.  __syncthreads() carries a
@@ -204,7 +244,7 @@ __device__ inline __amdgpu_buffer_rsrc_t slab_rsrc(const unsigned char* base, si
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
 // offset, so each transfer is one buffer load plus the M0 write.
 template <class Pol, int TM>
-__device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk,
+__device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
                             unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3) {
   constexpr int FPB = TM / Pol::RB;           // fragments per block side
@@ -235,14 +275,19 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
         Pol::zero(acc2[x][y][d]);
       }
 
-  // operand slabs: 0 = B_i[A] (rows), 1 = B_j[B] (cols), 2 = B_j[A], 3 = B_i[B]
+  // operand slabs (rows, cols of tile 1; rows, cols of tile 2):
+  //   tile pair:       B_i[A], B_j[B]; B_j[A], B_i[B]   (V^{ij}[A,B], V^{ji}[A,B])
+  //   diagonal blocks: B_i[A], B_j[A]; B_i[B], B_j[B]   (V^{ij}[A,A], V^{ij}[B,B])
+  const bool tile_pair = (kind == UNIT_TILE_PAIR);
+  const int64_t o1 = tile_pair ? j : i, o2 = tile_pair ? i : j;  // orbitals of slabs 2, 3
+  const int64_t b1 = tile_pair ? Bk : A, b2 = tile_pair ? A : Bk;  // blocks of slabs 1, 2
   const size_t per_ks = (size_t)f.nrb * Pol::P * FB;
   const size_t per_occ = (size_t)f.nks * per_ks;
   const size_t stage_stride = (size_t)KS * per_ks;
   const unsigned char* s0 = f.packed + i * per_occ + (size_t)A * CHUNK;
-  const unsigned char* s1 = f.packed + j * per_occ + (size_t)Bk * CHUNK;
-  const unsigned char* s2 = f.packed + j * per_occ + (size_t)A * CHUNK;
-  const unsigned char* s3 = f.packed + i * per_occ + (size_t)Bk * CHUNK;
+  const unsigned char* s1 = f.packed + j * per_occ + (size_t)b1 * CHUNK;
+  const unsigned char* s2 = f.packed + o1 * per_occ + (size_t)b2 * CHUNK;
+  const unsigned char* s3 = f.packed + o2 * per_occ + (size_t)Bk * CHUNK;
   const int64_t nst = f.nks / KS;
 
   // One buffer resource per slab, covering the rest of that occupied
@@ -252,8 +297,8 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // from slab op (resource rs[n]) at per-lane offset voff[n] plus the
   // wave-uniform stage offset.
   const __amdgpu_buffer_rsrc_t r0 = slab_rsrc(s0, per_occ - (size_t)A * CHUNK);
-  const __amdgpu_buffer_rsrc_t r1 = slab_rsrc(s1, per_occ - (size_t)Bk * CHUNK);
-  const __amdgpu_buffer_rsrc_t r2 = slab_rsrc(s2, per_occ - (size_t)A * CHUNK);
+  const __amdgpu_buffer_rsrc_t r1 = slab_rsrc(s1, per_occ - (size_t)b1 * CHUNK);
+  const __amdgpu_buffer_rsrc_t r2 = slab_rsrc(s2, per_occ - (size_t)b2 * CHUNK);
   const __amdgpu_buffer_rsrc_t r3 = slab_rsrc(s3, per_occ - (size_t)Bk * CHUNK);
   __amdgpu_buffer_rsrc_t rs[NI];
   uint32_t voff[NI];
@@ -352,27 +397,68 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // ---- epilogue: energy terms in FP64 ----
   const double ei = f.eps_o[i], ej = f.eps_o[j];
   double s = 0.0, c = 0.0;
+  if (tile_pair) {
 #pragma unroll
-  for (int x = 0; x < FW; x++)
+    for (int x = 0; x < FW; x++)
 #pragma unroll
-    for (int y = 0; y < FW; y++)
+      for (int y = 0; y < FW; y++)
 #pragma unroll
-      for (int r = 0; r < Pol::NE; r++) {
-        const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
-        const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
-        double v1 = Pol::value(acc1[x][y], r);
-        double v2 = same_occ ? v1 : Pol::value(acc2[x][y], r);
-        if (is_i8<Pol>::value) {
-          v1 *= f.sigma[i * f.nvp + a] * f.sigma[j * f.nvp + b];
-          if (!same_occ) v2 *= f.sigma[j * f.nvp + a] * f.sigma[i * f.nvp + b];
-          else v2 = v1;
+        for (int r = 0; r < Pol::NE; r++) {
+          const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
+          const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
+          double v1 = Pol::value(acc1[x][y], r);
+          double v2 = same_occ ? v1 : Pol::value(acc2[x][y], r);
+          if (is_i8<Pol>::value) {
+            v1 *= f.sigma[i * f.nvp + a] * f.sigma[j * f.nvp + b];
+            if (!same_occ) v2 *= f.sigma[j * f.nvp + a] * f.sigma[i * f.nvp + b];
+            else v2 = v1;
+          }
+          const double inv = 1.0 / (ei + ej - f.eps_v[a] - f.eps_v[b]);
+          double e;
+          if (same_blk) e = v1 * (2.0 * v1 - v2);
+          else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
+          kahan_add(s, c, e * inv);
         }
-        const double inv = 1.0 / (ei + ej - f.eps_v[a] - f.eps_v[b]);
-        double e;
-        if (same_blk) e = v1 * (2.0 * v1 - v2);
-        else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
-        kahan_add(s, c, e * inv);
-      }
+  } else if constexpr (packs_diagonal<Pol, TM>()) {
+    // diagonal block X: eps += sum_ab V_ab (2 V_ab - V_ba) / Delta_ab with
+    // V_ba read from an LDS copy of the tile (XOR-swizzled columns: the row
+    // writes and the transposed reads are both bank-conflict free)
+    double* T = reinterpret_cast<double*>(buf0);
+    auto swz = [](int a, int b) { return a * TM + (b ^ (a & 31)); };
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (t == 1 && kind == UNIT_ONE_DIAG) break;
+      const int64_t X = t ? Bk : A;
+      double v[FW][FW][Pol::NE];
+      __syncthreads();  // every wave's stage DMA has landed / the previous tile is read
+#pragma unroll
+      for (int x = 0; x < FW; x++)
+#pragma unroll
+        for (int y = 0; y < FW; y++)
+#pragma unroll
+          for (int r = 0; r < Pol::NE; r++) {
+            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
+            double w = Pol::value(t ? acc2[x][y] : acc1[x][y], r);
+            if (is_i8<Pol>::value) w *= f.sigma[i * f.nvp + X * TM + al] * f.sigma[j * f.nvp + X * TM + bl];
+            v[x][y][r] = w;
+            T[swz(al, bl)] = w;
+          }
+      __syncthreads();
+#pragma unroll
+      for (int x = 0; x < FW; x++)
+#pragma unroll
+        for (int y = 0; y < FW; y++)
+#pragma unroll
+          for (int r = 0; r < Pol::NE; r++) {
+            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
+            const double vab = v[x][y][r], vba = T[swz(bl, al)];
+            const double inv = 1.0 / (ei + ej - f.eps_v[X * TM + al] - f.eps_v[X * TM + bl]);
+            kahan_add(s, c, vab * (2.0 * vab - vba) * inv);
+          }
+    }
+  }
   double tot = s - c;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);

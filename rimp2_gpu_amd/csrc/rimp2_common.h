@@ -121,6 +121,8 @@ int mode_slices(int mode);
 
 // --- pair kernel (pair.hip) ---
 int choose_tile(int mode, int64_t nvirt, int64_t nocc);
+// Whether the pair kernel of (mode, tile) carries two diagonal blocks per unit.
+bool pair_packs_diagonal(int mode, int tile);
 void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const double* scales, int64_t nocc,
                           int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
                           double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
