@@ -180,17 +180,18 @@ def main():
         tte_host = time.perf_counter() - t0
 
     pair_ms = k_mode["pair"][0] / max(1, k_mode["pair"][1])
-    achieved = Fp / (pair_ms * 1e-3) / 1e12
-    peak = PEAK["fp64_tflops"] if a.mode == "fp64" else PEAK["fp32_mfma_tflops"]
+    # Roofline of the pair kernel against the peak of the MFMA type it runs:
+    # algorithmic work = F_pair times the slice / split products the mode needs
+    # per FP64-equivalent flop (int8 ops for the exact-slice modes, bf16 flops
+    # for the split-BF16 mode).
+    eff_tflops = Fp / (pair_ms * 1e-3) / 1e12
+    achieved = eff_tflops * NPROD[a.mode]
+    peak = {"fp64": PEAK["fp64_tflops"], "fp32": PEAK["fp32_mfma_tflops"], "bf16x3": PEAK["bf16_mfma_tflops"]}.get(
+        a.mode, PEAK["int8_mfma_tops"])
     roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak,
-                traffic=measured_traffic(a.mode, nocc, nvirt, naux))
-    hw = None
-    if a.mode.startswith("i8"):
-        ops = Fp * NPROD[a.mode] / (pair_ms * 1e-3) / 1e12
-        hw = dict(int8_tops=ops, peak_int8_tops=PEAK["int8_mfma_tops"], frac=ops / PEAK["int8_mfma_tops"])
-    elif a.mode == "bf16x3":
-        ops = Fp * 3 / (pair_ms * 1e-3) / 1e12
-        hw = dict(bf16_tflops=ops, peak_bf16_tflops=PEAK["bf16_mfma_tflops"], frac=ops / PEAK["bf16_mfma_tflops"])
+                traffic=measured_traffic(a.mode, nocc, nvirt, naux),
+                mfma_type={"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}.get(a.mode, "i8 (ops)"),
+                fp64_equivalent_tflops=eff_tflops)
 
     cpu = None
     E_ref = None
@@ -221,7 +222,6 @@ def main():
         "kernels_ms": {k: v[0] / max(1, v[1]) for k, v in k_mode.items()},
         "E_mode": E_mode,
         "roofline": roof,
-        "hw_mfma": hw,
         "cpu_baseline": cpu,
     }
     if res64:
