@@ -180,18 +180,19 @@ def main():
         tte_host = time.perf_counter() - t0
 
     pair_ms = k_mode["pair"][0] / max(1, k_mode["pair"][1])
-    # Roofline of the pair kernel against the peak of the MFMA type it runs:
-    # algorithmic work = F_pair times the slice / split products the mode needs
-    # per FP64-equivalent flop (int8 ops for the exact-slice modes, bf16 flops
-    # for the split-BF16 mode).
-    eff_tflops = Fp / (pair_ms * 1e-3) / 1e12
-    achieved = eff_tflops * NPROD[a.mode]
-    peak = {"fp64": PEAK["fp64_tflops"], "fp32": PEAK["fp32_mfma_tflops"], "bf16x3": PEAK["bf16_mfma_tflops"]}.get(
-        a.mode, PEAK["int8_mfma_tops"])
+    # Roofline of the contraction: F_pair / t against the FP32-matrix peak
+    # (FP64 peak for the fp64 mode).  Beside it, the rate of the MFMA type the
+    # mode executes: F_pair times the slice / split products per flop (int8 ops
+    # for the exact-slice modes, bf16 flops for split-BF16) against that peak.
+    achieved = Fp / (pair_ms * 1e-3) / 1e12
+    peak = PEAK["fp64_tflops"] if a.mode == "fp64" else PEAK["fp32_mfma_tflops"]
+    hw_rate = achieved * NPROD[a.mode]
+    hw_type, hw_peak = {"fp64": ("f64", PEAK["fp64_tflops"]), "fp32": ("f32", PEAK["fp32_mfma_tflops"]),
+                        "bf16x3": ("bf16", PEAK["bf16_mfma_tflops"])}.get(a.mode, ("i8", PEAK["int8_mfma_tops"]))
     roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak,
                 traffic=measured_traffic(a.mode, nocc, nvirt, naux),
-                mfma_type={"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}.get(a.mode, "i8 (ops)"),
-                fp64_equivalent_tflops=eff_tflops)
+                executed_mfma=dict(type=hw_type, rate=hw_rate, peak=hw_peak, frac=hw_rate / hw_peak,
+                                   unit="TOP/s" if hw_type == "i8" else "TFLOP/s"))
 
     cpu = None
     E_ref = None

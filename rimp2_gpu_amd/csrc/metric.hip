@@ -300,11 +300,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda,
 //                     a = 16 rb + l%16)  (the FP64 operand layout of pack.hip);
 // so every 512-byte fragment is contiguous in HBM and in LDS.  One workgroup
 // computes a 64 (Q rows) x 128 (virtuals of one occupied) tile of B = L^{-1} X
-// with a three-stage LDS ring fed by global_load_lds_dwordx4 (24 KB per
+// with a four-stage LDS ring fed by buffer_load_dwordx4 ... lds (24 KB per
 // 16-wide K stage); the K loop of a row tile stops at its last row (L^{-1} is
 // lower triangular).  Waves 2 x 2, wave tile 32 x 64 = 2 x 4 MFMA blocks.
 // ---------------------------------------------------------------------------
-constexpr int TR_TM = 64, TR_TN = 128, TR_KQ = 48;  // K padding: three 16-wide stages
+constexpr int TR_TM = 64, TR_TN = 128, TR_KQ = 64;  // K padding: four 16-wide stages
 
 __global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64_t MB, int64_t KS,
                                  double* __restrict__ Lp) {
@@ -332,26 +332,38 @@ __global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
   constexpr int A_STAGE = 4 * 4 * FB;         // 4 k-steps x 4 row blocks
   constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks
   constexpr int SB = A_STAGE + B_STAGE;       // 24 KB
-  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage
+  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage (0-1: L^{-1}, 2-5: X)
+  static_assert(A_STAGE == 2 * 4096, "transfers 0 and 1 of every wave carry the L^{-1} slab");
   __shared__ __align__(16) unsigned char buf0[SB];
   __shared__ __align__(16) unsigned char buf1[SB];
   __shared__ __align__(16) unsigned char buf2[SB];
+  __shared__ __align__(16) unsigned char buf3[SB];
   const int64_t bid = blockIdx.x;
   const int64_t wu = (bid & 7) * p.chunk + (bid >> 3);
   if (wu >= p.nwu) return;
   const int64_t mt = wu % p.MT, col = wu / p.MT, i = col / p.NT, nt = col % p.NT;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t m0 = mt * TR_TM;
+  // L^{-1} is lower triangular: the K loop of row tile mt stops at its last
+  // row (m0 + 64 is a whole number of four-stage groups; so is KS * 4)
   int64_t kend = m0 + TR_TM;
   if (kend > p.KS * 4) kend = p.KS * 4;
-  kend = ((kend + TR_KQ - 1) / TR_KQ) * TR_KQ;  // multiple of three stages
-  if (kend > p.KS * 4) kend = p.KS * 4;
-  const int64_t nst = kend / 16;  // KS*4 is a multiple of 48, so nst % 3 == 0
-  const unsigned char* abase = p.Lp + (size_t)(mt * 4) * FB;
-  const size_t a_ks = (size_t)p.MB * FB;
-  const unsigned char* bbase = p.Xp + ((size_t)i * p.KS * p.NRB + (size_t)nt * 8) * FB;
-  const size_t b_ks = (size_t)p.NRB * FB;
+  const int64_t nst = kend / 16;
+  const size_t a_ks = (size_t)p.MB * FB, b_ks = (size_t)p.NRB * FB;
+  const size_t a_off = (size_t)(mt * 4) * FB, b_off = (size_t)nt * 8 * FB;
+  const __amdgpu_buffer_rsrc_t ra =
+      pairk::slab_rsrc(p.Lp + a_off, (size_t)p.KS * a_ks - a_off);
+  const __amdgpu_buffer_rsrc_t rb =
+      pairk::slab_rsrc(p.Xp + (size_t)i * p.KS * b_ks + b_off, (size_t)p.KS * b_ks - b_off);
+  uint32_t voff[NI];
+#pragma unroll
+  for (int n = 0; n < NI; n++) {
+    const int wb = (n * 4 + wave) * 1024;
+    if (n < 2) voff[n] = (uint32_t)((wb / (4 * FB)) * a_ks + wb % (4 * FB) + lane * 16);
+    else voff[n] = (uint32_t)(((wb - A_STAGE) / (8 * FB)) * b_ks + (wb - A_STAGE) % (8 * FB) + lane * 16);
+  }
 
   v4d acc[2][4];
 #pragma unroll
@@ -360,51 +372,67 @@ __global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
     for (int y = 0; y < 4; y++) acc[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
 
   auto issue = [&](int64_t st, unsigned char* dst) {
+    const uint32_t sa = (uint32_t)((size_t)st * 4 * a_ks), sb = (uint32_t)((size_t)st * 4 * b_ks);
 #pragma unroll
-    for (int n = 0; n < NI; n++) {
-      const int wb = (n * 4 + wave) * 1024;
-      const unsigned char* g;
-      if (wb < A_STAGE) {
-        const int ks = wb / (4 * FB), off = wb % (4 * FB) + lane * 16;
-        g = abase + (size_t)(st * 4 + ks) * a_ks + off;
-      } else {
-        const int w2 = wb - A_STAGE;
-        const int ks = w2 / (8 * FB), off = w2 % (8 * FB) + lane * 16;
-        g = bbase + (size_t)(st * 4 + ks) * b_ks + off;
-      }
-      __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)(dst + wb), 16, 0, 0);
-    }
+    for (int n = 0; n < NI; n++)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(n < 2 ? ra : rb, (lds_ptr_t)(dst + (n * 4 + wave) * 1024), 16,
+                                               voff[n], n < 2 ? sa : sb, 0, 0);
   };
-  auto compute = [&](const unsigned char* stg) {
+  struct Ops {
+    double af[4][2], bf[4][4];
+  };
+  auto load = [&](const unsigned char* stg, Ops& o) {
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
-      double af[2], bf[4];
 #pragma unroll
       for (int x = 0; x < 2; x++)
-        af[x] = *reinterpret_cast<const double*>(stg + (ks * 4 + wr * 2 + x) * FB + lane * 8);
+        o.af[ks][x] = *reinterpret_cast<const double*>(stg + (ks * 4 + wr * 2 + x) * FB + lane * 8);
 #pragma unroll
       for (int y = 0; y < 4; y++)
-        bf[y] = *reinterpret_cast<const double*>(stg + A_STAGE + (ks * 8 + wc * 4 + y) * FB + lane * 8);
+        o.bf[ks][y] = *reinterpret_cast<const double*>(stg + A_STAGE + (ks * 8 + wc * 4 + y) * FB + lane * 8);
+    }
+  };
+  auto mma = [&](const Ops& o) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++)
 #pragma unroll
       for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+        for (int y = 0; y < 4; y++)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.af[ks][x], o.bf[ks][y], acc[x][y], 0, 0, 0);
+  };
+  // same schedule as the pair kernel: wait for the next stage and for this
+  // wave's reads of the current one, barrier, refill the freed buffer three
+  // stages ahead, read the next stage under the current stage's MFMAs, one
+  // DMA + four LDS reads per five or six MFMAs
+  auto step = [&](int64_t st, Ops& cur, Ops& nxt, unsigned char* freed, const unsigned char* next) {
+    RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
+    pipeline_barrier();
+    issue(st + 4 < nst ? st + 4 : nst - 1, freed);
+    load(next, nxt);
+    mma(cur);
+#pragma unroll
+    for (int g = 0; g < NI; g++) {
+      __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
     }
   };
-  auto step = [&](int64_t st, const unsigned char* cur, unsigned char* next2) {
-    RIMP2_WAIT_VMCNT(NI);
-    pipeline_barrier();
-    issue(st + 2 < nst ? st + 2 : nst - 1, next2);
-    compute(cur);
-  };
+  Ops oa, ob;
   issue(0, buf0);
   issue(1, buf1);
-  for (int64_t st = 0; st < nst; st += 3) {
-    step(st, buf0, buf2);
-    step(st + 1, buf1, buf0);
-    step(st + 2, buf2, buf1);
+  issue(2, buf2);
+  issue(3, buf3);
+  RIMP2_WAIT_VMCNT(3 * NI);
+  pipeline_barrier();
+  load(buf0, oa);
+  for (int64_t st = 0; st < nst; st += 4) {
+    step(st, oa, ob, buf0, buf1);
+    step(st + 1, ob, oa, buf1, buf2);
+    step(st + 2, oa, ob, buf2, buf3);
+    step(st + 3, ob, oa, buf3, buf0);
   }
-  RIMP2_WAIT_VMCNT(0);
+  RIMP2_WAIT_VMCNT_LGKM0(0);
   // epilogue: D layout d[r] = D[4r + l/16][l%16]
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -507,6 +535,10 @@ void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   p.NRB = round_up(nvirt, TR_TN) / 16;
   p.MT = round_up(n, TR_TM) / TR_TM;
   p.NT = p.NRB / 8;
+  // the transform kernel addresses L^{-1} and one occupied orbital's X with 32-bit offsets
+  if ((size_t)p.KS * p.MB * 512 > 0xFFFFFFFFull || (size_t)p.KS * p.NRB * 512 > 0xFFFFFFFFull)
+    fail(RIMP2_ERR_ARG, "metric transform operands exceed 4 GiB per slab (naux=%lld nvirt=%lld)", (long long)n,
+         (long long)nvirt);
   p.nwu = p.MT * p.NT * nocc;
   p.chunk = ceil_div(p.nwu, 8);
   PackGeom gx;
