@@ -37,7 +37,12 @@ constexpr int NB = 64;  // Cholesky / inverse panel width
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
                                                        const double* A, int64_t lda, const double* B, int64_t ldb,
-                                                       double beta, double* C, int64_t ldc) {
+                                                       double beta, double* C, int64_t ldc, int64_t sA,
+                                                       int64_t sB, int64_t sC) {
+  // batch member blockIdx.z (element strides sA, sB, sC between members)
+  A += (int64_t)blockIdx.z * sA;
+  B += (int64_t)blockIdx.z * sB;
+  C += (int64_t)blockIdx.z * sC;
   constexpr int BK = 16;
   constexpr int PA = BM + 16;  // As[k][m]
   constexpr int PB = BK + 1;   // Bs[n][k]
@@ -183,11 +188,12 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
 
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 void gemm_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
-              const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
-  if (M <= 0 || N <= 0) return;
-  dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN));
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t batch = 1, int64_t sA = 0,
+              int64_t sB = 0, int64_t sC = 0) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  dim3 grid((unsigned)ceil_div(M, BM), (unsigned)ceil_div(N, BN), (unsigned)batch);
   hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TRANSB, TRI_A, LOWER_C>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda,
-                     B, ldb, beta, C, ldc);
+                     B, ldb, beta, C, ldc, sA, sB, sC);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
@@ -487,20 +493,28 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
     // L^{-1} by recursive doubling over the 64-wide diagonal blocks whose
     // inverses potrf_diag already wrote:  for [[P,0],[Q,R]],
     // inv = [[P^-1, 0], [-R^-1 Q P^-1, R^-1]].
+    // The groups of one level are independent and (but for a ragged last
+    // one) equally shaped with constant pointer strides, so each level is one
+    // batched launch per product.
     ScopedTimer t(c, RIMP2_K_TRTRI);
     const int64_t nblk = ceil_div(n, NB);
     double* T = static_cast<double*>(c->work.ensure((size_t)(n * n) * sizeof(double)));
     for (int64_t w = 1; w < nblk; w *= 2) {
-      for (int64_t g = 0; g * 2 * w < nblk; g++) {
-        const int64_t lo = g * 2 * w * NB, mid = (g * 2 * w + w) * NB;
-        if (mid >= n) continue;
-        const int64_t hi = ((g * 2 * w + 2 * w) * NB < n) ? (g * 2 * w + 2 * w) * NB : n;
-        const int64_t sp = mid - lo, sr = hi - mid;
-        // T = Q * P^{-1}   (Q = L[mid:hi, lo:mid], P^{-1} = Linv[lo:mid, lo:mid])
-        gemm_f64<64, 64, false, false, false>(s, sr, sp, sp, 1.0, L + mid + n * lo, n, Li + lo + n * lo, n, 0.0, T,
-                                              sr);
-        // Linv[mid:hi, lo:mid] = -R^{-1} T   (R^{-1} lower triangular)
-        gemm_f64<64, 64, false, true, false>(s, sr, sp, sr, -1.0, Li + mid + n * mid, n, T, sr, 0.0,
+      const int64_t half = w * NB, span = 2 * half, nfull = n / span, gs = span * (1 + n);
+      if (nfull > 0) {
+        // T_g = Q_g P_g^{-1}   (Q = L[mid:hi, lo:mid], P^{-1} = Linv[lo:mid, lo:mid]; T_g is half x half)
+        gemm_f64<64, 64, false, false, false>(s, half, half, half, 1.0, L + half, n, Li, n, 0.0, T, half, nfull, gs,
+                                              gs, half * half);
+        // Linv[mid:hi, lo:mid] = -R_g^{-1} T_g   (R^{-1} lower triangular)
+        gemm_f64<64, 64, false, true, false>(s, half, half, half, -1.0, Li + half * (1 + n), n, T, half, 0.0,
+                                             Li + half, n, nfull, gs, half * half, gs);
+      }
+      const int64_t lo = nfull * span, mid = lo + half;
+      if (mid < n) {  // ragged last group
+        const int64_t sr = n - mid;
+        gemm_f64<64, 64, false, false, false>(s, sr, half, half, 1.0, L + mid + n * lo, n, Li + lo + n * lo, n, 0.0,
+                                              T, sr);
+        gemm_f64<64, 64, false, true, false>(s, sr, half, sr, -1.0, Li + mid + n * mid, n, T, sr, 0.0,
                                              Li + mid + n * lo, n);
       }
     }
