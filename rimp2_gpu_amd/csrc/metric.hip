@@ -482,9 +482,10 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
       // panel: L21 = A21 * L11^{-T}   (in place; one N tile covers kb <= 64)
       gemm_f64<64, 64, true, false, false>(s, m, kb, kb, 1.0, L + r0 + n * k0, n, Li + k0 + n * k0, n, 0.0,
                                            L + r0 + n * k0, n);
-      // trailing: A22 -= L21 L21^T  (lower tiles only)
-      gemm_f64<128, 128, true, false, true>(s, m, m, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
-                                            L + r0 + n * r0, n);
+      // trailing: A22 -= L21 L21^T  (lower tiles only; K = 64, so small
+      // tiles: the update is latency-bound and wants many workgroups)
+      gemm_f64<64, 64, true, false, true>(s, m, m, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
+                                          L + r0 + n * r0, n);
     }
     hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, s, L, n, n);
     RIMP2_HIP_CHECK(hipGetLastError());
