@@ -146,7 +146,7 @@ int rimp2_batch(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode
 int rimp2_batch_prepare(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode);
 int rimp2_batch_run(void* ctx, double* d_e_frag);
 int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed);
-/* Tile size chosen by the autotuner for the prepared batch (32, 64 or 128). */
+/* Tile size chosen by the autotuner for the prepared batch (32 or 64). */
 int rimp2_batch_tile(void* ctx, int64_t* tile);
 
 /* ---- pair-screening surrogate (BF16 MLP 16 -> 64 -> 64 -> 1, GELU) ---- */

@@ -75,7 +75,7 @@ struct BatchParams {
 };
 
 template <class Pol, int TM>
-__global__ __launch_bounds__(256) void batch_pair_kernel(BatchParams p) {
+__global__ __launch_bounds__(PAIR_THREADS) void batch_pair_kernel(BatchParams p) {
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
   const int64_t end = p.range[xcd + 1];
@@ -146,14 +146,14 @@ __global__ void pad_eps_kernel(const double* ev, int64_t nvirt, int64_t nvp, dou
 template <class Pol, int TM>
 int occupancy_of() {
   int blocks = 0;
-  RIMP2_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_pair_kernel<Pol, TM>, 256, 0));
+  RIMP2_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_pair_kernel<Pol, TM>, PAIR_THREADS, 0));
   return blocks < 1 ? 1 : blocks;
 }
 
 template <class Pol, int TM>
 void launch_batch_t(hipStream_t s, const BatchParams& p, int ncu) {
   const int grid = 8 * ((ncu * occupancy_of<Pol, TM>() + 7) / 8);  // whole XCD slots
-  hipLaunchKernelGGL((batch_pair_kernel<Pol, TM>), dim3(grid), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((batch_pair_kernel<Pol, TM>), dim3(grid), dim3(PAIR_THREADS), 0, s, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
@@ -165,11 +165,9 @@ void launch_batch(int mode, int tile, hipStream_t s, const BatchParams& p, int n
       break;
     case RIMP2_MODE_FP32:
       if (tile == 64) return launch_batch_t<PolF32, 64>(s, p, ncu);
-      if (tile == 128) return launch_batch_t<PolF32, 128>(s, p, ncu);
       break;
     case RIMP2_MODE_BF16X3:
       if (tile == 64) return launch_batch_t<PolBF16x3, 64>(s, p, ncu);
-      if (tile == 128) return launch_batch_t<PolBF16x3, 128>(s, p, ncu);
       break;
     case RIMP2_MODE_I8S4:
       if (tile == 64) return launch_batch_t<PolI8<4, 4>, 64>(s, p, ncu);
@@ -185,7 +183,7 @@ std::vector<int> tile_candidates(int mode) {
   switch (mode) {
     case RIMP2_MODE_FP64: return {32, 64};
     case RIMP2_MODE_FP32:
-    case RIMP2_MODE_BF16X3: return {64, 128};
+    case RIMP2_MODE_BF16X3: return {64};
     default: return {64};
   }
 }

@@ -39,7 +39,7 @@ struct PairParams {
 };
 
 template <class Pol, int TM>
-__global__ __launch_bounds__(256) void pair_kernel(PairParams p) {
+__global__ __launch_bounds__(PAIR_THREADS) void pair_kernel(PairParams p) {
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   // XCD-aware mapping: consecutive workgroups land on different XCDs
   // (round-robin dispatch), so give each XCD slot a contiguous run of work
@@ -128,7 +128,7 @@ __global__ void denom_check_kernel(const double* eo, int64_t nocc, const double*
 template <class Pol, int TM>
 void launch_pair(hipStream_t s, const PairParams& p) {
   const unsigned grid = (unsigned)(8 * p.chunk);
-  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3(grid), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3(grid), dim3(PAIR_THREADS), 0, s, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
@@ -141,7 +141,7 @@ int choose_tile(int mode, int64_t nvirt, int64_t nocc) {
   switch (mode) {
     case RIMP2_MODE_FP64: return nvirt <= 48 ? 32 : 64;
     case RIMP2_MODE_FP32:
-    case RIMP2_MODE_BF16X3: return nvirt <= 96 ? 64 : 128;
+    case RIMP2_MODE_BF16X3: return 64;  // a 128 tile's split wave groups would spill at two waves per SIMD
     default: return 64;
   }
 }
@@ -150,8 +150,8 @@ int choose_tile(int mode, int64_t nvirt, int64_t nocc) {
 bool pair_packs_diagonal(int mode, int tile) {
   switch (mode) {
     case RIMP2_MODE_FP64: return tile == 32 ? packs_diagonal<PolF64, 32>() : packs_diagonal<PolF64, 64>();
-    case RIMP2_MODE_FP32: return tile == 64 ? packs_diagonal<PolF32, 64>() : packs_diagonal<PolF32, 128>();
-    case RIMP2_MODE_BF16X3: return tile == 64 ? packs_diagonal<PolBF16x3, 64>() : packs_diagonal<PolBF16x3, 128>();
+    case RIMP2_MODE_FP32: return packs_diagonal<PolF32, 64>();
+    case RIMP2_MODE_BF16X3: return packs_diagonal<PolBF16x3, 64>();
     case RIMP2_MODE_I8S4: return packs_diagonal<PolI8<4, 4>, 64>();
     default: return packs_diagonal<PolI8<5, 4>, 64>();
   }
@@ -165,11 +165,9 @@ static void dispatch_pair(int mode, int tile, hipStream_t s, const PairParams& p
       break;
     case RIMP2_MODE_FP32:
       if (tile == 64) return launch_pair<PolF32, 64>(s, p);
-      if (tile == 128) return launch_pair<PolF32, 128>(s, p);
       break;
     case RIMP2_MODE_BF16X3:
       if (tile == 64) return launch_pair<PolBF16x3, 64>(s, p);
-      if (tile == 128) return launch_pair<PolBF16x3, 128>(s, p);
       break;
     case RIMP2_MODE_I8S4:
       if (tile == 64) return launch_pair<PolI8<4, 4>, 64>(s, p);

@@ -217,17 +217,26 @@ __device__ inline __amdgpu_buffer_rsrc_t slab_rsrc(const unsigned char* base, si
 // Same, and additionally every LDS read of this wave has returned (lgkmcnt 0).
 #define RIMP2_WAIT_VMCNT_LGKM0(n) __builtin_amdgcn_s_waitcnt((((n) & 15) | (((n) >> 4) << 14)) | 0x0070)
 
-// One work unit: occupied pair (i <= j), virtual blocks A <= Bk.  All 256
-// threads call it; the return value (the unit's energy partial) is valid in
-// thread 0.  Ends with a barrier, so the stage buffers may be reused at once.
+// Threads per pair workgroup: eight waves, four per tile.
+constexpr int PAIR_THREADS = 512;
+
+// One work unit of occupied pair (i <= j) (blocks A, Bk and the unit kind from
+// unit_decode).  All PAIR_THREADS threads call it; the return value (the
+// unit's energy partial) is valid in thread 0.  Ends with a barrier, so the
+// stage buffers may be reused at once.
+//
+// Waves 0-3 form tile 1, waves 4-7 tile 2, each group on a 2 x 2 grid of
+// wave tiles.  Splitting the tiles over two wave groups halves the
+// accumulators per wave (the exact-slice modes hold five int32 diagonals per
+// output) so that two waves share each SIMD: when one waits at a barrier or
+// behind a full memory queue, the other keeps the matrix pipe busy.
 //
 // Operand pipeline: four LDS stage buffers fed by buffer_load_dwordx4 ... lds
 // (each wave instruction moves 1 KB, no staging registers) and two register
 // sets per wave.  Step st:
 //   wait   own DMA of stage st+1 landed; own LDS reads of stage st finished
 //   barrier  stage st+1 is in LDS for every wave; buffer st%4 is free
-//   issue  stage st+4 -> buffer st%4           (three steps of latency cover:
-//          loaded L2-miss latency exceeds two steps of MFMA work)
+//   issue  stage st+4 -> buffer st%4           (three steps of latency cover)
 //   read   stage st+1 -> register set (st+1)&1 (LDS latency hidden ...)
 //   MFMA   stage st from register set st&1      (... behind these)
 // The body is unrolled four times (4 buffers, 2 register sets) so every
@@ -247,6 +256,7 @@ template <class Pol, int TM>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
                             unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3) {
+  constexpr int NW = PAIR_THREADS / 64;       // waves
   constexpr int FPB = TM / Pol::RB;           // fragments per block side
   constexpr int WT = TM / 2;                  // wave tile side
   constexpr int FW = WT / Pol::RB;            // fragments per wave side
@@ -254,26 +264,25 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   constexpr int CHUNK = FPB * Pol::P * FB;    // contiguous bytes per operand per k-step
   constexpr int OPB = Pol::KSTEPS * CHUNK;    // bytes per operand per stage
   constexpr int SB = 4 * OPB;                 // bytes per stage
-  constexpr int NI = SB / 4096;               // 1-KB DMA instructions per wave per stage
+  constexpr int NI = SB / (NW * 1024);        // 1-KB DMA instructions per wave per stage
   constexpr int KS = Pol::KSTEPS;
   using frag_t = typename Pol::frag_t;
   static_assert(FW >= 1, "wave tile smaller than one MFMA fragment");
-  static_assert(SB % 4096 == 0 && OPB % 1024 == 0, "stage must split into whole 1-KB wave transfers");
+  static_assert(SB % (NW * 1024) == 0 && OPB % 1024 == 0, "stage must split into whole 1-KB wave transfers");
+  static_assert(WT * WT * 8 <= SB, "the exchange hand-over needs one FP64 wave tile per stage buffer");
   const bool same_occ = (i == j), same_blk = (A == Bk);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1;
+  const int grp = wave >> 2;  // 0: tile 1, 1: tile 2
+  const int wr = (wave >> 1) & 1, wc = wave & 1;
 
-  typename Pol::acc_t acc1[FW][FW][Pol::ND], acc2[FW][FW][Pol::ND];
+  typename Pol::acc_t acc[FW][FW][Pol::ND];
 #pragma unroll
   for (int x = 0; x < FW; x++)
 #pragma unroll
     for (int y = 0; y < FW; y++)
 #pragma unroll
-      for (int d = 0; d < Pol::ND; d++) {
-        Pol::zero(acc1[x][y][d]);
-        Pol::zero(acc2[x][y][d]);
-      }
+      for (int d = 0; d < Pol::ND; d++) Pol::zero(acc[x][y][d]);
 
   // operand slabs (rows, cols of tile 1; rows, cols of tile 2):
   //   tile pair:       B_i[A], B_j[B]; B_j[A], B_i[B]   (V^{ij}[A,B], V^{ji}[A,B])
@@ -293,7 +302,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // One buffer resource per slab, covering the rest of that occupied
   // orbital's packed block (per_occ < 4 GiB is checked on the host; a transfer
   // outside the record range would read zeros, never other memory).
-  // Transfer n of this wave: LDS bytes [(4n + wave) KB, +1 KB) of the stage,
+  // Transfer n of this wave: LDS bytes [(NW n + wave) KB, +1 KB) of the stage,
   // from slab op (resource rs[n]) at per-lane offset voff[n] plus the
   // wave-uniform stage offset.
   const __amdgpu_buffer_rsrc_t r0 = slab_rsrc(s0, per_occ - (size_t)A * CHUNK);
@@ -304,7 +313,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   uint32_t voff[NI];
 #pragma unroll
   for (int n = 0; n < NI; n++) {
-    const int wb = (n * 4 + wave) * 1024;
+    const int wb = (n * NW + wave) * 1024;
     const int op = wb / OPB;
     const int rem = wb - op * OPB + lane * 16;
     voff[n] = (uint32_t)((rem / CHUNK) * per_ks + (rem % CHUNK));
@@ -314,19 +323,19 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     const uint32_t so = (uint32_t)((size_t)st * stage_stride);
 #pragma unroll
     for (int n = 0; n < NI; n++)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[n], (lds_ptr_t)(dst + (n * 4 + wave) * 1024), 16, voff[n], so, 0,
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs[n], (lds_ptr_t)(dst + (n * NW + wave) * 1024), 16, voff[n], so, 0,
                                                0);
   };
 
   struct Ops {
-    frag_t r1[KS][FW][Pol::P], c1[KS][FW][Pol::P], r2[KS][FW][Pol::P], c2[KS][FW][Pol::P];
+    frag_t r[KS][FW][Pol::P], c[KS][FW][Pol::P];
   };
+  // this wave's operands: rows from slab 2 grp, columns from slab 2 grp + 1
+  const int slab_r = 2 * grp, slab_c = 2 * grp + 1;
   auto frag = [&](const unsigned char* stage, int ks, int slab, int blk, int pl) {
     return *reinterpret_cast<const frag_t*>(stage + ks * CHUNK + slab * OPB + (blk * Pol::P + pl) * FB +
                                             lane * Pol::LB);
   };
-  // all four slabs are read also for i == j (uniform wait counts; 1/nocc of
-  // the units)
   auto load = [&](const unsigned char* stage, Ops& o) {
 #pragma unroll
     for (int ks = 0; ks < KS; ks++)
@@ -334,34 +343,24 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
       for (int x = 0; x < FW; x++)
 #pragma unroll
         for (int pl = 0; pl < Pol::P; pl++) {
-          o.r1[ks][x][pl] = frag(stage, ks, 0, wr * FW + x, pl);
-          o.c1[ks][x][pl] = frag(stage, ks, 1, wc * FW + x, pl);
-          o.r2[ks][x][pl] = frag(stage, ks, 2, wr * FW + x, pl);
-          o.c2[ks][x][pl] = frag(stage, ks, 3, wc * FW + x, pl);
+          o.r[ks][x][pl] = frag(stage, ks, slab_r, wr * FW + x, pl);
+          o.c[ks][x][pl] = frag(stage, ks, slab_c, wc * FW + x, pl);
         }
   };
-  // Tile 2 is formed also for i == j (where the epilogue uses tile 1 twice):
-  // a branch would split the step into two scheduling regions and 1/nocc of
-  // the units does not pay for that.
   auto mma = [&](const Ops& o) {
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
+    for (int ks = 0; ks < KS; ks++)
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
-        for (int y = 0; y < FW; y++) Pol::mma(acc1[x][y], o.r1[ks][x], o.c1[ks][y]);
-#pragma unroll
-      for (int x = 0; x < FW; x++)
-#pragma unroll
-        for (int y = 0; y < FW; y++) Pol::mma(acc2[x][y], o.r2[ks][x], o.c2[ks][y]);
-    }
+        for (int y = 0; y < FW; y++) Pol::mma(acc[x][y], o.r[ks][x], o.c[ks][y]);
   };
   // Issue is in order per wave: a DMA or LDS read that waits for a full queue
   // also holds back every MFMA behind it.  Interleaving one DMA and a share of
   // the LDS reads between groups of MFMAs keeps both queues short and the
   // matrix pipe fed while the stage transfer proceeds.
-  constexpr int NLDS = 4 * KS * FW * Pol::P;
-  constexpr int NMMA = 2 * KS * FW * FW * Pol::MMA_PER_CALL;
+  constexpr int NLDS = 2 * KS * FW * Pol::P;
+  constexpr int NMMA = KS * FW * FW * Pol::MMA_PER_CALL;
   constexpr int GL = (NLDS + NI - 1) / NI, GM = (NMMA + NI - 1) / NI;
   auto step = [&](int64_t st, Ops& cur, Ops& nxt, unsigned char* freed, const unsigned char* next) {
     RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
@@ -397,40 +396,58 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // ---- epilogue: energy terms in FP64 ----
   const double ei = f.eps_o[i], ej = f.eps_o[j];
   double s = 0.0, c = 0.0;
+  auto buf_of = [&](int k) { return k == 0 ? buf0 : k == 1 ? buf1 : k == 2 ? buf2 : buf3; };
+  __syncthreads();  // every wave's stage DMA has landed and its reads are done
   if (tile_pair) {
+    // tile 2's wave (wr, wc) hands its scaled V^{ji} values to tile 1's wave
+    // (wr, wc) through stage buffer 2 wr + wc, in register order (lane-
+    // contiguous, conflict free); tile 1's wave forms the energy terms
+    double* X = reinterpret_cast<double*>(buf_of(2 * wr + wc));
+    if (grp == 1 && !same_occ) {
 #pragma unroll
-    for (int x = 0; x < FW; x++)
+      for (int x = 0; x < FW; x++)
 #pragma unroll
-      for (int y = 0; y < FW; y++)
+        for (int y = 0; y < FW; y++)
 #pragma unroll
-        for (int r = 0; r < Pol::NE; r++) {
-          const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
-          const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
-          double v1 = Pol::value(acc1[x][y], r);
-          double v2 = same_occ ? v1 : Pol::value(acc2[x][y], r);
-          if (is_i8<Pol>::value) {
-            v1 *= f.sigma[i * f.nvp + a] * f.sigma[j * f.nvp + b];
-            if (!same_occ) v2 *= f.sigma[j * f.nvp + a] * f.sigma[i * f.nvp + b];
-            else v2 = v1;
+          for (int r = 0; r < Pol::NE; r++) {
+            const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
+            double v2 = Pol::value(acc[x][y], r);
+            if (is_i8<Pol>::value) v2 *= f.sigma[j * f.nvp + a] * f.sigma[i * f.nvp + b];
+            X[((x * FW + y) * Pol::NE + r) * 64 + lane] = v2;
           }
-          const double inv = 1.0 / (ei + ej - f.eps_v[a] - f.eps_v[b]);
-          double e;
-          if (same_blk) e = v1 * (2.0 * v1 - v2);
-          else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
-          kahan_add(s, c, e * inv);
-        }
-  } else if constexpr (packs_diagonal<Pol, TM>()) {
-    // diagonal block X: eps += sum_ab V_ab (2 V_ab - V_ba) / Delta_ab with
-    // V_ba read from an LDS copy of the tile (XOR-swizzled columns: the row
-    // writes and the transposed reads are both bank-conflict free)
-    double* T = reinterpret_cast<double*>(buf0);
-    auto swz = [](int a, int b) { return a * TM + (b ^ (a & 31)); };
+    }
+    __syncthreads();
+    if (grp == 0) {
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      if (t == 1 && kind == UNIT_ONE_DIAG) break;
-      const int64_t X = t ? Bk : A;
-      double v[FW][FW][Pol::NE];
-      __syncthreads();  // every wave's stage DMA has landed / the previous tile is read
+      for (int x = 0; x < FW; x++)
+#pragma unroll
+        for (int y = 0; y < FW; y++)
+#pragma unroll
+          for (int r = 0; r < Pol::NE; r++) {
+            const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
+            double v1 = Pol::value(acc[x][y], r);
+            if (is_i8<Pol>::value) v1 *= f.sigma[i * f.nvp + a] * f.sigma[j * f.nvp + b];
+            const double v2 = same_occ ? v1 : X[((x * FW + y) * Pol::NE + r) * 64 + lane];
+            const double inv = 1.0 / (ei + ej - f.eps_v[a] - f.eps_v[b]);
+            double e;
+            if (same_blk) e = v1 * (2.0 * v1 - v2);
+            else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
+            kahan_add(s, c, e * inv);
+          }
+    }
+  } else if constexpr (packs_diagonal<Pol, TM>()) {
+    // each wave group holds one diagonal block X (group 1: block Bk, idle for
+    // a lone block): eps += sum_ab V_ab (2 V_ab - V_ba) / Delta_ab with V_ba
+    // from an LDS copy of the tile in stage buffer grp (XOR-swizzled columns:
+    // the row writes and the transposed reads are both bank-conflict free)
+    const bool active = !(grp == 1 && kind == UNIT_ONE_DIAG);
+    const int64_t X = grp ? Bk : A;
+    double* T = reinterpret_cast<double*>(grp ? buf1 : buf0);
+    auto swz = [](int a, int b) { return a * TM + (b ^ (a & 31)); };
+    double v[FW][FW][Pol::NE];
+    if (active) {
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
@@ -439,12 +456,14 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            double w = Pol::value(t ? acc2[x][y] : acc1[x][y], r);
+            double w = Pol::value(acc[x][y], r);
             if (is_i8<Pol>::value) w *= f.sigma[i * f.nvp + X * TM + al] * f.sigma[j * f.nvp + X * TM + bl];
             v[x][y][r] = w;
             T[swz(al, bl)] = w;
           }
-      __syncthreads();
+    }
+    __syncthreads();
+    if (active) {
 #pragma unroll
       for (int x = 0; x < FW; x++)
 #pragma unroll
@@ -462,13 +481,14 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   double tot = s - c;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-  // the wave sums go through buf0 once every wave is done with the stages
-  // (keeps the workgroup's LDS at exactly the stage buffers)
+  // the wave sums go through buf3 (unused by both epilogues' second phase)
   __syncthreads();
-  double* wsum = reinterpret_cast<double*>(buf0);
+  double* wsum = reinterpret_cast<double*>(buf3) + 1024;
   if (lane == 0) wsum[wave] = tot;
   __syncthreads();
-  const double res = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+  double res = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) res += wsum[w];
   __syncthreads();
   return res;
 }
