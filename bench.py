@@ -11,6 +11,7 @@ timed in the plain FP64 path to give the mixed-precision speed-up and the
 energy difference in microhartree.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode i8s5] [--no-cpu]
+  python bench.py --workload c1|c3|c4|c5     (the other configurations, every mode)
 
 Prints ONE JSON line (plus progress on stderr).
 """
@@ -126,6 +127,132 @@ def copy_probe(r, mib):
     print(json.dumps({"copy_probe_bytes": n, "copies": 2}), flush=True)
 
 
+MODES = ["fp64", "i8s5", "i8s4", "fp32", "bf16x3"]
+
+
+def modes_on_fragment(r, ctx, dims, dptrs, steps, warmup):
+    """Times every precision mode on one device-resident fragment; energies
+    are compared with the fp64 GPU path."""
+    naux, nvirt, nocc = dims
+    Fp, Fm = r.pair_flops(nocc, nvirt, naux), r.metric_flops(nocc, nvirt, naux)
+    out = {}
+    for mode in MODES:
+        t, kern, E, eps = time_mode(r, ctx, dims, dptrs, mode, steps, warmup)
+        pair_s = kern["pair"][0] / max(1, kern["pair"][1]) * 1e-3
+        out[mode] = dict(ms_per_step=t * 1e3, pair_kernel_ms=pair_s * 1e3, tflops=(Fp + Fm) / t / 1e12,
+                         pair_tflops=Fp / pair_s / 1e12, E=E)
+        log(f"[bench] {mode}: {t * 1e3:.3f} ms/step, pair {pair_s * 1e3:.3f} ms, E={E!r}")
+    for m in out:
+        out[m]["dE_uEh_vs_fp64"] = (out[m]["E"] - out["fp64"]["E"]) * 1e6
+        out[m]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out[m]["ms_per_step"]
+    return out
+
+
+def workload_single(r, name, seed, naux, nvirt, nocc, steps, warmup, oracle_pairs):
+    """C1 / C5: one fragment, every mode.  Inputs are generated on the device
+    (the host generator's counter-based formulas; values agree with it to
+    libm rounding).  The FP64 pair energies are checked against the oracle:
+    all pairs for a small fragment, a seeded subset otherwise."""
+    ctx = r.Context(0)
+    bufs = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    log(f"[bench] {name}: generating nocc={nocc} nvirt={nvirt} naux={naux} seed={seed} on the device")
+    ctx.gen_device(seed, naux, nvirt, nocc, *bufs)
+    res = modes_on_fragment(r, ctx, (naux, nvirt, nocc), bufs + [dpe, det], steps, warmup)
+    # oracle check: rerun fp64 to have its B and pair matrix in the context
+    ctx.energy_device(naux, nvirt, nocc, *bufs, "fp64", dpe, det)
+    ctx.sync()
+    eps64 = ctx.from_device(dpe, (nocc, nocc))
+    eo, ev = ctx.from_device(bufs[2], (nocc,)), ctx.from_device(bufs[3], (nvirt,))
+    B = ctx.get_B(naux, nvirt, nocc)
+    import oracle
+    cores = oracle.num_threads()
+    t0 = time.perf_counter()
+    if oracle_pairs is None:
+        E_or, eps_or = oracle.pair_energies(B, eo, ev)
+        pi, pj = np.triu_indices(nocc)
+        ref, got = eps_or[pi, pj], eps64[pi, pj]
+    else:
+        rng = np.random.default_rng(0)
+        pi, pj = rng.integers(0, nocc, oracle_pairs), rng.integers(0, nocc, oracle_pairs)
+        ref, got = oracle.pair_energies_subset(B, eo, ev, pi, pj), eps64[pi, pj]
+        E_or = None
+    t_or = time.perf_counter() - t0
+    del B
+    orc = dict(pairs_checked=int(len(ref)), max_rel_err_fp64_pairs=float(np.max(np.abs(got - ref) / np.abs(ref))),
+               seconds=t_or, cores=cores)
+    if E_or is not None:
+        orc["E_oracle"] = E_or
+        for m in res:
+            res[m]["dE_uEh_vs_oracle"] = (res[m]["E"] - E_or) * 1e6
+    ctx.close()
+    return dict(workload=name, nocc=nocc, nvirt=nvirt, naux=naux, seed=seed, F_pair=r.pair_flops(nocc, nvirt, naux),
+                modes=res, oracle=orc)
+
+
+def workload_batch(r, seed, nfrag, steps):
+    """C3: nfrag DEC fragments of uneven shape through the persistent,
+    graph-captured batch launch; every mode, fragment energies against the
+    fp64 batch and the oracle."""
+    o, v, q = r.gen_batch_shapes(seed, nfrag)
+    log(f"[bench] c3: generating {nfrag} fragments")
+    frags = [r.gen_fragment(seed + f, int(q[f]), int(v[f]), int(o[f])) for f in range(nfrag)]
+    Fp = sum(r.pair_flops(int(o[f]), int(v[f]), int(q[f])) for f in range(nfrag))
+    ctx = r.Context(0)
+    res = {}
+    for mode in MODES:
+        t0 = time.perf_counter()
+        ctx.batch_prepare(frags, mode)
+        ctx.sync()
+        t_prep = time.perf_counter() - t0
+        ctx.batch_run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.batch_run()
+        ctx.sync()
+        t_run = (time.perf_counter() - t0) / steps
+        ef, _ = ctx.batch_result(want_pairs=False)
+        res[mode] = dict(pair_stage_ms=t_run * 1e3, pair_tflops=Fp / t_run / 1e12, prepare_s=t_prep,
+                         tile=ctx.batch_tile(), E_sum=float(np.sum(ef)), e_frag=ef)
+        log(f"[bench] c3 {mode}: pair stage {t_run * 1e3:.3f} ms, prepare {t_prep:.2f} s, tile {ctx.batch_tile()}")
+    import oracle
+    t0 = time.perf_counter()
+    e_or = np.array([oracle.pair_energies(oracle.metric_transform(V, P), eo, ev)[0] for P, V, eo, ev in frags])
+    t_or = time.perf_counter() - t0
+    for m in res:
+        ef = res[m].pop("e_frag")
+        res[m]["max_abs_dE_uEh_vs_oracle"] = float(np.max(np.abs(ef - e_or)) * 1e6)
+        res[m]["sum_dE_uEh_vs_oracle"] = float((np.sum(ef) - np.sum(e_or)) * 1e6)
+    ctx.close()
+    return dict(workload="c3 batch", nfrag=nfrag, seed=seed, F_pair=Fp, modes=res,
+                oracle=dict(E_sum=float(np.sum(e_or)), seconds=t_or))
+
+
+def workload_surrogate(r, seed):
+    """C4: BF16 MLP trained on exact pair energies of 512 seeded 64/512/2048
+    fragments, used to screen the pairs of a 128/1024/3072 fragment below
+    1e-6 Eh; the exact kernel computes every kept pair.  Also on the
+    locality-decay variant of the generator, where small pairs exist."""
+    from rimp2_gpu_amd import surrogate
+    ctx = r.Context(0)
+    out = dict(workload="c4 surrogate")
+    for lam, ntrain in ((0.0, 512), (1.5, 128)):
+        t0 = time.perf_counter()
+        F, T = surrogate.training_set(ctx, ntrain, seed0=100000, lam=lam, mode="i8s5", log=log)
+        t_set = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        hist = surrogate.train(ctx, F, T, epochs=4, batch=4096, lr=2e-3, log=log)
+        t_train = time.perf_counter() - t0
+        rep = surrogate.screen_report(ctx, 128, 1024, 3072, seed, lam, 1e-6, mode="i8s5")
+        rep.update(train_pairs=int(len(T)), train_fragments=ntrain, training_set_s=t_set, train_s=t_train,
+                   loss_first_last=[hist[0], hist[-1]])
+        out[f"lambda_{lam}"] = rep
+        log(f"[bench] c4 lambda={lam}: {rep}")
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +266,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-memory (PCIe-inclusive) run")
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5"],
+                    help="run one of the other configurations in every mode (one JSON line) instead of the "
+                         "headline step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -150,6 +280,17 @@ def main():
     import rimp2_gpu_amd as r
     if a.copy_probe:
         copy_probe(r, a.copy_probe)
+        return
+    if a.workload:
+        if a.workload == "c1":
+            res = workload_single(r, "c1", 1, 512, 128, 16, a.steps, a.warmup, None)
+        elif a.workload == "c5":
+            res = workload_single(r, "c5", 5, 6144, 2048, 256, 1, 0, 4)
+        elif a.workload == "c3":
+            res = workload_batch(r, 3, 256, a.steps)
+        else:
+            res = workload_surrogate(r, 4)
+        print(json.dumps(res), flush=True)
         return
     naux, nvirt, nocc = a.naux, a.nvirt, a.nocc
     log(f"[bench] generating fragment nocc={nocc} nvirt={nvirt} naux={naux} seed={a.seed}")

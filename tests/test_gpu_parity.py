@@ -271,3 +271,23 @@ def test_get_dev_mem_both_names(ctx, rimp2):
     t2, f2 = ctypes.c_size_t(), ctypes.c_size_t()
     rimp2.lib().get_dev_mem(ctypes.byref(t2), ctypes.byref(f2))
     assert t2.value == t and 0 < f2.value <= t
+
+
+@pytest.mark.parametrize("lam", [0.0, 1.5])
+def test_device_generator_matches_host(ctx, rimp2, lam):
+    """The device generator evaluates the host generator's counter-based
+    formulas; values agree to libm / summation rounding."""
+    naux, nvirt, nocc, seed = 96, 40, 6, 17
+    say("device generator", lam)
+    P, V, eo, ev = rimp2.gen_fragment_local(seed, naux, nvirt, nocc, lam)
+    b = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
+    ctx.gen_device(seed, naux, nvirt, nocc, *b, lam)
+    Pd = ctx.from_device(b[0], (naux, nvirt, nocc))
+    Vd = ctx.from_device(b[1], (naux, naux))
+    eod, evd = ctx.from_device(b[2], (nocc,)), ctx.from_device(b[3], (nvirt,))
+    for p in b:
+        ctx.free(p)
+    np.testing.assert_array_equal(eod, eo)
+    np.testing.assert_array_equal(evd, ev)
+    np.testing.assert_allclose(Pd, P, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(Vd, V, rtol=1e-13, atol=1e-14)
