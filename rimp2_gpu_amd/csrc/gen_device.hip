@@ -32,25 +32,25 @@ __device__ inline double normal(uint64_t key, uint64_t n) {
 }
 
 __global__ void gen_x_kernel(uint64_t kx, int64_t naux, int64_t nvirt, int64_t nocc, double lam, double* X) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= naux * nvirt * nocc) return;
-  const int64_t q = t % naux, c = t / naux, a = c % nvirt, i = c / nvirt;
-  double v = normal(kx, (uint64_t)t) * (0.1 / sqrt(1.0 + (double)a / 16.0));
-  if (lam > 0.0) {
-    const double xa = ((double)a + 0.5) * (double)nocc / (double)nvirt;
-    const double xq = ((double)q + 0.5) * (double)nocc / (double)naux;
-    v *= exp(-fabs(xa - (double)i) / lam) * exp(-fabs(xq - (double)i) / lam);
+  RIMP2_GRID_STRIDE(t, naux * nvirt * nocc) {
+    const int64_t q = t % naux, c = t / naux, a = c % nvirt, i = c / nvirt;
+    double v = normal(kx, (uint64_t)t) * (0.1 / sqrt(1.0 + (double)a / 16.0));
+    if (lam > 0.0) {
+      const double xa = ((double)a + 0.5) * (double)nocc / (double)nvirt;
+      const double xq = ((double)q + 0.5) * (double)nocc / (double)naux;
+      v *= exp(-fabs(xa - (double)i) / lam) * exp(-fabs(xq - (double)i) / lam);
+    }
+    X[t] = v;
   }
-  X[t] = v;
 }
 
 __global__ void gen_a_kernel(uint64_t ka, int64_t naux, int64_t nocc, double lam, double* A) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= naux * naux) return;
-  const int64_t p = t % naux, k = t / naux;
-  double v = normal(ka, (uint64_t)t);
-  if (lam > 0.0) v *= exp(-fabs((double)(p - k)) * ((double)nocc / (double)naux) / lam);
-  A[t] = v;
+  RIMP2_GRID_STRIDE(t, naux * naux) {
+    const int64_t p = t % naux, k = t / naux;
+    double v = normal(ka, (uint64_t)t);
+    if (lam > 0.0) v *= exp(-fabs((double)(p - k)) * ((double)nocc / (double)naux) / lam);
+    A[t] = v;
+  }
 }
 
 __global__ void add_identity_kernel(double* V, int64_t n) {
@@ -78,13 +78,13 @@ extern "C" int rimp2_gen_fragment_device(void* ctx, uint64_t seed, int64_t naux,
   hipStream_t s = c->stream;
   if (d_Pia) {
     const int64_t n = naux * nvirt * nocc;
-    hipLaunchKernelGGL(gen_x_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, stream_key(seed, 0), naux,
+    hipLaunchKernelGGL(gen_x_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, s, stream_key(seed, 0), naux,
                        nvirt, nocc, lam, d_Pia);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
   if (d_V) {
     double* A = static_cast<double*>(c->work.ensure((size_t)(naux * naux) * sizeof(double)));
-    hipLaunchKernelGGL(gen_a_kernel, dim3((unsigned)ceil_div(naux * naux, 256)), dim3(256), 0, s, stream_key(seed, 1),
+    hipLaunchKernelGGL(gen_a_kernel, dim3(grid_blocks(naux * naux, 256)), dim3(256), 0, s, stream_key(seed, 1),
                        naux, nocc, lam, A);
     RIMP2_HIP_CHECK(hipGetLastError());
     gemm_nt_f64(s, naux, naux, naux, 1.0 / (double)naux, A, naux, A, naux, 0.0, d_V, naux);

@@ -314,12 +314,12 @@ constexpr int TR_TM = 64, TR_TN = 128, TR_KQ = 64;  // K padding: four 16-wide s
 
 __global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64_t MB, int64_t KS,
                                  double* __restrict__ Lp) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= KS * MB * 64) return;
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6, mb = f % MB, ks = f / MB;
-  const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
-  Lp[t] = (m < n && k < n && k <= m) ? Li[m + n * k] : 0.0;
+  RIMP2_GRID_STRIDE(t, KS * MB * 64) {
+    const int lane = (int)(t & 63);
+    const int64_t f = t >> 6, mb = f % MB, ks = f / MB;
+    const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
+    Lp[t] = (m < n && k < n && k <= m) ? Li[m + n * k] : 0.0;
+  }
 }
 
 struct TransformArgs {
@@ -333,20 +333,14 @@ struct TransformArgs {
   int64_t nwu, chunk;
 };
 
-__global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
-  constexpr int FB = 512;                     // bytes per fragment
-  constexpr int A_STAGE = 4 * 4 * FB;         // 4 k-steps x 4 row blocks
-  constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks
-  constexpr int SB = A_STAGE + B_STAGE;       // 24 KB
-  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage (0-1: L^{-1}, 2-5: X)
-  static_assert(A_STAGE == 2 * 4096, "transfers 0 and 1 of every wave carry the L^{-1} slab");
-  __shared__ __align__(16) unsigned char buf0[SB];
-  __shared__ __align__(16) unsigned char buf1[SB];
-  __shared__ __align__(16) unsigned char buf2[SB];
-  __shared__ __align__(16) unsigned char buf3[SB];
-  const int64_t bid = blockIdx.x;
-  const int64_t wu = (bid & 7) * p.chunk + (bid >> 3);
-  if (wu >= p.nwu) return;
+__device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
+                                      unsigned char* __restrict__ buf1, unsigned char* __restrict__ buf2,
+                                      unsigned char* __restrict__ buf3) {
+  constexpr int FB = 512;
+  constexpr int A_STAGE = 4 * 4 * FB;
+  constexpr int B_STAGE = 4 * 8 * FB;
+  constexpr int SB = A_STAGE + B_STAGE;
+  constexpr int NI = SB / 4096;
   const int64_t mt = wu % p.MT, col = wu / p.MT, i = col / p.NT, nt = col % p.NT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -450,13 +444,35 @@ __global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
         const int64_t a = nt * TR_TN + wc * 64 + y * 16 + (lane & 15);
         if (q < p.naux && a < p.nvirt) p.B[q + p.naux * (a + p.nvirt * i)] = acc[x][y][r];
       }
+  __syncthreads();  // every wave is done with the stage buffers before a next unit refills them
+}
+
+__global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
+  constexpr int FB = 512;                     // bytes per fragment
+  constexpr int A_STAGE = 4 * 4 * FB;         // 4 k-steps x 4 row blocks
+  constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks
+  constexpr int SB = A_STAGE + B_STAGE;       // 24 KB
+  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage (0-1: L^{-1}, 2-5: X)
+  static_assert(A_STAGE == 2 * 4096, "transfers 0 and 1 of every wave carry the L^{-1} slab");
+  __shared__ __align__(16) unsigned char buf0[SB];
+  __shared__ __align__(16) unsigned char buf1[SB];
+  __shared__ __align__(16) unsigned char buf2[SB];
+  __shared__ __align__(16) unsigned char buf3[SB];
+  // XCD-aware: slot s of XCD x takes units x * chunk + s, + nslot, ... (the
+  // loop bounds are uniform over the workgroup)
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+  for (int64_t k = slot; k < p.chunk; k += nslot) {
+    const int64_t wu = xcd * p.chunk + k;
+    if (wu >= p.nwu) break;
+    transform_unit(p, wu, buf0, buf1, buf2, buf3);
+  }
 }
 
 __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * n) return;
-  int64_t r = e % n, c = e / n;
-  if (c > r) A[r + lda * c] = 0.0;
+  RIMP2_GRID_STRIDE(e, n * n) {
+    int64_t r = e % n, c = e / n;
+    if (c > r) A[r + lda * c] = 0.0;
+  }
 }
 
 }  // namespace
@@ -487,7 +503,7 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
       gemm_f64<64, 64, true, false, true>(s, m, m, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
                                           L + r0 + n * r0, n);
     }
-    hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, s, L, n, n);
+    hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_blocks(n * n, 256)), dim3(256), 0, s, L, n, n);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
   {
@@ -524,7 +540,7 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
     // L^{-1} in A-fragment order for the transform GEMM
     const int64_t MB = round_up(n, TR_TM) / 16, KS = round_up(n, TR_KQ) / 4;
     double* Lp = static_cast<double*>(c->linv_packed.ensure((size_t)(KS * MB * 64) * sizeof(double)));
-    hipLaunchKernelGGL(pack_linv_kernel, dim3((unsigned)ceil_div(KS * MB * 64, 256)), dim3(256), 0, s, Li, n, MB, KS,
+    hipLaunchKernelGGL(pack_linv_kernel, dim3(grid_blocks(KS * MB * 64, 256)), dim3(256), 0, s, Li, n, MB, KS,
                        Lp);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
@@ -572,7 +588,8 @@ void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   p.Lp = c->linv_packed.as<unsigned char>();
   p.Xp = static_cast<const unsigned char*>(Xp);
   p.B = B;
-  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * p.chunk)), dim3(256), 0, c->stream, p);
+  const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
+  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(256), 0, c->stream, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

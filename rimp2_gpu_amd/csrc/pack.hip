@@ -95,25 +95,23 @@ __device__ inline double bget(const PackArgs& p, int64_t q, int64_t a, int64_t i
 
 // one thread per (i, ks, rb, lane)
 __global__ __launch_bounds__(256) void pack_fp64_kernel(PackArgs p) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t total = p.nocc * p.nks * p.nrb * 64;
-  if (t >= total) return;
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;  // fragment index = (i*nks + ks)*nrb + rb
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-  static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
+  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
+    const int lane = (int)(t & 63);
+    const int64_t f = t >> 6;  // fragment index = (i*nks + ks)*nrb + rb
+    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+    const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
+    static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
+  }
 }
 
 __global__ __launch_bounds__(256) void pack_fp32_kernel(PackArgs p) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t total = p.nocc * p.nks * p.nrb * 64;
-  if (t >= total) return;
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
-  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
+  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
+    const int lane = (int)(t & 63);
+    const int64_t f = t >> 6;
+    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+    const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
+    static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
+  }
 }
 
 // bf16 round-to-nearest-even of a float (no NaN inputs reach here).
@@ -126,59 +124,57 @@ __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h <<
 
 // planes: 0 = hi = bf16(b), 1 = lo = bf16(b - hi)
 __global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t total = p.nocc * p.nks * p.nrb * 64;
-  if (t >= total) return;
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
-  uint16_t hi[8], lo[8];
-#pragma unroll
-  for (int e = 0; e < 8; e++) {
-    float b = (float)bget(p, q0 + e, a, i);
-    hi[e] = f2bf(b);
-    lo[e] = f2bf(b - bf2f(hi[e]));
+  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
+    const int lane = (int)(t & 63);
+    const int64_t f = t >> 6;
+    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+    const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
+    uint16_t hi[8], lo[8];
+  #pragma unroll
+    for (int e = 0; e < 8; e++) {
+      float b = (float)bget(p, q0 + e, a, i);
+      hi[e] = f2bf(b);
+      lo[e] = f2bf(b - bf2f(hi[e]));
+    }
+    uint4 vh, vl;
+    vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
+    vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
+    vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
+    vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
+    uint4* dst = static_cast<uint4*>(p.dst);
+    dst[(f * 2 + 0) * 64 + lane] = vh;
+    dst[(f * 2 + 1) * 64 + lane] = vl;
   }
-  uint4 vh, vl;
-  vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
-  vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
-  vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
-  vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
-  uint4* dst = static_cast<uint4*>(p.dst);
-  dst[(f * 2 + 0) * 64 + lane] = vh;
-  dst[(f * 2 + 1) * 64 + lane] = vl;
 }
 
 template <int S>
 __global__ __launch_bounds__(256) void pack_i8_kernel(PackArgs p, int64_t nvp) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t total = p.nocc * p.nks * p.nrb * 64;
-  if (t >= total) return;
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
-  const double sg = p.sigma[i * nvp + a];
-  const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
-  int8_t sl[S][16];
-#pragma unroll
-  for (int e = 0; e < 16; e++) {
-    double r = bget(p, q0 + e, a, i) * inv;
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-      double s = rint(r);
-      s = fmin(127.0, fmax(-127.0, s));
-      sl[k][e] = (int8_t)s;
-      r = (r - s) * 254.0;
+  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
+    const int lane = (int)(t & 63);
+    const int64_t f = t >> 6;
+    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+    const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
+    const double sg = p.sigma[i * nvp + a];
+    const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
+    int8_t sl[S][16];
+  #pragma unroll
+    for (int e = 0; e < 16; e++) {
+      double r = bget(p, q0 + e, a, i) * inv;
+  #pragma unroll
+      for (int k = 0; k < S; k++) {
+        double s = rint(r);
+        s = fmin(127.0, fmax(-127.0, s));
+        sl[k][e] = (int8_t)s;
+        r = (r - s) * 254.0;
+      }
     }
-  }
-  uint4* dst = static_cast<uint4*>(p.dst);
-#pragma unroll
-  for (int k = 0; k < S; k++) {
-    uint4 v;
-    memcpy(&v, sl[k], 16);
-    dst[(f * S + k) * 64 + lane] = v;
+    uint4* dst = static_cast<uint4*>(p.dst);
+  #pragma unroll
+    for (int k = 0; k < S; k++) {
+      uint4 v;
+      memcpy(&v, sl[k], 16);
+      dst[(f * S + k) * 64 + lane] = v;
+    }
   }
 }
 
@@ -189,7 +185,7 @@ void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc,
   ScopedTimer tm(c, RIMP2_K_PACK);
   PackArgs p{dB, naux, nvirt, nocc, g.nks, g.nrb, dscales, dst};
   const int64_t threads = nocc * g.nks * g.nrb * 64;
-  const unsigned blocks = (unsigned)ceil_div(threads, 256);
+  const unsigned blocks = grid_blocks(threads, 256);
   switch (g.mode) {
     case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_fp64_kernel, dim3(blocks), dim3(256), 0, s, p); break;
     case RIMP2_MODE_FP32: hipLaunchKernelGGL(pack_fp32_kernel, dim3(blocks), dim3(256), 0, s, p); break;

@@ -42,17 +42,22 @@ template <class Pol, int TM>
 __global__ __launch_bounds__(PAIR_THREADS) void pair_kernel(PairParams p) {
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   // XCD-aware mapping: consecutive workgroups land on different XCDs
-  // (round-robin dispatch), so give each XCD slot a contiguous run of work
-  // units: neighbouring units share an occupied pair and hit the same L2.
-  const int64_t bid = blockIdx.x;
-  const int64_t wu = (bid & 7) * p.chunk + (bid >> 3);
-  if (wu >= p.nwu) return;
-  const int64_t pidx = wu / p.ntp, tp = wu % p.ntp;
-  int64_t i, j, A, Bk;
-  tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
-  const int kind = unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk);
-  const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
-  if (threadIdx.x == 0) p.partials[wu] = e;
+  // (round-robin dispatch), so each XCD owns a contiguous run of work units
+  // (neighbouring units share an occupied pair and hit the same L2); slot s
+  // of XCD x takes units x * chunk + s, + nslot, ...  (one unit per
+  // workgroup unless the grid had to be capped; the loop bounds are uniform
+  // over the workgroup).
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+  for (int64_t k = slot; k < p.chunk; k += nslot) {
+    const int64_t wu = xcd * p.chunk + k;
+    if (wu >= p.nwu) break;
+    const int64_t pidx = wu / p.ntp, tp = wu % p.ntp;
+    int64_t i, j, A, Bk;
+    tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
+    const int kind = unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk);
+    const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+    if (threadIdx.x == 0) p.partials[wu] = e;
+  }
 }
 
 // Per pair: Neumaier sum of its ntp partials in order -> eps_pair (both
@@ -127,8 +132,8 @@ __global__ void denom_check_kernel(const double* eo, int64_t nocc, const double*
 
 template <class Pol, int TM>
 void launch_pair(hipStream_t s, const PairParams& p) {
-  const unsigned grid = (unsigned)(8 * p.chunk);
-  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3(grid), dim3(PAIR_THREADS), 0, s, p);
+  const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
+  hipLaunchKernelGGL((pair_kernel<Pol, TM>), dim3((unsigned)(8 * nslot)), dim3(PAIR_THREADS), 0, s, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

@@ -136,6 +136,19 @@ void mlp_destroy(MlpState* m);
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
+// A dispatch counts its work-items in 32 bits, so a grid of blocks x threads
+// >= 2^32 wraps and silently runs only part of the work.  Launches over data-
+// sized ranges therefore cap the grid (grid_blocks) and their kernels loop
+// grid-stride (RIMP2_GRID_STRIDE); work-unit kernels loop over their units.
+constexpr int64_t kMaxGridBlocks = int64_t(1) << 20;
+inline unsigned grid_blocks(int64_t items, int64_t threads) {
+  const int64_t b = ceil_div(items, threads);
+  return (unsigned)(b < 1 ? 1 : (b > kMaxGridBlocks ? kMaxGridBlocks : b));
+}
+#define RIMP2_GRID_STRIDE(t, n)                                                        \
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)(n); \
+       t += (int64_t)gridDim.x * blockDim.x)
+
 }  // namespace rimp2
 
 // Wraps an extern "C" body: converts rimp2::Error into a status code.

@@ -276,7 +276,8 @@ def test_get_dev_mem_both_names(ctx, rimp2):
 @pytest.mark.parametrize("lam", [0.0, 1.5])
 def test_device_generator_matches_host(ctx, rimp2, lam):
     """The device generator evaluates the host generator's counter-based
-    formulas; values agree to libm / summation rounding."""
+    formulas; values agree to rounding (libm, FMA contraction of the affine
+    maps, GEMM summation order of V)."""
     naux, nvirt, nocc, seed = 96, 40, 6, 17
     say("device generator", lam)
     P, V, eo, ev = rimp2.gen_fragment_local(seed, naux, nvirt, nocc, lam)
@@ -287,7 +288,41 @@ def test_device_generator_matches_host(ctx, rimp2, lam):
     eod, evd = ctx.from_device(b[2], (nocc,)), ctx.from_device(b[3], (nvirt,))
     for p in b:
         ctx.free(p)
-    np.testing.assert_array_equal(eod, eo)
-    np.testing.assert_array_equal(evd, ev)
+    np.testing.assert_allclose(eod, eo, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(evd, ev, rtol=1e-15, atol=0)
     np.testing.assert_allclose(Pd, P, rtol=1e-13, atol=1e-16)
     np.testing.assert_allclose(Vd, V, rtol=1e-13, atol=1e-14)
+
+
+def test_launches_beyond_32bit_work_items(ctx, oracle, rimp2):
+    """256/1536/768: the pair launch covers 9.5M work units x 512 threads
+    (> 2^32 work-items, the width of a dispatch's grid size) and the packing /
+    generator kernels cover 302M elements (> their capped grids), so every
+    grid-stride and unit loop runs more than once.  Pair energies of a seeded
+    subset against the oracle; i8s5 against fp64."""
+    naux, nvirt, nocc = 768, 1536, 256
+    say("beyond 32-bit work-items", naux, nvirt, nocc)
+    b = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    try:
+        ctx.gen_device(31, naux, nvirt, nocc, *b)
+        ctx.energy_device(naux, nvirt, nocc, *b, "i8s5", dpe, det)
+        ctx.sync()
+        E8 = float(ctx.from_device(det, (1,))[0])
+        ctx.energy_device(naux, nvirt, nocc, *b, "fp64", dpe, det)
+        ctx.sync()
+        E64 = float(ctx.from_device(det, (1,))[0])
+        eps = ctx.from_device(dpe, (nocc, nocc))
+        eo, ev = ctx.from_device(b[2], (nocc,)), ctx.from_device(b[3], (nvirt,))
+        B = ctx.get_B(naux, nvirt, nocc)
+    finally:
+        for p in b + [dpe, det]:
+            ctx.free(p)
+    rng = np.random.default_rng(3)
+    pi, pj = rng.integers(0, nocc, 8), rng.integers(0, nocc, 8)
+    pi[0], pj[0] = nocc - 1, nocc - 1  # the last units of the launch
+    ref = oracle.pair_energies_subset(B, eo, ev, pi, pj)
+    del B
+    np.testing.assert_allclose(eps[pi, pj], ref, rtol=1e-11)
+    say("E64", E64, "E_i8s5", E8, "dE uEh", (E8 - E64) * 1e6)
+    assert abs(E8 - E64) < 1e-6
