@@ -152,12 +152,17 @@ int rimp2_batch_tile(void* ctx, int64_t* tile);
 /* ---- pair-screening surrogate (BF16 MLP 16 -> 64 -> 64 -> 1, GELU) ---- */
 #define RIMP2_NFEAT 16
 /* Per-pair features of the last transform: [npairs_unique, 16] float32,
- * pairs ordered j-major (i <= j, index j*(j+1)/2 + i). */
+ * pairs ordered j-major (i <= j, index j*(j+1)/2 + i).  Feature 15 is the
+ * base estimate log(sum_Q ||B_Qi||^2 ||B_Qj||^2 / (2 mean(e_a) - e_i - e_j));
+ * the others are dimensionless ratios of row-norm sums and orbital-energy
+ * gaps (mlp.hip lists them). */
 int rimp2_pair_features(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v,
                         float* feats);
 int rimp2_mlp_init(void* ctx, uint64_t seed);
 /* Standardisation used by every MLP call: feature f enters as
- * (x_f - mean[f]) / stdev[f]; the network predicts (log(-eps_ij) - tmean)/tstd. */
+ * (x_f - mean[f]) / stdev[f]; the network output y gives the prediction
+ * log(-eps_ij) = feature 15 + y tstd + tmean (a residual on the base
+ * estimate), so tmean / tstd are the statistics of log(-eps_ij) - feature 15. */
 int rimp2_mlp_set_norm(void* ctx, const float* mean, const float* stdev, float tmean, float tstd);
 /* One Adam step on a minibatch (host arrays: raw features [n,16], raw targets
  * log(-eps_ij)); loss receives the mean squared error of the standardised

@@ -9,8 +9,10 @@
  * Parameter vector (float32, as on the GPU):
  *   W1[16][64] | b1[64] | W2[64][64] (W2[k][m]: input k -> output m) | b2[64] | w3[64] | b3
  * norm: mean[16], std[16], tmean, tstd.  Inputs are standardised,
- * x = (feat - mean)/std;  the network predicts (target - tmean)/tstd where the
- * target is log(-eps_ij).  loss = mean over samples of (y - t_std)^2.
+ * x = (feat - mean)/std;  the network predicts the standardised residual
+ * (target - feat[15] - tmean)/tstd where the target is log(-eps_ij) and
+ * feature 15 is the base estimate; prediction = feat[15] + y tstd + tmean.
+ * loss = mean over samples of (y - t_std)^2.
  *
  * emulate_bf16 != 0 rounds to bf16 at the points the GPU kernel feeds its
  * MFMAs: x, W1, W2, w3, h1 = GELU(z1), h2 = GELU(z2), dZ2 and dZ1 (the
@@ -69,7 +71,11 @@ double oracle_mlp_loss_grad(int64_t n, const float* feats, const float* target, 
   double loss = 0.0;
   for (int64_t s = 0; s < n; s++) {
     double x[NF], z1[NH], h1[NH], z2[NH], h2[NH];
-    for (int f = 0; f < NF; f++) x[f] = rb(((double)feats[s * NF + f] - norm[f]) / norm[NF + f], em);
+    for (int f = 0; f < NF; f++) {
+      double z = ((double)feats[s * NF + f] - norm[f]) / norm[NF + f];
+      z = z > 4.0 ? 4.0 : (z < -4.0 ? -4.0 : z); /* inputs saturate at +-4 */
+      x[f] = rb(z, em);
+    }
     for (int k = 0; k < NH; k++) {
       double a = 0.0;
       for (int f = 0; f < NF; f++) a += x[f] * W1[f][k];
@@ -84,9 +90,10 @@ double oracle_mlp_loss_grad(int64_t n, const float* feats, const float* target, 
     }
     double y = b3;
     for (int k = 0; k < NH; k++) y += h2[k] * w3[k];
-    if (pred) pred[s] = y * norm[2 * NF + 1] + norm[2 * NF];
+    const double base = (double)feats[s * NF + (NF - 1)]; /* residual model: base estimate = feature 15 */
+    if (pred) pred[s] = base + y * norm[2 * NF + 1] + norm[2 * NF];
     if (!target) continue;
-    const double d = y - ((double)target[s] - norm[2 * NF]) / norm[2 * NF + 1];
+    const double d = y - ((double)target[s] - base - norm[2 * NF]) / norm[2 * NF + 1];
     loss += d * d;
     if (!grad) continue;
     const double dy = 2.0 * d / (double)n;

@@ -6,15 +6,22 @@
 // screened energies are always the exact kernel's energies on the kept pairs.
 //
 // Features of a unique pair (i <= j), from the FP64 B of the last transform
-// (r_i[Q] = ||B[Q,:,i]||, the row norm of occupied i at auxiliary index Q):
-//   0 log ||B_i||_F          1 log ||B_j||_F
-//   2..9  log f_s, f_s = sum_{Q in slab s} r_i[Q] r_j[Q]  (8 equal Q-slabs;
-//         f_s >= ||sum_{Q in s} B_Qi^T B_Qj||_F, a bound on the slab's share of V^ij)
-//   10 log sum_s f_s  (>= ||V^ij||_F)
-//   11 e_i + e_j     12 2 min(e_a) - e_i - e_j     13 2 mean(e_a) - e_i - e_j
-//   14 [i == j]      15 log((sum_s f_s)^2 / (2 mean(e_a) - e_i - e_j))
-// Target: log(-eps_ij).  Features and target are standardised with the
-// statistics set by rimp2_mlp_set_norm (written into the MLP state).
+// (r_i[Q] = ||B[Q,:,i]||, the row norm of occupied i at auxiliary index Q;
+// t = sum_Q r_i[Q] r_j[Q] >= ||V^ij||_F (Cauchy-Schwarz per Q),
+// s2 = sum_Q r_i[Q]^2 r_j[Q]^2, which ||V^ij||_F^2 approaches when the
+// auxiliary contributions add incoherently; gbar = 2 mean(e_a) - e_i - e_j):
+//   0 log(||B_i|| ||B_j|| / t)     1 log(||B_i|| / ||B_j||)
+//   2..9  log(f_s / t), f_s = sum_{Q in slab s} r_i[Q] r_j[Q]  (8 equal Q-slabs)
+//   10 log(t^2 / s2)  (coherence: 0 .. log naux)
+//   11 e_i + e_j     12 2 min(e_a) - e_i - e_j     13 gbar
+//   14 [i == j]      15 log(s2 / gbar)  (the base estimate of -eps_ij)
+// All but the energies and the base estimate are dimensionless, so a network
+// trained on one fragment shape applies to others.  The network predicts the
+// residual log(-eps_ij) - feature 15: prediction = feature 15 + output, so an
+// untrained or out-of-range network falls back to the base estimate.
+// Features and the residual target are standardised with the statistics set
+// by rimp2_mlp_set_norm (written into the MLP state); standardised features
+// saturate at +-4.
 //
 // Network: z1 = x W1 + b1, h1 = GELU(z1), z2 = h1 W2 + b2, h2 = GELU(z2),
 // y = h2 w3 + b3.  Weights and activations feed the MFMAs as BF16
@@ -39,6 +46,9 @@ constexpr int P_B2 = P_W2 + NH * NH;     // [NH]
 constexpr int P_W3 = P_B2 + NH;          // [NH]
 constexpr int P_B3 = P_W3 + NH;          // [1]
 constexpr int NPARAM = P_B3 + 1;         // 5313
+// standardised inputs saturate here: a fragment whose features leave the
+// training range cannot drive the network into wild extrapolation
+constexpr float XCLIP = 4.0f;
 
 struct MlpState {
   bool init = false;
@@ -95,14 +105,15 @@ __global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int6
   while ((j + 1) * (j + 2) / 2 <= p) j++;
   const int64_t i = p - j * (j + 1) / 2;
   __shared__ double sl[8][256];
-  __shared__ double ni[256], nj[256], evs[2][256];
+  __shared__ double ni[256], nj[256], nc[256], evs[2][256];
   double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  double a2 = 0, b2 = 0;
+  double a2 = 0, b2 = 0, c2 = 0;
   for (int64_t q = threadIdx.x; q < naux; q += 256) {
     const double x = r[i * naux + q], y = r[j * naux + q];
     f[(q * 8) / naux] += x * y;
     a2 += x * x;
     b2 += y * y;
+    c2 += (x * y) * (x * y);
   }
   double emin = INFINITY, esum = 0.0;
   for (int64_t a = threadIdx.x; a < nvirt; a += 256) {
@@ -112,6 +123,7 @@ __global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int6
   for (int s = 0; s < 8; s++) sl[s][threadIdx.x] = f[s];
   ni[threadIdx.x] = a2;
   nj[threadIdx.x] = b2;
+  nc[threadIdx.x] = c2;
   evs[0][threadIdx.x] = emin;
   evs[1][threadIdx.x] = esum;
   __syncthreads();
@@ -120,6 +132,7 @@ __global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int6
       for (int s = 0; s < 8; s++) sl[s][threadIdx.x] += sl[s][threadIdx.x + w];
       ni[threadIdx.x] += ni[threadIdx.x + w];
       nj[threadIdx.x] += nj[threadIdx.x + w];
+      nc[threadIdx.x] += nc[threadIdx.x + w];
       evs[0][threadIdx.x] = fmin(evs[0][threadIdx.x], evs[0][threadIdx.x + w]);
       evs[1][threadIdx.x] += evs[1][threadIdx.x + w];
     }
@@ -128,21 +141,21 @@ __global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int6
   if (threadIdx.x == 0) {
     const double tiny = 1e-300;
     float* o = F + p * NF;
-    o[0] = (float)(0.5 * log(ni[0] + tiny));
-    o[1] = (float)(0.5 * log(nj[0] + tiny));
     double tot = 0.0;
-    for (int s = 0; s < 8; s++) {
-      o[2 + s] = (float)log(sl[s][0] + tiny);
-      tot += sl[s][0];
-    }
-    o[10] = (float)log(tot + tiny);
+    for (int s = 0; s < 8; s++) tot += sl[s][0];
+    const double ltot = log(tot + tiny), lni = 0.5 * log(ni[0] + tiny), lnj = 0.5 * log(nj[0] + tiny);
+    const double ls2 = log(nc[0] + tiny);
+    o[0] = (float)(lni + lnj - ltot);
+    o[1] = (float)(lni - lnj);
+    for (int s = 0; s < 8; s++) o[2 + s] = (float)(log(sl[s][0] + tiny) - ltot);
+    o[10] = (float)(2.0 * ltot - ls2);
     const double eij = eo[i] + eo[j];
     const double gmin = 2.0 * evs[0][0] - eij, gmean = 2.0 * evs[1][0] / (double)nvirt - eij;
     o[11] = (float)eij;
     o[12] = (float)gmin;
     o[13] = (float)gmean;
     o[14] = (i == j) ? 1.0f : 0.0f;
-    o[15] = (float)log(tot * tot / gmean + tiny);
+    o[15] = (float)(ls2 - log(gmean));
   }
 }
 
@@ -187,7 +200,7 @@ __global__ __launch_bounds__(256) void mlp_kernel(MlpArgs a) {
   for (int e = tid; e < 64 * NF; e += 256) {
     const int s = e / NF, f = e % NF;
     float x = 0.f;
-    if (s0 + s < a.n) x = (a.feats[(s0 + s) * NF + f] - a.norm[f]) / a.norm[NF + f];
+    if (s0 + s < a.n) x = fminf(XCLIP, fmaxf(-XCLIP, (a.feats[(s0 + s) * NF + f] - a.norm[f]) / a.norm[NF + f]));
     Xs[s][f] = tobf(x);
     XT[f][s] = tobf(x);
   }
@@ -249,9 +262,10 @@ __global__ __launch_bounds__(256) void mlp_kernel(MlpArgs a) {
     if (part == 0) {
       const float y = acc + P[P_B3];
       yv[s] = y;
-      if (a.pred && s0 + s < a.n) a.pred[s0 + s] = y * a.norm[2 * NF + 1] + a.norm[2 * NF];
+      const float base = s0 + s < a.n ? a.feats[(s0 + s) * NF + (NF - 1)] : 0.f;  // residual model
+      if (a.pred && s0 + s < a.n) a.pred[s0 + s] = base + y * a.norm[2 * NF + 1] + a.norm[2 * NF];
       float d = 0.f;
-      if (TRAIN && s0 + s < a.n) d = y - (a.target[s0 + s] - a.norm[2 * NF]) / a.norm[2 * NF + 1];
+      if (TRAIN && s0 + s < a.n) d = y - (a.target[s0 + s] - base - a.norm[2 * NF]) / a.norm[2 * NF + 1];
       dy[s] = d;  // residual; the loss gradient is 2 d / n
     }
   }

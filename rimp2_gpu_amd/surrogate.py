@@ -9,7 +9,7 @@ Adam kernels live in the HIP library (mlp.hip); this module drives them:
   * train: standardises features / target (log(-eps_ij)) and runs Adam steps
     on minibatches;
   * screen_report: runs the exact energy, the MLP-screened energy and a
-    network-free screen (the feature-15 bound estimate at the same threshold)
+    network-free screen (the feature-15 base estimate at the same threshold)
     and reports skipped pairs, energy errors and times.
 
 No part of the energy ever comes from the network: screened energies are the
@@ -32,9 +32,13 @@ def _device_fragment(ctx, seed, naux, nvirt, nocc, lam, bufs):
     return eo, ev
 
 
-def training_set(ctx, nfrag, nocc=64, nvirt=512, naux=2048, seed0=100000, lam=0.0, mode="i8s5", log=None):
-    """Features [npairs_total, 16] and targets log(-eps_ij) over nfrag
-    device-generated fragments (seeds seed0 .. seed0+nfrag-1)."""
+def training_set(ctx, nfrag, nocc=64, nvirt=512, naux=2048, seed0=100000, lam=0.0, mode="i8s5", log=None,
+                 min_log_eps=-46.0):
+    """Features [npairs, 16] and targets log(-eps_ij) over nfrag
+    device-generated fragments (seeds seed0 .. seed0+nfrag-1).  Pairs with
+    -eps_ij below exp(min_log_eps) (1e-20 by default, far below any screening
+    threshold; some underflow to zero) are left out: they cannot change a
+    screening decision and would dominate the residual's statistics."""
     bufs = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
     dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
     iu, ju = np.triu_indices(nocc)  # i <= j ... ordered j-major below
@@ -55,14 +59,19 @@ def training_set(ctx, nfrag, nocc=64, nvirt=512, naux=2048, seed0=100000, lam=0.
         for p in bufs + [dpe, det]:
             ctx.free(p)
     del iu, ju
-    return np.concatenate(F), np.concatenate(T)
+    F, T = np.concatenate(F), np.concatenate(T)
+    keep = T > min_log_eps
+    return F[keep], T[keep]
 
 
 def standardise(F, T):
+    """Feature statistics, and those of the residual target T - F[:, 15] (the
+    network corrects the base estimate carried in feature 15)."""
     mean = F.mean(axis=0).astype(np.float32)
     std = F.std(axis=0).astype(np.float32)
     std[std < 1e-6] = 1.0
-    return mean, std, np.float32(T.mean()), np.float32(max(T.std(), 1e-6))
+    res = T - F[:, 15]
+    return mean, std, np.float32(res.mean()), np.float32(max(res.std(), 1e-6))
 
 
 def train(ctx, F, T, epochs=4, batch=4096, lr=2e-3, seed=0, log=None):
@@ -112,16 +121,22 @@ def screen_report(ctx, nocc, nvirt, naux, seed, lam, thresh, mode="i8s5"):
     ii = np.concatenate([np.arange(j + 1) for j in range(nocc)])
     true = -eps[ii, jj]
     mult = np.where(ii == jj, 1.0, 2.0)
-    # network-free screen: the bound estimate exp(feature 15) at the same threshold
+    # network-free screen: the base estimate exp(feature 15) at the same threshold
     bound_keep = np.exp(F[:, 15].astype(np.float64)) >= thresh
+    # E_screened - E_exact = sum over skipped pairs of mult * |eps_ij| (eps_ij < 0)
     dE_bound = float(np.sum(mult * true * (~bound_keep)))
     mlp_keep = np.exp(pred.astype(np.float64)) >= thresh
-    # an oracle screen that knows the exact pair energies (the best any screen can do)
+    # a screen that knows the exact pair energies (the best any screen at this threshold can do)
     ideal_keep = true >= thresh
+    dE_ideal = float(np.sum(mult * true * (~ideal_keep)))
+    # prediction error over the pairs that can matter (some far pairs underflow to zero)
+    rel = true > 1e-20
+    ltrue = np.log(true[rel])
     return dict(
         E_exact=E, E_exact_again=E2, E_screened=Es, dE_screened=Es - E, nskipped=int(nskip),
         npairs=int(len(true)), t_exact_s=t_exact, t_screened_s=t_screen,
         mlp_skipped=int((~mlp_keep).sum()), bound_skipped=int((~bound_keep).sum()),
-        ideal_skipped=int((~ideal_keep).sum()), dE_bound_screen=-dE_bound,
-        pred_log_rmse=float(np.sqrt(np.mean((pred - np.log(np.maximum(true, 1e-300))) ** 2))),
+        ideal_skipped=int((~ideal_keep).sum()), dE_bound_screen=dE_bound, dE_ideal_screen=dE_ideal,
+        pred_log_rmse=float(np.sqrt(np.mean((pred[rel] - ltrue) ** 2))),
+        base_log_rmse=float(np.sqrt(np.mean((F[rel, 15].astype(np.float64) - ltrue) ** 2))),
     )

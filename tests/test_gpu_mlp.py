@@ -1,6 +1,7 @@
 """Pair-screening surrogate on the GPU.
 
-Parity: the GPU MLP forward and backward (BF16 operands, FP32 accumulation)
+Parity: the GPU MLP forward and backward (BF16 operands, FP32 accumulation;
+the network output is a residual on the base estimate in feature 15)
 against the FP64 oracle evaluated with the same BF16 rounding points
 (oracle.mlp_loss_grad(emulate_bf16=True)); what remains is FP32-vs-FP64
 accumulation order and the rare bf16 rounding flip, so predictions are held to
@@ -93,10 +94,12 @@ def test_features_match_numpy(ctx, rimp2):
         for i in range(j + 1):
             fs = np.array([np.sum(r[slab == s, i] * r[slab == s, j]) for s in range(8)])
             tot = fs.sum()
+            s2 = np.sum((r[:, i] * r[:, j]) ** 2)
+            ni, nj = np.sqrt((B[:, :, i] ** 2).sum()), np.sqrt((B[:, :, j] ** 2).sum())
             gmean = 2 * ev.mean() - eo[i] - eo[j]
-            exp = np.concatenate([[0.5 * np.log((B[:, :, i] ** 2).sum()), 0.5 * np.log((B[:, :, j] ** 2).sum())],
-                                  np.log(fs), [np.log(tot), eo[i] + eo[j], 2 * ev.min() - eo[i] - eo[j], gmean,
-                                               float(i == j), np.log(tot * tot / gmean)]])
+            exp = np.concatenate([[np.log(ni * nj / tot), np.log(ni / nj)], np.log(fs / tot),
+                                  [np.log(tot * tot / s2), eo[i] + eo[j], 2 * ev.min() - eo[i] - eo[j], gmean,
+                                   float(i == j), np.log(s2 / gmean)]])
             np.testing.assert_allclose(F[k], exp, rtol=2e-5, atol=2e-5)
             k += 1
 
@@ -123,3 +126,22 @@ def test_screened_energy_is_judged_by_the_exact_kernel(ctx, rimp2):
     assert nskip > 0
     assert np.abs(eps[skipped]).max() < 100 * thresh
     assert abs(Es - E) < 2 * nskip * 100 * thresh
+
+
+def test_network_gains_over_the_base_estimate_across_shapes(ctx, rimp2):
+    """Trained on 32/128/512 locality fragments, applied to a 48/192/768 one
+    (another shape): at the same threshold the network's screen skips about
+    the pairs an oracle screen would and loses less energy than the network-
+    free screen on the base estimate (feature 15) alone."""
+    from rimp2_gpu_amd import surrogate
+    lam, thresh = 1.5, 1e-6
+    say("gain over base estimate")
+    F, T = surrogate.training_set(ctx, 48, nocc=32, nvirt=128, naux=512, seed0=700, lam=lam, mode="fp64")
+    surrogate.train(ctx, F, T, epochs=6, batch=1024, lr=3e-3)
+    rep = surrogate.screen_report(ctx, 48, 192, 768, 4242, lam, thresh, mode="fp64")
+    say({k: rep[k] for k in ("mlp_skipped", "bound_skipped", "ideal_skipped", "dE_screened", "dE_bound_screen",
+                             "dE_ideal_screen", "pred_log_rmse", "base_log_rmse")})
+    assert rep["ideal_skipped"] > 0
+    assert rep["pred_log_rmse"] < rep["base_log_rmse"]
+    assert abs(rep["dE_screened"]) < abs(rep["dE_bound_screen"])
+    assert rep["mlp_skipped"] >= 0.9 * rep["ideal_skipped"]
