@@ -305,12 +305,12 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda,
 //   Xp[i][ks][rb][lane]: B fragments of (P|ia), lane l <-> (q = 4 ks + l/16,
 //                     a = 16 rb + l%16)  (the FP64 operand layout of pack.hip);
 // so every 512-byte fragment is contiguous in HBM and in LDS.  One workgroup
-// computes a 64 (Q rows) x 128 (virtuals of one occupied) tile of B = L^{-1} X
-// with a four-stage LDS ring fed by buffer_load_dwordx4 ... lds (24 KB per
+// computes a 128 (Q rows) x 128 (virtuals of one occupied) tile of B = L^{-1} X
+// with a four-stage LDS ring fed by buffer_load_dwordx4 ... lds (32 KB per
 // 16-wide K stage); the K loop of a row tile stops at its last row (L^{-1} is
-// lower triangular).  Waves 2 x 2, wave tile 32 x 64 = 2 x 4 MFMA blocks.
+// lower triangular).  Waves 4 x 2, wave tile 32 x 64 = 2 x 4 MFMA blocks.
 // ---------------------------------------------------------------------------
-constexpr int TR_TM = 64, TR_TN = 128, TR_KQ = 64;  // K padding: four 16-wide stages
+constexpr int TR_TM = 128, TR_TN = 128, TR_KQ = 64;  // K padding: four 16-wide stages
 
 __global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64_t MB, int64_t KS,
                                  double* __restrict__ Lp) {
@@ -333,35 +333,42 @@ struct TransformArgs {
   int64_t nwu, chunk;
 };
 
+// One 128 (Q rows) x 128 (virtuals of occupied i) tile of B = L^{-1} X.
+// Eight waves on a 4 x 2 grid, wave tile 32 x 64 (2 x 4 MFMA blocks of 16x16),
+// two waves per SIMD.  Same operand pipeline as the pair kernel: four LDS
+// stage buffers (32 KB per 16-deep K stage: 16 KB of L^{-1}, 16 KB of X) fed
+// by buffer_load_dwordx4 ... lds, next-stage operands read under the current
+// stage's MFMAs, one DMA + six LDS reads per eight MFMAs.
 __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
                                       unsigned char* __restrict__ buf1, unsigned char* __restrict__ buf2,
                                       unsigned char* __restrict__ buf3) {
-  constexpr int FB = 512;
-  constexpr int A_STAGE = 4 * 4 * FB;
-  constexpr int B_STAGE = 4 * 8 * FB;
-  constexpr int SB = A_STAGE + B_STAGE;
-  constexpr int NI = SB / 4096;
+  constexpr int FB = 512;                     // bytes per fragment
+  constexpr int NW = 8;                       // waves
+  constexpr int A_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 row blocks of L^{-1}
+  constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks of X
+  constexpr int SB = A_STAGE + B_STAGE;       // 32 KB
+  constexpr int NI = SB / (NW * 1024);        // 4 DMA transfers per wave per stage (0-1: L^{-1}, 2-3: X)
+  static_assert(A_STAGE == 2 * NW * 1024, "transfers 0 and 1 of every wave carry the L^{-1} slab");
   const int64_t mt = wu % p.MT, col = wu / p.MT, i = col / p.NT, nt = col % p.NT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
   const int64_t m0 = mt * TR_TM;
   // L^{-1} is lower triangular: the K loop of row tile mt stops at its last
-  // row (m0 + 64 is a whole number of four-stage groups; so is KS * 4)
+  // row (m0 + 128 is a whole number of four-stage groups; so is KS * 4)
   int64_t kend = m0 + TR_TM;
   if (kend > p.KS * 4) kend = p.KS * 4;
   const int64_t nst = kend / 16;
   const size_t a_ks = (size_t)p.MB * FB, b_ks = (size_t)p.NRB * FB;
-  const size_t a_off = (size_t)(mt * 4) * FB, b_off = (size_t)nt * 8 * FB;
-  const __amdgpu_buffer_rsrc_t ra =
-      pairk::slab_rsrc(p.Lp + a_off, (size_t)p.KS * a_ks - a_off);
+  const size_t a_off = (size_t)(mt * 8) * FB, b_off = (size_t)nt * 8 * FB;
+  const __amdgpu_buffer_rsrc_t ra = pairk::slab_rsrc(p.Lp + a_off, (size_t)p.KS * a_ks - a_off);
   const __amdgpu_buffer_rsrc_t rb =
       pairk::slab_rsrc(p.Xp + (size_t)i * p.KS * b_ks + b_off, (size_t)p.KS * b_ks - b_off);
   uint32_t voff[NI];
 #pragma unroll
   for (int n = 0; n < NI; n++) {
-    const int wb = (n * 4 + wave) * 1024;
-    if (n < 2) voff[n] = (uint32_t)((wb / (4 * FB)) * a_ks + wb % (4 * FB) + lane * 16);
+    const int wb = (n * NW + wave) * 1024;
+    if (n < 2) voff[n] = (uint32_t)((wb / (8 * FB)) * a_ks + wb % (8 * FB) + lane * 16);
     else voff[n] = (uint32_t)(((wb - A_STAGE) / (8 * FB)) * b_ks + (wb - A_STAGE) % (8 * FB) + lane * 16);
   }
 
@@ -375,7 +382,7 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
     const uint32_t sa = (uint32_t)((size_t)st * 4 * a_ks), sb = (uint32_t)((size_t)st * 4 * b_ks);
 #pragma unroll
     for (int n = 0; n < NI; n++)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(n < 2 ? ra : rb, (lds_ptr_t)(dst + (n * 4 + wave) * 1024), 16,
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(n < 2 ? ra : rb, (lds_ptr_t)(dst + (n * NW + wave) * 1024), 16,
                                                voff[n], n < 2 ? sa : sb, 0, 0);
   };
   struct Ops {
@@ -386,7 +393,7 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
     for (int ks = 0; ks < 4; ks++) {
 #pragma unroll
       for (int x = 0; x < 2; x++)
-        o.af[ks][x] = *reinterpret_cast<const double*>(stg + (ks * 4 + wr * 2 + x) * FB + lane * 8);
+        o.af[ks][x] = *reinterpret_cast<const double*>(stg + (ks * 8 + wr * 2 + x) * FB + lane * 8);
 #pragma unroll
       for (int y = 0; y < 4; y++)
         o.bf[ks][y] = *reinterpret_cast<const double*>(stg + A_STAGE + (ks * 8 + wc * 4 + y) * FB + lane * 8);
@@ -401,10 +408,6 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
         for (int y = 0; y < 4; y++)
           acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(o.af[ks][x], o.bf[ks][y], acc[x][y], 0, 0, 0);
   };
-  // same schedule as the pair kernel: wait for the next stage and for this
-  // wave's reads of the current one, barrier, refill the freed buffer three
-  // stages ahead, read the next stage under the current stage's MFMAs, one
-  // DMA + four LDS reads per five or six MFMAs
   auto step = [&](int64_t st, Ops& cur, Ops& nxt, unsigned char* freed, const unsigned char* next) {
     RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
     pipeline_barrier();
@@ -414,8 +417,8 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
 #pragma unroll
     for (int g = 0; g < NI; g++) {
       __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
-      __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
+      __builtin_amdgcn_sched_group_barrier(0x0100, 6, 0);
+      __builtin_amdgcn_sched_group_barrier(0x0008, 8, 0);
     }
   };
   Ops oa, ob;
@@ -447,13 +450,8 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
   __syncthreads();  // every wave is done with the stage buffers before a next unit refills them
 }
 
-__global__ __launch_bounds__(256) void transform_kernel(TransformArgs p) {
-  constexpr int FB = 512;                     // bytes per fragment
-  constexpr int A_STAGE = 4 * 4 * FB;         // 4 k-steps x 4 row blocks
-  constexpr int B_STAGE = 4 * 8 * FB;         // 4 k-steps x 8 column blocks
-  constexpr int SB = A_STAGE + B_STAGE;       // 24 KB
-  constexpr int NI = SB / 4096;               // 6 DMA transfers per wave per stage (0-1: L^{-1}, 2-5: X)
-  static_assert(A_STAGE == 2 * 4096, "transfers 0 and 1 of every wave carry the L^{-1} slab");
+__global__ __launch_bounds__(512) void transform_kernel(TransformArgs p) {
+  constexpr int SB = 32768;
   __shared__ __align__(16) unsigned char buf0[SB];
   __shared__ __align__(16) unsigned char buf1[SB];
   __shared__ __align__(16) unsigned char buf2[SB];
@@ -589,7 +587,7 @@ void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   p.Xp = static_cast<const unsigned char*>(Xp);
   p.B = B;
   const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
-  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(256), 0, c->stream, p);
+  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, c->stream, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
