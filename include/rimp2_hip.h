@@ -144,6 +144,14 @@ int rimp2_batch(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode
  * replays the captured graph (pair kernel + reductions) and leaves energies in
  * device memory (d_e_frag [nfrag]). */
 int rimp2_batch_prepare(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode);
+/* As rimp2_batch_prepare, with every array of the descriptors in device memory
+ * (inputs already in HBM: no host copies).  Input checks (non-finite values,
+ * denominators, positive definiteness) run on the device and are reported per
+ * fragment when the call returns.  Serves the reference's fragment loop
+ * (`do ifile=1,2`, RIMP2.GPU.F90:52) when the caller already holds each
+ * fragment's arrays on the device, where the reference reads them from file
+ * into host memory per fragment (RIMP2.GPU.F90:83-88). */
+int rimp2_batch_prepare_device(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode);
 int rimp2_batch_run(void* ctx, double* d_e_frag);
 int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed);
 /* Tile size chosen by the autotuner for the prepared batch (32 or 64). */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "rimp2_get_B": (ctypes.c_int, [_vp, _dp]),
     "rimp2_batch": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(FragDesc), ctypes.c_int, _dp, _dp]),
     "rimp2_batch_prepare": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(FragDesc), ctypes.c_int]),
+    "rimp2_batch_prepare_device": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(FragDesc), ctypes.c_int]),
     "rimp2_batch_run": (ctypes.c_int, [_vp, _vp]),
     "rimp2_batch_result": (ctypes.c_int, [_vp, _dp, _dp]),
     "rimp2_batch_tile": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
@@ -332,6 +333,17 @@ class Context:
         self._chk(lib().rimp2_batch_prepare(self.h, len(frags), arr, _mode(mode)))
         self._batch_n = len(frags)
         self._batch_npp = sum(int(f[2].size) ** 2 for f in frags)
+
+    def batch_prepare_device(self, frags, mode="fp32"):
+        """frags: (naux, nvirt, nocc, d_Pia, d_V or None, d_eps_o, d_eps_v)
+        with device pointers (ints, from alloc)."""
+        arr = (FragDesc * len(frags))()
+        for k, (naux, nvirt, nocc, dP, dV, deo, dev) in enumerate(frags):
+            arr[k] = FragDesc(nocc, nvirt, naux, ctypes.cast(dP, _dp), None if dV is None else ctypes.cast(dV, _dp),
+                              ctypes.cast(deo, _dp), ctypes.cast(dev, _dp))
+        self._chk(lib().rimp2_batch_prepare_device(self.h, len(frags), arr, _mode(mode)))
+        self._batch_n = len(frags)
+        self._batch_npp = sum(int(f[2]) ** 2 for f in frags)
 
     def batch_run(self, d_e_frag=None):
         self._chk(lib().rimp2_batch_run(self.h, d_e_frag))

@@ -65,7 +65,7 @@ static hipEvent_t get_event(Ctx* c) {
 }
 
 ScopedTimer::ScopedTimer(Ctx* ctx, int k) : c(ctx), kid(k) {
-  if (!c->timing) return;
+  if (!c || !c->timing) return;
   e0 = get_event(c);
   e1 = get_event(c);
   RIMP2_HIP_CHECK(hipEventRecord(e0, c->stream));

@@ -475,18 +475,43 @@ __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
 
 }  // namespace
 
-// dV_in (device, naux x naux) -> c->V holds L (lower, zeros above),
-// c->Linv holds L^{-1}.  A non-positive pivot sets bit 2 of the context's
-// status word (cleared by the API entry points, read by check_status).
-void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
-  hipStream_t s = c->stream;
-  double* L = static_cast<double*>(c->V.ensure((size_t)(n * n) * sizeof(double)));
-  double* Li = static_cast<double*>(c->Linv.ensure((size_t)(n * n) * sizeof(double)));
-  int* status = static_cast<int*>(c->status.ensure(64));
+// Bytes of L^{-1} in A-fragment order, and of one X in B-fragment order.
+static size_t linv_packed_bytes(int64_t n) {
+  return (size_t)(round_up(n, TR_KQ) / 4) * (size_t)(round_up(n, TR_TM) / 16) * 64 * sizeof(double);
+}
+static size_t x_packed_bytes(int64_t n, int64_t nvirt, int64_t nocc) {
+  return (size_t)(round_up(n, TR_KQ) / 4) * (size_t)(round_up(nvirt, TR_TN) / 16) * 64 * sizeof(double) *
+         (size_t)nocc;
+}
+
+void metric_reserve(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc) {
+  const size_t nn = (size_t)(n * n) * sizeof(double);
+  m.L.ensure(nn);
+  m.Linv.ensure(nn);
+  m.work.ensure(nn);
+  m.linv_packed.ensure(linv_packed_bytes(n));
+  m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
+}
+
+MetricView ctx_metric_view(Ctx* c) {
+  return MetricView{c->stream, c->V, c->Linv, c->work, c->linv_packed, c->x_packed,
+                    static_cast<int*>(c->status.ensure(64)), c};
+}
+
+// dV_in (device, n x n) -> m.L holds L (lower, zeros above), m.Linv holds
+// L^{-1}, m.linv_packed L^{-1} in transform-operand order.  A non-positive
+// pivot sets bit 2 of m.status (cleared by the caller) and m.status[1] to the
+// first failing pivot.  Buffers grow on demand: a caller that queues work for
+// several inputs on one view sizes them for the largest first.
+void metric_factor(const MetricView& m, int64_t n, const double* dV_in) {
+  hipStream_t s = m.stream;
+  double* L = static_cast<double*>(m.L.ensure((size_t)(n * n) * sizeof(double)));
+  double* Li = static_cast<double*>(m.Linv.ensure((size_t)(n * n) * sizeof(double)));
+  int* status = m.status;
   if (dV_in != L) RIMP2_HIP_CHECK(hipMemcpyAsync(L, dV_in, (size_t)(n * n) * sizeof(double), hipMemcpyDeviceToDevice, s));
   RIMP2_HIP_CHECK(hipMemsetAsync(Li, 0, (size_t)(n * n) * sizeof(double), s));
   {
-    ScopedTimer t(c, RIMP2_K_CHOLESKY);
+    ScopedTimer t(m.timer, RIMP2_K_CHOLESKY);
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
       const int kb = (int)((n - k0) < NB ? (n - k0) : NB);
       hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, L, n, Li, n, k0, kb, status);
@@ -511,9 +536,9 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
     // The groups of one level are independent and (but for a ragged last
     // one) equally shaped with constant pointer strides, so each level is one
     // batched launch per product.
-    ScopedTimer t(c, RIMP2_K_TRTRI);
+    ScopedTimer t(m.timer, RIMP2_K_TRTRI);
     const int64_t nblk = ceil_div(n, NB);
-    double* T = static_cast<double*>(c->work.ensure((size_t)(n * n) * sizeof(double)));
+    double* T = static_cast<double*>(m.work.ensure((size_t)(n * n) * sizeof(double)));
     for (int64_t w = 1; w < nblk; w *= 2) {
       const int64_t half = w * NB, span = 2 * half, nfull = n / span, gs = span * (1 + n);
       if (nfull > 0) {
@@ -537,11 +562,16 @@ void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
   {
     // L^{-1} in A-fragment order for the transform GEMM
     const int64_t MB = round_up(n, TR_TM) / 16, KS = round_up(n, TR_KQ) / 4;
-    double* Lp = static_cast<double*>(c->linv_packed.ensure((size_t)(KS * MB * 64) * sizeof(double)));
+    double* Lp = static_cast<double*>(m.linv_packed.ensure(linv_packed_bytes(n)));
     hipLaunchKernelGGL(pack_linv_kernel, dim3(grid_blocks(KS * MB * 64, 256)), dim3(256), 0, s, Li, n, MB, KS,
                        Lp);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
+}
+
+// The context's metric: its own buffers and stream; c->naux records the size.
+void metric_factor(Ctx* c, int64_t n, const double* dV_in) {
+  metric_factor(ctx_metric_view(c), n, dV_in);
   c->naux = n;
 }
 
@@ -550,11 +580,9 @@ void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
   gemm_f64<128, 128, true, false, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-// B64 = Linv * X  (X: naux x (nvirt*nocc) device, column-major): X is packed
+// dB = Linv * X  (X: n x (nvirt*nocc) device, column-major): X is packed
 // into B-fragment order, then one DMA-pipelined FP64 MFMA GEMM.
-void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
-  const int64_t n = c->naux, ncol = nvirt * nocc;
-  double* B = static_cast<double*>(c->B64.ensure((size_t)(n * ncol) * sizeof(double)));
+void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* B) {
   TransformArgs p;
   p.naux = n;
   p.nvirt = nvirt;
@@ -580,15 +608,22 @@ void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   gx.nqp = p.KS * 4;
   gx.nks = p.KS;
   gx.nrb = p.NRB;
-  void* Xp = c->x_packed.ensure(gx.bytes_per_occ() * (size_t)nocc);
-  ScopedTimer t(c, RIMP2_K_TRANSFORM);
-  pack_B(c, dX, n, nvirt, nocc, gx, Xp, nullptr, c->stream);
-  p.Lp = c->linv_packed.as<unsigned char>();
+  void* Xp = m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
+  ScopedTimer t(m.timer, RIMP2_K_TRANSFORM);
+  pack_B(m.timer, dX, n, nvirt, nocc, gx, Xp, nullptr, m.stream);
+  p.Lp = m.linv_packed.as<unsigned char>();
   p.Xp = static_cast<const unsigned char*>(Xp);
   p.B = B;
   const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
-  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, c->stream, p);
+  hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, m.stream, p);
   RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+// c->B64 = Linv * X for the context's metric (metric_factor(c, ...) first).
+void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
+  const int64_t n = c->naux;
+  double* B = static_cast<double*>(c->B64.ensure((size_t)(n * nvirt * nocc) * sizeof(double)));
+  metric_transform(ctx_metric_view(c), n, nvirt, nocc, dX, B);
 }
 
 }  // namespace rimp2

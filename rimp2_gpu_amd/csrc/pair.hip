@@ -232,18 +232,24 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
-void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what) {
-  (void)what;
-  int* status = static_cast<int*>(c->status.ensure(64));
+void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n) {
   const unsigned blocks = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(1, ceil_div(n, 256 * 8)));
-  hipLaunchKernelGGL(finite_check_kernel, dim3(blocks), dim3(256), 0, c->stream, p, n, status);
+  hipLaunchKernelGGL(finite_check_kernel, dim3(blocks), dim3(256), 0, s, p, n, status);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
-void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev, int64_t nvirt) {
-  int* status = static_cast<int*>(c->status.ensure(64));
-  hipLaunchKernelGGL(denom_check_kernel, dim3(1), dim3(64), 0, c->stream, eo, nocc, ev, nvirt, status);
+void check_denom_on(hipStream_t s, int* status, const double* eo, int64_t nocc, const double* ev, int64_t nvirt) {
+  hipLaunchKernelGGL(denom_check_kernel, dim3(1), dim3(64), 0, s, eo, nocc, ev, nvirt, status);
   RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what) {
+  (void)what;
+  check_finite_on(c->stream, static_cast<int*>(c->status.ensure(64)), p, n);
+}
+
+void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev, int64_t nvirt) {
+  check_denom_on(c->stream, static_cast<int*>(c->status.ensure(64)), eo, nocc, ev, nvirt);
 }
 
 }  // namespace rimp2

@@ -167,10 +167,11 @@ __host__ __device__ inline int64_t pair_units(int64_t nb, bool packed) {
 
 enum UnitKind { UNIT_TILE_PAIR = 0, UNIT_TWO_DIAG = 1, UNIT_ONE_DIAG = 2 };
 
-// Unit u of an occupied pair -> blocks (A, B) and its kind: a tile pair
-// (A < B, or A <= B without packing), two diagonal blocks A and B = A + 1, or
-// the last diagonal block A alone (odd nb; B = A, its second tile unused).
-__device__ inline int unit_decode(int64_t u, int64_t nb, bool packed, int64_t& A, int64_t& B) {
+// Unit u of a triangle of nb blocks in flat order -> blocks (A, B) and its
+// kind: a tile pair (A < B, or A <= B without packing), two diagonal blocks A
+// and B = A + 1, or the last diagonal block A alone (odd nb; B = A, its second
+// tile unused).
+__device__ inline int unit_decode_flat(int64_t u, int64_t nb, bool packed, int64_t& A, int64_t& B) {
   if (!packed) {
     tri_decode(u, A, B);
     return UNIT_TILE_PAIR;
@@ -188,6 +189,46 @@ __device__ inline int unit_decode(int64_t u, int64_t nb, bool packed, int64_t& A
     return UNIT_ONE_DIAG;
   }
   return UNIT_TWO_DIAG;
+}
+
+// Super-tile edge (in blocks) of the unit order below.
+constexpr int64_t UNIT_SUPER = 8;
+
+// Unit u of an occupied pair -> blocks (A, B) and its kind.  The triangle of
+// nb x nb blocks is cut into super-tiles of UNIT_SUPER x UNIT_SUPER blocks,
+// taken column by column (super-column GB: the off-diagonal super-tiles
+// GA < GB, then the diagonal one).  An off-diagonal super-tile of 8 x gB
+// blocks lists its units row by row (A outer, B inner); a diagonal one uses
+// the flat order over its own gB blocks.  Workgroups that run at the same time
+// on one XCD take consecutive units; a run of 32 of them then reads 16 or 24
+// distinct operand slabs (46 on average in the flat order at nb = 32), so
+// more of the operand stream is served from that XCD's L2.
+// For nb <= UNIT_SUPER the order is the flat one.
+__device__ inline int unit_decode(int64_t u, int64_t nb, bool packed, int64_t& A, int64_t& B) {
+  const int64_t G = UNIT_SUPER;
+  const int64_t ng = (nb + G - 1) / G;
+  // units before full super-column g: sum_{c<g} (G*G*c + dG) = 32 g(g-1) + dG g
+  const int64_t dG = packed ? G * (G - 1) / 2 + G / 2 : G * (G + 1) / 2;
+  auto before = [&](int64_t g) { return (G * G / 2) * g * (g - 1) + dG * g; };
+  const double b = (double)(dG - G * G / 2);
+  int64_t gb = (int64_t)((-b + sqrt(b * b + 2.0 * G * G * (double)u)) / (double)(G * G));
+  if (gb > ng - 1) gb = ng - 1;
+  if (gb < 0) gb = 0;
+  while (gb > 0 && before(gb) > u) gb--;
+  while (gb + 1 < ng && before(gb + 1) <= u) gb++;
+  int64_t v = u - before(gb);
+  const int64_t gB = (nb - G * gb < G) ? nb - G * gb : G;
+  if (v < gb * G * gB) {
+    const int64_t ga = v / (G * gB), w = v % (G * gB);
+    A = G * ga + w / gB;
+    B = G * gb + w % gB;
+    return UNIT_TILE_PAIR;
+  }
+  v -= gb * G * gB;
+  const int kind = unit_decode_flat(v, gB, packed, A, B);
+  A += G * gb;
+  B += G * gb;
+  return kind;
 }
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;

@@ -97,7 +97,8 @@ struct Ctx {
   MlpState* mlp = nullptr;
 };
 
-// Records hipEvents around a launch on ctx->stream when timing is on.
+// Records hipEvents around a launch on ctx->stream when timing is on (no-op
+// for a null ctx).
 struct ScopedTimer {
   Ctx* c;
   int kid;
@@ -107,6 +108,24 @@ struct ScopedTimer {
 };
 
 // --- metric (metric.hip) ---
+// The stream and scratch buffers one metric pipeline (Cholesky, triangular
+// inverse, transform) runs on.  The context's own view uses its stream and
+// buffers; a batch runs several lanes of its own side by side.
+struct MetricView {
+  hipStream_t stream;
+  DevBuf &L, &Linv, &work, &linv_packed, &x_packed;
+  int* status;  // device: [0] error bits, [1] first failing pivot (1-based)
+  Ctx* timer;   // context whose kernel timers record the launches (or null)
+};
+MetricView ctx_metric_view(Ctx* c);
+// Grows m's buffers to what metric_factor(n) and metric_transform(n, nvirt,
+// nocc) use (grow-only), so that work queued on m for several inputs of up to
+// these sizes never reallocates a buffer still in use.
+void metric_reserve(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc);
+// L L^T = V, L^{-1}, and L^{-1} in transform-operand order (dV_in: device, n x n)
+void metric_factor(const MetricView& m, int64_t n, const double* dV_in);
+// dB[n, nvirt, nocc] = L^{-1} X  (after metric_factor on the same view)
+void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* dB);
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
 // C = alpha * A * B^T + beta * C (column-major FP64 MFMA GEMM)
@@ -127,6 +146,10 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
                           int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
                           double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
+// Status bits on a given stream: 4 = a non-finite value among p[0..n);
+// 8 = max(eo) >= min(ev) (some pair denominator >= 0).
+void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n);
+void check_denom_on(hipStream_t s, int* status, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
 
 // --- batch / mlp cleanup ---
 void batch_destroy(BatchState* b);

@@ -139,3 +139,66 @@ def test_batch_mixes_fitted_and_metric_fragments(ctx, oracle, rimp2, mode):
             assert abs(ef[k] - ref[k]) < TOL_ABS[mode], (k, ef[k], ref[k])
         else:
             assert abs(ef[k] - ref[k]) <= TOL_REL[mode] * abs(ref[k]), (k, ef[k], ref[k])
+
+
+def _to_device_descs(ctx, frags):
+    descs, ptrs = [], []
+    for P, V, eo, ev in frags:
+        dP, deo, dev = ctx.to_device(P), ctx.to_device(eo), ctx.to_device(ev)
+        dV = None if V is None else ctx.to_device(V)
+        ptrs += [p for p in (dP, dV, deo, dev) if p is not None]
+        naux, nvirt, nocc = P.shape
+        descs.append((naux, nvirt, nocc, dP, dV, deo, dev))
+    return descs, ptrs
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5"])
+def test_batch_prepare_device_matches_host(ctx, rimp2, mode):
+    """Descriptors holding device pointers (inputs already in HBM) give the
+    fragment energies and pair matrices of host descriptors bit for bit (the
+    same kernels run on the same data); a fitted fragment (V = NULL) is among
+    them."""
+    frags = make_batch(rimp2, 41, 7, scale=4)
+    P, V, eo, ev = frags[3]
+    frags[3] = (np.asfortranarray(np.random.default_rng(2).standard_normal(P.shape) * 0.05), None, eo, ev)
+    say("batch prepare device", mode, [f[0].shape for f in frags])
+    ctx.batch_prepare(frags, mode)
+    ctx.batch_run()
+    ef_h, pp_h = ctx.batch_result()
+    descs, ptrs = _to_device_descs(ctx, frags)
+    try:
+        ctx.batch_prepare_device(descs, mode)
+        ctx.batch_run()
+        ef_d, pp_d = ctx.batch_result()
+    finally:
+        for p in ptrs:
+            ctx.free(p)
+    np.testing.assert_array_equal(ef_d, ef_h)
+    np.testing.assert_array_equal(pp_d, pp_h)
+
+
+def test_batch_prepare_device_errors(ctx, rimp2):
+    """The device-side input checks name the failing fragment: a metric that
+    is not positive definite, a non-finite coefficient, a denominator >= 0;
+    the context stays usable."""
+    good = rimp2.gen_fragment(3, 40, 10, 3)
+    P, V, eo, ev = good
+    Vbad = V.copy(order="F")
+    Vbad[5, 5] = -1.0
+    Pbad = P.copy(order="F")
+    Pbad[1, 2, 0] = np.nan
+    cases = [((P, Vbad, eo, ev), rimp2.ERR_NOT_SPD), ((Pbad, V, eo, ev), rimp2.ERR_NUMERIC),
+             ((P, V, eo, ev - 10.0), rimp2.ERR_NUMERIC)]
+    for bad, code in cases:
+        say("batch prepare device error", code)
+        descs, ptrs = _to_device_descs(ctx, [good, bad])
+        try:
+            with pytest.raises(rimp2.Rimp2Error) as ei:
+                ctx.batch_prepare_device(descs, "fp64")
+        finally:
+            for p in ptrs:
+                ctx.free(p)
+        assert ei.value.code == code
+        assert "fragment 1" in str(ei.value)
+    ef, _ = ctx.batch([good], "fp64")
+    assert ef[0] < 0

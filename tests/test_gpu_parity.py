@@ -83,7 +83,10 @@ def test_modes_against_golden(ctx, rimp2, mode, fname):
     assert np.all(eps < 0)
 
 
-EDGE = [(16, 1, 1), (8, 3, 1), (20, 33, 2), (37, 65, 3), (64, 130, 5), (5, 2, 4), (48, 100, 1)]
+# The last two run more than one super-tile column of 8 x 8 virtual blocks
+# (700: 11 blocks of 64, the last column 3 wide; 1090: 18 blocks, last 2 wide).
+EDGE = [(16, 1, 1), (8, 3, 1), (20, 33, 2), (37, 65, 3), (64, 130, 5), (5, 2, 4), (48, 100, 1), (96, 700, 3),
+        (40, 1090, 2)]
 
 
 @pytest.mark.parametrize("mode", ["fp64", "i8s5", "fp32", "bf16x3"])
