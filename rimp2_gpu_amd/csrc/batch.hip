@@ -38,29 +38,6 @@ struct BatchFrag {
   int64_t orig;      // index of the fragment in the caller's list
 };
 
-// Side lanes of the batch prepare.  A fragment's metric pipeline is a chain
-// of small launches (each 64-wide Cholesky step is three kernels of one to a
-// few dozen workgroups), so one stream leaves most of the GPU idle; the lanes
-// run several fragments' chains at once, each lane on a default-priority
-// stream of its own with the scratch buffers it reuses fragment after
-// fragment.  Four lanes: a process gets four hardware queues by default, and
-// further streams would share them.
-constexpr int kBatchLanes = 4;
-
-struct BatchLane {
-  hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
-  DevBuf L, Linv, work, linv_packed, x_packed, X;
-  MetricView view(int* st) { return MetricView{stream, L, Linv, work, linv_packed, x_packed, st, nullptr}; }
-  void release() {
-    for (DevBuf* b : {&L, &Linv, &work, &linv_packed, &x_packed, &X}) b->release();
-    if (done) (void)hipEventDestroy(done);
-    if (stream) (void)hipStreamDestroy(stream);
-    done = nullptr;
-    stream = nullptr;
-  }
-};
-
 struct BatchState {
   int mode = -1;
   int tile = 0;
@@ -76,16 +53,15 @@ struct BatchState {
   double tune_ms[4] = {0, 0, 0, 0};
   int tune_tiles[4] = {0, 0, 0, 0};
   int ntune = 0;
-  BatchLane lanes[kBatchLanes];
-  hipEvent_t lanes_go = nullptr;  // recorded on the context stream once the lanes may start
-  DevBuf fstatus;  // two status ints per fragment (bits as the context's status word)
+  MetricGroup group;  // grouped metric pipeline of the prepare
+  DevBuf stage;       // host inputs staged on the device (V, then X, of the fragments with a metric)
+  DevBuf fstatus;     // two status ints per fragment (bits as the context's status word)
   ~BatchState() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags, &fstatus})
+    for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags, &stage, &fstatus})
       b->release();
-    for (BatchLane& l : lanes) l.release();
-    if (lanes_go) (void)hipEventDestroy(lanes_go);
+    group.release();
   }
 };
 
@@ -379,9 +355,12 @@ int size_class(const BatchState* b) {
   return k;
 }
 
-// Fragment-wise metric, transform and staging of B into the b64 arena, spread
-// over the side lanes; on_device: every pointer of the descriptors is device
-// memory (inputs already in HBM), else host memory.
+// Metric, transform and staging of B into the b64 arena for every fragment:
+// one grouped pipeline (one launch per Cholesky step, inverse level, packing
+// and transform for all fragments together); on_device: every pointer of
+// the descriptors is device memory (inputs already in HBM), else host memory
+// (staged on the device first).  Input checks set per-fragment status words,
+// read once at the end.
 void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool on_device) {
   if (nfrag <= 0 || !frags) fail(RIMP2_ERR_ARG, "empty fragment list");
   if (mode < 0 || mode >= RIMP2_NUM_MODES) fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
@@ -394,7 +373,7 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
   b->nvirt.resize(nfrag);
   b->naux.resize(nfrag);
   b->b64_off.resize(nfrag);
-  size_t b64n = 0, occ_total = 0, virt_total = 0;
+  size_t b64n = 0, occ_total = 0, virt_total = 0, stage_n = 0;
   for (int64_t f = 0; f < nfrag; f++) {
     const rimp2_frag_desc& d = frags[f];
     if (d.nocc <= 0 || d.nvirt <= 0 || d.naux <= 0 || d.nocc > 65536 || d.nvirt > 65536 || d.naux > 65536)
@@ -408,6 +387,7 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
       for (int64_t a = 0; a < d.nvirt; a++) mv = fmin(mv, d.eps_v[a]);
       if (!(2.0 * mo - 2.0 * mv < 0.0) || !std::isfinite(mo) || !std::isfinite(mv))
         fail(RIMP2_ERR_NUMERIC, "fragment %lld: orbital-energy denominator >= 0 or non-finite", (long long)f);
+      if (d.V) stage_n += (size_t)(d.naux * d.naux + d.naux * d.nvirt * d.nocc);
     }
     b->nocc[f] = d.nocc;
     b->nvirt[f] = d.nvirt;
@@ -417,112 +397,47 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
     occ_total += (size_t)d.nocc;
     virt_total += (size_t)d.nvirt;
   }
-  // Lane plan: fragments in decreasing estimated time, each to the lane that
-  // frees first (a 64-wide Cholesky step costs ~70 us of dependent launches;
-  // the transform runs at ~40 FP64 TFLOP/s).
-  auto est = [&](int64_t f) {
-    const rimp2_frag_desc& d = frags[f];
-    const double n = (double)d.naux;
-    if (!d.V) return 1e-6;
-    return 70e-6 * std::ceil(n / 64.0) + n * n * (double)(d.nvirt * d.nocc) / 40e12;
-  };
-  std::vector<int64_t> byc(nfrag);
-  std::iota(byc.begin(), byc.end(), 0);
-  std::stable_sort(byc.begin(), byc.end(), [&](int64_t x, int64_t y) { return est(x) > est(y); });
-  std::vector<int> lane_of(nfrag);
-  {
-    double load[kBatchLanes] = {0};
-    for (int64_t f : byc) {
-      int l = 0;
-      for (int k = 1; k < kBatchLanes; k++)
-        if (load[k] < load[l]) l = k;
-      lane_of[f] = l;
-      load[l] += est(f);
-    }
-  }
   double* b64 = static_cast<double*>(b->b64.ensure(b64n * sizeof(double)));
   double* eo = static_cast<double*>(b->eps_o.ensure((occ_total + virt_total) * sizeof(double)));
   double* ev = eo + occ_total;
   int* fst = static_cast<int*>(b->fstatus.ensure((size_t)nfrag * 2 * sizeof(int)));
-  // Size every lane's buffers for its largest fragment before anything is
-  // queued on it: a buffer that grew between two fragments of one lane
-  // would be freed under the earlier fragment's kernels.
-  for (int k = 0; k < kBatchLanes; k++) {
-    BatchLane& l = b->lanes[k];
-    if (!l.stream) RIMP2_HIP_CHECK(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
-    if (!l.done) RIMP2_HIP_CHECK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
-  }
-  if (!b->lanes_go) RIMP2_HIP_CHECK(hipEventCreateWithFlags(&b->lanes_go, hipEventDisableTiming));
-  for (int64_t f = 0; f < nfrag; f++) {
-    const rimp2_frag_desc& d = frags[f];
-    if (!d.V) continue;
-    BatchLane& l = b->lanes[lane_of[f]];
-    metric_reserve(l.view(nullptr), d.naux, d.nvirt, d.nocc);
-    if (!on_device) l.X.ensure((size_t)(d.naux * d.nvirt * d.nocc) * sizeof(double));
-  }
+  double* stage = stage_n ? static_cast<double*>(b->stage.ensure(stage_n * sizeof(double))) : nullptr;
   hipStream_t s = c->stream;
   RIMP2_HIP_CHECK(hipMemsetAsync(fst, 0, (size_t)nfrag * 2 * sizeof(int), s));
-  const hipMemcpyKind in_kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  {
-    size_t o = 0, v = 0;
-    for (int64_t f = 0; f < nfrag; f++) {
-      const rimp2_frag_desc& d = frags[f];
-      RIMP2_HIP_CHECK(hipMemcpyAsync(eo + o, d.eps_o, (size_t)d.nocc * sizeof(double), in_kind, s));
-      RIMP2_HIP_CHECK(hipMemcpyAsync(ev + v, d.eps_v, (size_t)d.nvirt * sizeof(double), in_kind, s));
-      if (on_device) {
-        check_finite_on(s, fst + 2 * f, eo + o, d.nocc);
-        check_finite_on(s, fst + 2 * f, ev + v, d.nvirt);
-        check_denom_on(s, fst + 2 * f, eo + o, d.nocc, ev + v, d.nvirt);
-      }
-      o += (size_t)d.nocc;
-      v += (size_t)d.nvirt;
-    }
-  }
-  // the lanes start once the status words are cleared
-  RIMP2_HIP_CHECK(hipEventRecord(b->lanes_go, s));
-  for (BatchLane& l : b->lanes) RIMP2_HIP_CHECK(hipStreamWaitEvent(l.stream, b->lanes_go, 0));
-  // A failure while queueing lets the work already queued on the lanes finish
-  // before the error leaves (a later call may regrow their buffers).
-  auto drain = [&]() {
-    for (BatchLane& l : b->lanes) (void)hipStreamSynchronize(l.stream);
-  };
-  try {
-    for (int64_t f : byc) {
-      const rimp2_frag_desc& d = frags[f];
-      BatchLane& l = b->lanes[lane_of[f]];
-      int* st = fst + 2 * f;
-      const int64_t n = d.naux;
-      const size_t nx = (size_t)(d.naux * d.nvirt * d.nocc);
-      double* dst = b64 + b->b64_off[f];
+  std::vector<mg::Item> items;
+  std::vector<mg::Copy> copies;
+  std::vector<mg::Denom> denoms;
+  size_t o = 0, v = 0, so = 0;
+  for (int64_t f = 0; f < nfrag; f++) {
+    const rimp2_frag_desc& d = frags[f];
+    int* st = fst + 2 * f;
+    const int64_t nx = d.naux * d.nvirt * d.nocc;
+    double* dst = b64 + b->b64_off[f];
+    if (on_device) {
+      copies.push_back(mg::Copy{d.eps_o, eo + o, d.nocc, st});
+      copies.push_back(mg::Copy{d.eps_v, ev + v, d.nvirt, st});
+      if (d.V) items.push_back(mg::Item{d.naux, d.nvirt, d.nocc, d.V, d.Pia, dst, st});
+      else copies.push_back(mg::Copy{d.Pia, dst, nx, st});  // already fitted: Pia is B
+    } else {
+      RIMP2_HIP_CHECK(hipMemcpyAsync(eo + o, d.eps_o, (size_t)d.nocc * sizeof(double), hipMemcpyHostToDevice, s));
+      RIMP2_HIP_CHECK(hipMemcpyAsync(ev + v, d.eps_v, (size_t)d.nvirt * sizeof(double), hipMemcpyHostToDevice, s));
       if (d.V) {
-        const MetricView m = l.view(st);
-        const double* dV = d.V;
-        const double* dX = d.Pia;
-        if (!on_device) {
-          RIMP2_HIP_CHECK(hipMemcpyAsync(l.L.ptr, d.V, (size_t)(n * n) * sizeof(double), hipMemcpyHostToDevice,
-                                         l.stream));
-          RIMP2_HIP_CHECK(hipMemcpyAsync(l.X.ptr, d.Pia, nx * sizeof(double), hipMemcpyHostToDevice, l.stream));
-          dV = l.L.as<double>();
-          dX = l.X.as<double>();
-        }
-        check_finite_on(l.stream, st, dV, n * n);
-        check_finite_on(l.stream, st, dX, (int64_t)nx);
-        metric_factor(m, n, dV);
-        metric_transform(m, n, d.nvirt, d.nocc, dX, dst);
+        double* sV = stage + so;
+        double* sX = sV + d.naux * d.naux;
+        so += (size_t)(d.naux * d.naux + nx);
+        RIMP2_HIP_CHECK(hipMemcpyAsync(sV, d.V, (size_t)(d.naux * d.naux) * sizeof(double), hipMemcpyHostToDevice, s));
+        RIMP2_HIP_CHECK(hipMemcpyAsync(sX, d.Pia, (size_t)nx * sizeof(double), hipMemcpyHostToDevice, s));
+        items.push_back(mg::Item{d.naux, d.nvirt, d.nocc, sV, sX, dst, st});
       } else {
-        // already fitted: Pia is B
-        RIMP2_HIP_CHECK(hipMemcpyAsync(dst, d.Pia, nx * sizeof(double), in_kind, l.stream));
-        check_finite_on(l.stream, st, dst, (int64_t)nx);
+        RIMP2_HIP_CHECK(hipMemcpyAsync(dst, d.Pia, (size_t)nx * sizeof(double), hipMemcpyHostToDevice, s));
+        copies.push_back(mg::Copy{dst, dst, nx, st});
       }
     }
-  } catch (...) {
-    drain();
-    throw;
+    denoms.push_back(mg::Denom{eo + o, ev + v, d.nocc, d.nvirt, st});
+    o += (size_t)d.nocc;
+    v += (size_t)d.nvirt;
   }
-  for (BatchLane& l : b->lanes) {
-    RIMP2_HIP_CHECK(hipEventRecord(l.done, l.stream));
-    RIMP2_HIP_CHECK(hipStreamWaitEvent(s, l.done, 0));
-  }
+  metric_group_run(b->group, s, items, copies, denoms);
   std::vector<int> hst((size_t)nfrag * 2);
   RIMP2_HIP_CHECK(hipMemcpyAsync(hst.data(), fst, hst.size() * sizeof(int), hipMemcpyDeviceToHost, s));
   RIMP2_HIP_CHECK(hipStreamSynchronize(s));

@@ -6,6 +6,10 @@
 // The reference receives the fitted tensor Calpha(NBA,nvirt,nocc) from its
 // parent code (RIMP2.F90:48-49, "alphaCD ... in the diagonal basis"); this file
 // builds that tensor from the raw (P|ia) and the metric V_PQ.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
 #include "pair_core.h"
 
 namespace rimp2 {
@@ -16,7 +20,7 @@ using pairk::pipeline_barrier;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int NB = 64;  // Cholesky / inverse panel width
+constexpr int NB = (int)mg::kPanel;  // Cholesky / inverse panel width
 
 // ---------------------------------------------------------------------------
 // Generic column-major FP64 GEMM on MFMA:
@@ -34,15 +38,11 @@ constexpr int NB = 64;  // Cholesky / inverse panel width
 // LDS: As[k][m] with pitch BM+16 doubles and Bs[n][k] with pitch 17 doubles,
 // so both fragment reads are free of bank conflicts.
 // ---------------------------------------------------------------------------
+// One BM x BN tile (row tile mt, column tile nt) of C; all 256 threads.
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
-                                                       const double* A, int64_t lda, const double* B, int64_t ldb,
-                                                       double beta, double* C, int64_t ldc, int64_t sA,
-                                                       int64_t sB, int64_t sC) {
-  // batch member blockIdx.z (element strides sA, sB, sC between members)
-  A += (int64_t)blockIdx.z * sA;
-  B += (int64_t)blockIdx.z * sB;
-  C += (int64_t)blockIdx.z * sC;
+__device__ inline void gemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t mt,
+                                     int64_t nt) {
   constexpr int BK = 16;
   constexpr int PA = BM + 16;  // As[k][m]
   constexpr int PB = BK + 1;   // Bs[n][k]
@@ -53,8 +53,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
   __shared__ double As[BK * PA];
   __shared__ double Bs[BN * PB];
 
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
-  const int64_t n0 = (int64_t)blockIdx.y * BN;
+  const int64_t m0 = mt * BM;
+  const int64_t n0 = nt * BN;
   if (LOWER_C && n0 > m0 + BM - 1) return;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -187,6 +187,50 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
 }
 
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+                                                       const double* A, int64_t lda, const double* B, int64_t ldb,
+                                                       double beta, double* C, int64_t ldc, int64_t sA,
+                                                       int64_t sB, int64_t sC) {
+  // batch member blockIdx.z (element strides sA, sB, sC between members)
+  const int64_t z = blockIdx.z;
+  gemm_f64_tile<BM, BN, TRANSB, TRI_A, LOWER_C>(M, N, K, alpha, A + z * sA, lda, B + z * sB, ldb, beta, C + z * sC,
+                                                ldc, blockIdx.x, blockIdx.y);
+}
+
+// Job lookup of the grouped kernels: the last job j with jobs[j].first <= t
+// (jobs in ascending order of first, jobs[0].first = 0).
+template <class Job>
+__device__ inline int job_find(const Job* __restrict__ jobs, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Grouped GEMM: independent problems of any shapes in one launch (job
+// tables of metric_group.h); LOWER_C jobs list only the lower tiles,
+// t = mt (mt + 1) / 2 + nt.
+using mg::GemmJob;
+
+template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
+__global__ __launch_bounds__(256) void gemm_f64_group_kernel(const GemmJob* __restrict__ jobs, int njobs) {
+  const int64_t t = blockIdx.x;
+  const GemmJob& J = jobs[job_find(jobs, njobs, t)];
+  const int64_t u = t - J.first;
+  int64_t mt, nt;
+  if (LOWER_C) pairk::tri_decode(u, nt, mt);
+  else {
+    mt = u % J.tm;
+    nt = u / J.tm;
+  }
+  gemm_f64_tile<BM, BN, TRANSB, TRI_A, LOWER_C>(J.M, J.N, J.K, J.alpha, J.A, J.lda, J.B, J.ldb, J.beta, J.C, J.ldc,
+                                                mt, nt);
+}
+
+template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 void gemm_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t batch = 1, int64_t sA = 0,
               int64_t sB = 0, int64_t sC = 0) {
@@ -227,8 +271,8 @@ __device__ inline void wave_lds_sync() {
 // entries), so the arithmetic is lane-local; X goes through LDS to be stored
 // with coalesced writes by all four waves.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
-                                                         int64_t k0, int kb, int* status) {
+__device__ inline void potrf_diag_block(double* A, int64_t lda, double* Linv, int64_t ldi, int64_t k0, int kb,
+                                        int* status) {
   __shared__ double Ls[NB][NB + 1];
   __shared__ double Xs[NB][NB + 1];
   __shared__ __align__(16) double colb[2][NB];
@@ -298,6 +342,19 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda,
   }
 }
 
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
+                                                         int64_t k0, int kb, int* status) {
+  potrf_diag_block(A, lda, Linv, ldi, k0, kb, status);
+}
+
+// Grouped: workgroup g factors the diagonal block of job g.
+using mg::PotrfJob;
+
+__global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfJob* __restrict__ jobs) {
+  const PotrfJob& J = jobs[blockIdx.x];
+  potrf_diag_block(J.A, J.n, J.Linv, J.n, J.k0, J.kb, J.status);
+}
+
 // ---------------------------------------------------------------------------
 // Metric transform on packed operands.
 //   Lp[ks][mb][lane]: A fragments of L^{-1}, lane l <-> (m = 16 mb + l%16,
@@ -310,7 +367,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda,
 // 16-wide K stage); the K loop of a row tile stops at its last row (L^{-1} is
 // lower triangular).  Waves 4 x 2, wave tile 32 x 64 = 2 x 4 MFMA blocks.
 // ---------------------------------------------------------------------------
-constexpr int TR_TM = 128, TR_TN = 128, TR_KQ = 64;  // K padding: four 16-wide stages
+constexpr int TR_TM = (int)mg::kTrTM, TR_TN = (int)mg::kTrTN, TR_KQ = (int)mg::kTrKQ;  // K padding: four 16-wide stages
 
 __global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64_t MB, int64_t KS,
                                  double* __restrict__ Lp) {
@@ -322,16 +379,7 @@ __global__ void pack_linv_kernel(const double* __restrict__ Li, int64_t n, int64
   }
 }
 
-struct TransformArgs {
-  const unsigned char* Lp;  // packed L^{-1}
-  const unsigned char* Xp;  // packed X
-  double* B;                // [naux, nvirt, nocc]
-  int64_t naux, nvirt, nocc;
-  int64_t MB, KS;           // L^{-1} row blocks (16) and k-steps (4)
-  int64_t NRB;              // X row blocks (16 virtuals) per occupied
-  int64_t MT, NT;           // row tiles, column tiles per occupied
-  int64_t nwu, chunk;
-};
+using mg::TransformArgs;
 
 // One 128 (Q rows) x 128 (virtuals of occupied i) tile of B = L^{-1} X.
 // Eight waves on a 4 x 2 grid, wave tile 32 x 64 (2 x 4 MFMA blocks of 16x16),
@@ -473,25 +521,99 @@ __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grouped metric pipeline: the Cholesky steps, inverse levels, packing and
+// transform of many fragments, one launch per step for the whole group, the
+// per-fragment arguments in job tables in device memory.
+// ---------------------------------------------------------------------------
+
+// Copy src -> dst (dst == src: check only); bit 4 of status on a non-finite value.
+using mg::CopyJob;
+
+__global__ __launch_bounds__(256) void copy_check_group_kernel(const CopyJob* __restrict__ jobs, int njobs,
+                                                               int64_t total) {
+  RIMP2_GRID_STRIDE(t, total) {
+    const CopyJob& J = jobs[job_find(jobs, njobs, t)];
+    const int64_t e = t - J.first;
+    const double v = J.src[e];
+    if (J.dst != J.src) J.dst[e] = v;
+    if (!isfinite(v)) atomicOr(J.status, 4);
+  }
+}
+
+// One wave per fragment: bit 8 of status when max(eo) >= min(ev).
+using mg::DenomJob;
+
+__global__ __launch_bounds__(64) void denom_group_kernel(const DenomJob* __restrict__ jobs) {
+  const DenomJob& J = jobs[blockIdx.x];
+  const int lane = threadIdx.x;
+  double mo = -INFINITY, mv = INFINITY;
+  for (int64_t i = lane; i < J.nocc; i += 64) mo = fmax(mo, J.eo[i]);
+  for (int64_t a = lane; a < J.nvirt; a += 64) mv = fmin(mv, J.ev[a]);
+  for (int off = 32; off > 0; off >>= 1) {
+    mo = fmax(mo, __shfl_xor(mo, off));
+    mv = fmin(mv, __shfl_xor(mv, off));
+  }
+  if (lane == 0 && !(2.0 * mo - 2.0 * mv < 0.0)) atomicOr(J.status, 8);
+}
+
+// L^{-1} of one fragment into A-fragment order (as pack_linv_kernel).
+using mg::LinvPackJob;
+
+__global__ __launch_bounds__(256) void pack_linv_group_kernel(const LinvPackJob* __restrict__ jobs, int njobs,
+                                                              int64_t total) {
+  RIMP2_GRID_STRIDE(t, total) {
+    const LinvPackJob& J = jobs[job_find(jobs, njobs, t)];
+    const int64_t e = t - J.first;
+    const int lane = (int)(e & 63);
+    const int64_t f = e >> 6, mb = f % J.MB, ks = f / J.MB;
+    const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
+    J.Lp[e] = (m < J.n && k < J.n && k <= m) ? J.Li[m + J.n * k] : 0.0;
+  }
+}
+
+// X of one fragment into FP64 B-fragment order (the layout of pack.hip's
+// fp64 packing), bit 4 of status on a non-finite value.
+using mg::XPackJob;
+
+__global__ __launch_bounds__(256) void pack_x_group_kernel(const XPackJob* __restrict__ jobs, int njobs,
+                                                           int64_t total) {
+  RIMP2_GRID_STRIDE(t, total) {
+    const XPackJob& J = jobs[job_find(jobs, njobs, t)];
+    const int64_t e = t - J.first;
+    const int lane = (int)(e & 63);
+    const int64_t f = e >> 6;
+    const int64_t rb = f % J.nrb, ks = (f / J.nrb) % J.nks, i = f / (J.nrb * J.nks);
+    const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
+    const double v = (q < J.naux && a < J.nvirt) ? J.X[q + J.naux * (a + J.nvirt * i)] : 0.0;
+    J.Xp[e] = v;
+    if (!isfinite(v)) atomicOr(J.status, 4);
+  }
+}
+
+using mg::TransformJob;
+
+__global__ __launch_bounds__(512) void transform_group_kernel(const TransformJob* __restrict__ jobs, int njobs,
+                                                              int64_t total, int64_t chunk) {
+  constexpr int SB = 32768;
+  __shared__ __align__(16) unsigned char buf0[SB];
+  __shared__ __align__(16) unsigned char buf1[SB];
+  __shared__ __align__(16) unsigned char buf2[SB];
+  __shared__ __align__(16) unsigned char buf3[SB];
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+  for (int64_t k = slot; k < chunk; k += nslot) {
+    const int64_t wu = xcd * chunk + k;
+    if (wu >= total) break;
+    const int j = job_find(jobs, njobs, wu);
+    const TransformArgs a = jobs[j].a;
+    transform_unit(a, wu - jobs[j].first, buf0, buf1, buf2, buf3);
+  }
+}
+
 }  // namespace
 
-// Bytes of L^{-1} in A-fragment order, and of one X in B-fragment order.
-static size_t linv_packed_bytes(int64_t n) {
-  return (size_t)(round_up(n, TR_KQ) / 4) * (size_t)(round_up(n, TR_TM) / 16) * 64 * sizeof(double);
-}
-static size_t x_packed_bytes(int64_t n, int64_t nvirt, int64_t nocc) {
-  return (size_t)(round_up(n, TR_KQ) / 4) * (size_t)(round_up(nvirt, TR_TN) / 16) * 64 * sizeof(double) *
-         (size_t)nocc;
-}
-
-void metric_reserve(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc) {
-  const size_t nn = (size_t)(n * n) * sizeof(double);
-  m.L.ensure(nn);
-  m.Linv.ensure(nn);
-  m.work.ensure(nn);
-  m.linv_packed.ensure(linv_packed_bytes(n));
-  m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
-}
+using mg::linv_packed_bytes;
+using mg::x_packed_bytes;
 
 MetricView ctx_metric_view(Ctx* c) {
   return MetricView{c->stream, c->V, c->Linv, c->work, c->linv_packed, c->x_packed,
@@ -624,6 +746,88 @@ void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
   const int64_t n = c->naux;
   double* B = static_cast<double*>(c->B64.ensure((size_t)(n * nvirt * nocc) * sizeof(double)));
   metric_transform(ctx_metric_view(c), n, nvirt, nocc, dX, B);
+}
+
+
+// ---------------------------------------------------------------------------
+// Grouped metric pipeline (host side)
+// ---------------------------------------------------------------------------
+void MetricGroup::release() {
+  for (DevBuf* b : {&L, &Linv, &T, &Lp, &Xp, &jobs}) b->release();
+  blob.clear();
+}
+
+// Lays the arenas out, plans the job tables (metric_group.h), sends them to
+// the device in one copy and queues one launch per step.
+void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>& items,
+                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms) {
+  for (const mg::Item& it : items) {
+    const TransformArgs a = mg::transform_args(it.n, it.nvirt, it.nocc);
+    if ((size_t)a.KS * a.MB * 512 > 0xFFFFFFFFull || (size_t)a.KS * a.NRB * 512 > 0xFFFFFFFFull)
+      fail(RIMP2_ERR_ARG, "metric transform operands exceed 4 GiB per slab (naux=%lld nvirt=%lld)", (long long)it.n,
+           (long long)it.nvirt);
+  }
+  const mg::Layout g = mg::layout(items);
+  mg::Arenas ar;
+  ar.L = static_cast<unsigned char*>(G.L.ensure(g.ln ? g.ln : 256));
+  ar.Linv = static_cast<unsigned char*>(G.Linv.ensure(g.ln ? g.ln : 256));
+  ar.T = static_cast<unsigned char*>(G.T.ensure(g.ln ? g.ln : 256));
+  ar.Lp = static_cast<unsigned char*>(G.Lp.ensure(g.lpn ? g.lpn : 256));
+  ar.Xp = static_cast<unsigned char*>(G.Xp.ensure(g.xpn ? g.xpn : 256));
+  std::vector<mg::GroupStep> steps;
+  mg::plan(items, copies, denoms, g, ar, steps, G.blob);
+  if (steps.empty()) return;
+  unsigned char* dj = static_cast<unsigned char*>(G.jobs.ensure(G.blob.size()));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dj, G.blob.data(), G.blob.size(), hipMemcpyHostToDevice, s));
+  for (const mg::GroupStep& st : steps) {
+    const void* jp = dj + st.off;
+    const unsigned eb = grid_blocks(st.total, 256);
+    if (st.kind >= mg::GL_PANEL && st.kind <= mg::GL_TRI_L && st.total > kMaxGridBlocks)
+      fail(RIMP2_ERR_ARG, "grouped metric launch too large (%lld tiles)", (long long)st.total);
+    switch (st.kind) {
+      case mg::GL_COPY:
+        hipLaunchKernelGGL(copy_check_group_kernel, dim3(eb), dim3(256), 0, s, (const CopyJob*)jp, st.njobs, st.total);
+        break;
+      case mg::GL_DENOM:
+        hipLaunchKernelGGL(denom_group_kernel, dim3((unsigned)st.njobs), dim3(64), 0, s, (const DenomJob*)jp);
+        break;
+      case mg::GL_ZERO_LINV: RIMP2_HIP_CHECK(hipMemsetAsync(ar.Linv, 0, (size_t)st.total, s)); break;
+      case mg::GL_POTRF:
+        hipLaunchKernelGGL(potrf_diag_group_kernel, dim3((unsigned)st.njobs), dim3(256), 0, s, (const PotrfJob*)jp);
+        break;
+      case mg::GL_PANEL:  // L21 = A21 L11^{-T}
+        hipLaunchKernelGGL((gemm_f64_group_kernel<64, 64, true, false, false>), dim3((unsigned)st.total), dim3(256), 0,
+                           s, (const GemmJob*)jp, st.njobs);
+        break;
+      case mg::GL_SYRK:  // A22 -= L21 L21^T, lower tiles
+        hipLaunchKernelGGL((gemm_f64_group_kernel<64, 64, true, false, true>), dim3((unsigned)st.total), dim3(256), 0,
+                           s, (const GemmJob*)jp, st.njobs);
+        break;
+      case mg::GL_TRI_T:  // T = Q P^{-1}
+        hipLaunchKernelGGL((gemm_f64_group_kernel<64, 64, false, false, false>), dim3((unsigned)st.total), dim3(256),
+                           0, s, (const GemmJob*)jp, st.njobs);
+        break;
+      case mg::GL_TRI_L:  // Linv block = -R^{-1} T, R^{-1} lower triangular
+        hipLaunchKernelGGL((gemm_f64_group_kernel<64, 64, false, true, false>), dim3((unsigned)st.total), dim3(256),
+                           0, s, (const GemmJob*)jp, st.njobs);
+        break;
+      case mg::GL_LPACK:
+        hipLaunchKernelGGL(pack_linv_group_kernel, dim3(eb), dim3(256), 0, s, (const LinvPackJob*)jp, st.njobs,
+                           st.total);
+        break;
+      case mg::GL_XPACK:
+        hipLaunchKernelGGL(pack_x_group_kernel, dim3(eb), dim3(256), 0, s, (const XPackJob*)jp, st.njobs, st.total);
+        break;
+      case mg::GL_TRANSFORM: {
+        const int64_t chunk = ceil_div(st.total, (int64_t)8);
+        const int64_t nslot = chunk < kMaxGridBlocks / 8 ? chunk : kMaxGridBlocks / 8;
+        hipLaunchKernelGGL(transform_group_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, s,
+                           (const TransformJob*)jp, st.njobs, st.total, chunk);
+        break;
+      }
+    }
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
 }
 
 }  // namespace rimp2

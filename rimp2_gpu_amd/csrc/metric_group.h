@@ -1,0 +1,297 @@
+// Plan of the grouped metric pipeline: the job tables that let one launch per
+// step serve the Cholesky, triangular inverse, packing and transform of many
+// fragments at once.  Plain C++ (no HIP): the library queues the plan's steps
+// on a stream, and a host emulation checks the plan's indexing on the CPU.
+//
+// Per fragment (column-major, n = naux):
+//   L slot    n x n     V copied in, factored in place (lower part used)
+//   Linv slot n x n     L^{-1}: 64-wide diagonal blocks from the Cholesky step,
+//                       off-diagonal blocks by recursive doubling
+//   T slot    n x n     products of the doubling levels
+//   Lp        L^{-1} in MFMA A-fragment order, Xp  X in B-fragment order
+// Steps: copies with finiteness checks (every V into its L slot first),
+// denominator checks, Linv cleared, then for each 64-column step k0 of the
+// longest fragment: diagonal blocks (one workgroup per fragment still that
+// long), panels L21 = A21 L11^{-T}, trailing updates A22 -= L21 L21^T; then
+// per doubling level w (blocks of 64 w): T = Q P^{-1}, Linv[Q] = -R^{-1} T;
+// then the L^{-1} and X packings and the transform B = L^{-1} X.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace rimp2 {
+namespace mg {
+
+constexpr int64_t kPanel = 64;                             // Cholesky / inverse block width
+constexpr int64_t kTrTM = 128, kTrTN = 128, kTrKQ = 64;    // transform tile rows, columns, K padding
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
+
+// Bytes of L^{-1} in A-fragment order, and of one X in B-fragment order.
+inline size_t linv_packed_bytes(int64_t n) {
+  return (size_t)(rup(n, kTrKQ) / 4) * (size_t)(rup(n, kTrTM) / 16) * 64 * sizeof(double);
+}
+inline size_t x_packed_bytes(int64_t n, int64_t nvirt, int64_t nocc) {
+  return (size_t)(rup(n, kTrKQ) / 4) * (size_t)(rup(nvirt, kTrTN) / 16) * 64 * sizeof(double) * (size_t)nocc;
+}
+
+struct TransformArgs {
+  const unsigned char* Lp;  // packed L^{-1}
+  const unsigned char* Xp;  // packed X
+  double* B;                // [naux, nvirt, nocc]
+  int64_t naux, nvirt, nocc;
+  int64_t MB, KS;           // L^{-1} row blocks (16) and k-steps (4)
+  int64_t NRB;              // X row blocks (16 virtuals) per occupied
+  int64_t MT, NT;           // row tiles, column tiles per occupied
+  int64_t nwu, chunk;
+};
+
+inline TransformArgs transform_args(int64_t n, int64_t nvirt, int64_t nocc) {
+  TransformArgs a{};
+  a.naux = n;
+  a.nvirt = nvirt;
+  a.nocc = nocc;
+  a.MB = rup(n, kTrTM) / 16;
+  a.KS = rup(n, kTrKQ) / 4;
+  a.NRB = rup(nvirt, kTrTN) / 16;
+  a.MT = rup(n, kTrTM) / kTrTM;
+  a.NT = a.NRB / 8;
+  a.nwu = a.MT * a.NT * nocc;
+  a.chunk = cdiv(a.nwu, 8);
+  return a;
+}
+
+// C = alpha A op(B) + beta C over 64 x 64 tiles; job j owns the global tiles
+// [first, first + tiles); LOWER_C jobs (M = N) list only the lower tiles.
+struct GemmJob {
+  const double* A;
+  const double* B;
+  double* C;
+  int64_t M, N, K, lda, ldb, ldc;
+  double alpha, beta;
+  int64_t tm;     // row tiles
+  int64_t first;  // first global tile
+};
+struct PotrfJob {  // one 64-wide diagonal block of one fragment
+  double* A;       // L slot (in place), leading dimension n
+  double* Linv;    // Linv slot, leading dimension n
+  int64_t n, k0;
+  int kb;
+  int* status;
+};
+struct CopyJob {  // dst = src (dst == src: check only); bit 4 on a non-finite value
+  const double* src;
+  double* dst;
+  int* status;
+  int64_t first;  // first global element
+};
+struct DenomJob {  // bit 8 when max(eo) >= min(ev)
+  const double* eo;
+  const double* ev;
+  int64_t nocc, nvirt;
+  int* status;
+};
+struct LinvPackJob {
+  const double* Li;
+  double* Lp;
+  int64_t n, MB;
+  int64_t first;
+};
+struct XPackJob {
+  const double* X;
+  double* Xp;
+  int64_t naux, nvirt, nks, nrb;
+  int* status;
+  int64_t first;
+};
+struct TransformJob {
+  TransformArgs a;
+  int64_t first;  // first global work unit
+};
+
+enum GroupLaunch { GL_COPY, GL_DENOM, GL_ZERO_LINV, GL_POTRF, GL_PANEL, GL_SYRK, GL_TRI_T, GL_TRI_L, GL_LPACK,
+                   GL_XPACK, GL_TRANSFORM };
+
+// GL_PANEL: TRANSB, GL_SYRK: TRANSB + LOWER_C, GL_TRI_T: plain, GL_TRI_L: TRI_A.
+struct GroupStep {
+  GroupLaunch kind;
+  size_t off;     // job table offset in the blob (GL_ZERO_LINV: none)
+  int njobs;
+  int64_t total;  // elements / tiles / units / workgroups of the launch (GL_ZERO_LINV: bytes)
+};
+
+struct Item {      // one fragment with a metric (device arrays)
+  int64_t n, nvirt, nocc;
+  const double* V;  // [n, n]
+  const double* X;  // [n, nvirt, nocc]
+  double* B;        // [n, nvirt, nocc] out
+  int* status;      // 2 ints: bit 2 + [1] first failing pivot; bit 4 non-finite V or X
+};
+struct Copy {      // dst[0..n) = src[0..n) (dst == src: check only)
+  const double* src;
+  double* dst;
+  int64_t n;
+  int* status;
+};
+struct Denom {
+  const double* eo;
+  const double* ev;
+  int64_t nocc, nvirt;
+  int* status;
+};
+
+// Arena layout: fragments by decreasing n, every slot 256-byte aligned.
+struct Layout {
+  std::vector<size_t> order, lo, lpo, xpo;  // lo: L / Linv / T slot offsets (bytes)
+  size_t ln = 0, lpn = 0, xpn = 0;          // arena bytes
+  int64_t nmax = 0;
+};
+
+inline Layout layout(const std::vector<Item>& items) {
+  Layout g;
+  const size_t ni = items.size();
+  g.order.resize(ni);
+  std::iota(g.order.begin(), g.order.end(), 0);
+  std::stable_sort(g.order.begin(), g.order.end(), [&](size_t x, size_t y) { return items[x].n > items[y].n; });
+  g.lo.resize(ni);
+  g.lpo.resize(ni);
+  g.xpo.resize(ni);
+  for (size_t k : g.order) {
+    const Item& it = items[k];
+    g.lo[k] = g.ln;
+    g.ln += (size_t)rup((int64_t)(it.n * it.n * sizeof(double)), 256);
+    g.lpo[k] = g.lpn;
+    g.lpn += (size_t)rup((int64_t)linv_packed_bytes(it.n), 256);
+    g.xpo[k] = g.xpn;
+    g.xpn += (size_t)rup((int64_t)x_packed_bytes(it.n, it.nvirt, it.nocc), 256);
+    g.nmax = std::max(g.nmax, it.n);
+  }
+  return g;
+}
+
+struct Arenas {
+  unsigned char *L, *Linv, *T, *Lp, *Xp;  // at least ln, ln, ln, lpn, xpn bytes
+};
+
+// Fills steps and blob (the job tables, each 256-byte aligned in the blob).
+inline void plan(const std::vector<Item>& items, const std::vector<Copy>& copies, const std::vector<Denom>& denoms,
+                 const Layout& g, const Arenas& ar, std::vector<GroupStep>& steps, std::vector<unsigned char>& blob) {
+  steps.clear();
+  blob.clear();
+  auto Lk = [&](size_t k) { return reinterpret_cast<double*>(ar.L + g.lo[k]); };
+  auto Lik = [&](size_t k) { return reinterpret_cast<double*>(ar.Linv + g.lo[k]); };
+  auto Tk = [&](size_t k) { return reinterpret_cast<double*>(ar.T + g.lo[k]); };
+  auto put = [&](GroupLaunch kind, const void* p, size_t bytes, int njobs, int64_t total) {
+    if (njobs == 0 || total == 0) return;
+    const size_t off = (size_t)rup((int64_t)blob.size(), 256);
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, p, bytes);
+    steps.push_back(GroupStep{kind, off, njobs, total});
+  };
+  auto gemm = [](const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc, int64_t M,
+                 int64_t N, int64_t K, double alpha, double beta, bool lower, int64_t& first) {
+    GemmJob j{A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, cdiv(M, kPanel), first};
+    first += lower ? j.tm * (j.tm + 1) / 2 : j.tm * cdiv(N, kPanel);
+    return j;
+  };
+  {  // the caller's copies, then every V into its L slot
+    std::vector<CopyJob> cj;
+    int64_t e = 0;
+    for (const Copy& c : copies) {
+      if (c.n <= 0) continue;
+      cj.push_back(CopyJob{c.src, c.dst, c.status, e});
+      e += c.n;
+    }
+    for (size_t k : g.order) {
+      cj.push_back(CopyJob{items[k].V, Lk(k), items[k].status, e});
+      e += items[k].n * items[k].n;
+    }
+    put(GL_COPY, cj.data(), cj.size() * sizeof(CopyJob), (int)cj.size(), e);
+  }
+  {
+    std::vector<DenomJob> dj;
+    for (const Denom& d : denoms) dj.push_back(DenomJob{d.eo, d.ev, d.nocc, d.nvirt, d.status});
+    put(GL_DENOM, dj.data(), dj.size() * sizeof(DenomJob), (int)dj.size(), (int64_t)dj.size());
+  }
+  if (!items.empty()) steps.push_back(GroupStep{GL_ZERO_LINV, 0, 1, (int64_t)g.ln});
+  for (int64_t k0 = 0; k0 < g.nmax; k0 += kPanel) {
+    std::vector<PotrfJob> pj;
+    std::vector<GemmJob> panel, syrk;
+    int64_t tp = 0, ts = 0;
+    for (size_t k : g.order) {
+      const Item& it = items[k];
+      const int64_t n = it.n;
+      if (n <= k0) break;  // by decreasing n: no later fragment is this long
+      const int kb = (int)std::min<int64_t>(kPanel, n - k0);
+      pj.push_back(PotrfJob{Lk(k), Lik(k), n, k0, kb, it.status});
+      const int64_t r0 = k0 + kb, m = n - r0;
+      if (m <= 0) continue;
+      double* L21 = Lk(k) + r0 + n * k0;
+      panel.push_back(gemm(L21, n, Lik(k) + k0 + n * k0, n, L21, n, m, kb, kb, 1.0, 0.0, false, tp));
+      syrk.push_back(gemm(L21, n, L21, n, Lk(k) + r0 + n * r0, n, m, m, kb, -1.0, 1.0, true, ts));
+    }
+    put(GL_POTRF, pj.data(), pj.size() * sizeof(PotrfJob), (int)pj.size(), (int64_t)pj.size());
+    put(GL_PANEL, panel.data(), panel.size() * sizeof(GemmJob), (int)panel.size(), tp);
+    put(GL_SYRK, syrk.data(), syrk.size() * sizeof(GemmJob), (int)syrk.size(), ts);
+  }
+  // for [[P, 0], [Q, R]]: inverse [[P^-1, 0], [-R^-1 Q P^-1, R^-1]]; the
+  // groups of a level are independent; a ragged last group's T follows the
+  // full groups' T in the slot (nfull half^2 + sr half <= n^2)
+  for (int64_t w = 1; w * kPanel < g.nmax; w *= 2) {
+    std::vector<GemmJob> tj, lj;
+    int64_t tt = 0, tl = 0;
+    for (size_t k : g.order) {
+      const int64_t n = items[k].n;
+      const int64_t half = w * kPanel, span = 2 * half, nfull = n / span;
+      if (half >= n) break;
+      double *Lf = Lk(k), *Lif = Lik(k), *Tf = Tk(k);
+      for (int64_t q = 0; q < nfull; q++) {
+        const int64_t l0 = q * span, mid = l0 + half;
+        double* Tq = Tf + q * half * half;
+        tj.push_back(gemm(Lf + mid + n * l0, n, Lif + l0 + n * l0, n, Tq, half, half, half, half, 1.0, 0.0, false, tt));
+        lj.push_back(gemm(Lif + mid * (1 + n), n, Tq, half, Lif + mid + n * l0, n, half, half, half, -1.0, 0.0, false,
+                          tl));
+      }
+      const int64_t l0 = nfull * span, mid = l0 + half;
+      if (mid < n) {
+        const int64_t sr = n - mid;
+        double* Tq = Tf + nfull * half * half;
+        tj.push_back(gemm(Lf + mid + n * l0, n, Lif + l0 + n * l0, n, Tq, sr, sr, half, half, 1.0, 0.0, false, tt));
+        lj.push_back(gemm(Lif + mid * (1 + n), n, Tq, sr, Lif + mid + n * l0, n, sr, half, sr, -1.0, 0.0, false, tl));
+      }
+    }
+    put(GL_TRI_T, tj.data(), tj.size() * sizeof(GemmJob), (int)tj.size(), tt);
+    put(GL_TRI_L, lj.data(), lj.size() * sizeof(GemmJob), (int)lj.size(), tl);
+  }
+  {
+    std::vector<LinvPackJob> lp;
+    std::vector<XPackJob> xp;
+    std::vector<TransformJob> tr;
+    int64_t el = 0, ex = 0, eu = 0;
+    for (size_t k : g.order) {
+      const Item& it = items[k];
+      TransformArgs a = transform_args(it.n, it.nvirt, it.nocc);
+      a.Lp = ar.Lp + g.lpo[k];
+      a.Xp = ar.Xp + g.xpo[k];
+      a.B = it.B;
+      lp.push_back(LinvPackJob{Lik(k), reinterpret_cast<double*>(ar.Lp + g.lpo[k]), it.n, a.MB, el});
+      el += a.KS * a.MB * 64;
+      xp.push_back(XPackJob{it.X, reinterpret_cast<double*>(ar.Xp + g.xpo[k]), it.n, it.nvirt, a.KS, a.NRB, it.status,
+                            ex});
+      ex += it.nocc * a.KS * a.NRB * 64;
+      tr.push_back(TransformJob{a, eu});
+      eu += a.nwu;
+    }
+    put(GL_LPACK, lp.data(), lp.size() * sizeof(LinvPackJob), (int)lp.size(), el);
+    put(GL_XPACK, xp.data(), xp.size() * sizeof(XPackJob), (int)xp.size(), ex);
+    put(GL_TRANSFORM, tr.data(), tr.size() * sizeof(TransformJob), (int)tr.size(), eu);
+  }
+}
+
+}  // namespace mg
+}  // namespace rimp2

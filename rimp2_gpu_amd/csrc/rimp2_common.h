@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/rimp2_hip.h"
+#include "metric_group.h"
 
 namespace rimp2 {
 
@@ -109,8 +110,7 @@ struct ScopedTimer {
 
 // --- metric (metric.hip) ---
 // The stream and scratch buffers one metric pipeline (Cholesky, triangular
-// inverse, transform) runs on.  The context's own view uses its stream and
-// buffers; a batch runs several lanes of its own side by side.
+// inverse, transform) runs on.
 struct MetricView {
   hipStream_t stream;
   DevBuf &L, &Linv, &work, &linv_packed, &x_packed;
@@ -118,16 +118,26 @@ struct MetricView {
   Ctx* timer;   // context whose kernel timers record the launches (or null)
 };
 MetricView ctx_metric_view(Ctx* c);
-// Grows m's buffers to what metric_factor(n) and metric_transform(n, nvirt,
-// nocc) use (grow-only), so that work queued on m for several inputs of up to
-// these sizes never reallocates a buffer still in use.
-void metric_reserve(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc);
 // L L^T = V, L^{-1}, and L^{-1} in transform-operand order (dV_in: device, n x n)
 void metric_factor(const MetricView& m, int64_t n, const double* dV_in);
 // dB[n, nvirt, nocc] = L^{-1} X  (after metric_factor on the same view)
 void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* dB);
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
+// Grouped metric pipeline: the Cholesky, inverse and transform of many
+// fragments with one launch per step for the whole group (metric_group.h).
+// Arenas and job tables of a grouped pipeline (grow-only, reused call after
+// call; blob keeps the host copy of the job tables until the caller has
+// synchronized the stream).
+struct MetricGroup {
+  DevBuf L, Linv, T, Lp, Xp, jobs;
+  std::vector<unsigned char> blob;
+  void release();
+};
+// Queues on s: the copies, the denominator checks, then the metric pipeline
+// of every item.  The caller synchronizes s before the next call on G.
+void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>& items,
+                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms);
 // C = alpha * A * B^T + beta * C (column-major FP64 MFMA GEMM)
 void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
