@@ -54,12 +54,15 @@ struct BatchState {
   int tune_tiles[4] = {0, 0, 0, 0};
   int ntune = 0;
   MetricGroup group;  // grouped metric pipeline of the prepare
+  DevBuf pack_jobs;   // job tables of the grouped packing
+  std::vector<unsigned char> pack_blob;
   DevBuf stage;       // host inputs staged on the device (V, then X, of the fragments with a metric)
   DevBuf fstatus;     // two status ints per fragment (bits as the context's status word)
   ~BatchState() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags, &stage, &fstatus})
+    for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags, &stage, &fstatus,
+                      &pack_jobs})
       b->release();
     group.release();
   }
@@ -140,11 +143,6 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchFrag* frag
     __syncthreads();
   }
   if (threadIdx.x == 0) e_frag[fr.orig] = ss[0] + cc[0];
-}
-
-__global__ void pad_eps_kernel(const double* ev, int64_t nvirt, int64_t nvp, double* out) {
-  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < nvp) out[a] = (a < nvirt) ? ev[a] : 1.0;
 }
 
 template <class Pol, int TM>
@@ -233,12 +231,13 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
       v += (size_t)b->nvirt[f];
     }
   }
-  for (int64_t f = 0; f < nf; f++) {
-    pack_B(c, b->b64.as<double>() + b->b64_off[f], b->naux[f], b->nvirt[f], b->nocc[f], g[f], pk + poff[f],
-           sc ? sc + soff[f] : nullptr, s);
-    hipLaunchKernelGGL(pad_eps_kernel, dim3((unsigned)ceil_div(g[f].nvp, 256)), dim3(256), 0, s, ev_raw + voff[f],
-                       b->nvirt[f], g[f].nvp, evp + eoff[f]);
-    RIMP2_HIP_CHECK(hipGetLastError());
+  {
+    std::vector<PackItem> items(nf);
+    for (int64_t f = 0; f < nf; f++)
+      items[f] = PackItem{b->b64.as<double>() + b->b64_off[f], b->naux[f], b->nvirt[f], b->nocc[f], g[f],
+                          pk + poff[f], sc ? sc + soff[f] : nullptr, ev_raw + voff[f], evp + eoff[f]};
+    ScopedTimer tm(c, RIMP2_K_PACK);
+    pack_B_group(items, b->mode, s, b->pack_jobs, b->pack_blob);
   }
   // launch order: most expensive work units (longest auxiliary loop) first
   std::vector<int64_t> order(nf);

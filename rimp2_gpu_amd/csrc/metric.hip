@@ -40,7 +40,7 @@ constexpr int NB = (int)mg::kPanel;  // Cholesky / inverse panel width
 // ---------------------------------------------------------------------------
 // One BM x BN tile (row tile mt, column tile nt) of C; all 256 threads.
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
-__device__ inline void gemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+__device__ __attribute__((always_inline)) inline void gemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                                      const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t mt,
                                      int64_t nt) {
   constexpr int BK = 16;
@@ -271,7 +271,7 @@ __device__ inline void wave_lds_sync() {
 // entries), so the arithmetic is lane-local; X goes through LDS to be stored
 // with coalesced writes by all four waves.
 // ---------------------------------------------------------------------------
-__device__ inline void potrf_diag_block(double* A, int64_t lda, double* Linv, int64_t ldi, int64_t k0, int kb,
+__device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A, int64_t lda, double* Linv, int64_t ldi, int64_t k0, int kb,
                                         int* status) {
   __shared__ double Ls[NB][NB + 1];
   __shared__ double Xs[NB][NB + 1];
@@ -387,7 +387,7 @@ using mg::TransformArgs;
 // stage buffers (32 KB per 16-deep K stage: 16 KB of L^{-1}, 16 KB of X) fed
 // by buffer_load_dwordx4 ... lds, next-stage operands read under the current
 // stage's MFMAs, one DMA + six LDS reads per eight MFMAs.
-__device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
+__device__ __attribute__((always_inline)) inline void transform_unit(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
                                       unsigned char* __restrict__ buf1, unsigned char* __restrict__ buf2,
                                       unsigned char* __restrict__ buf3) {
   constexpr int FB = 512;                     // bytes per fragment
@@ -498,6 +498,19 @@ __device__ inline void transform_unit(const TransformArgs& p, int64_t wu, unsign
   __syncthreads();  // every wave is done with the stage buffers before a next unit refills them
 }
 
+// Work unit of the transform kernels: row tiles t and MT-1-t of one column
+// tile (col = wu / MTH, t = wu % MTH, MTH = ceil(MT / 2)).  Row tile t runs
+// t + 1 K tiles (L^{-1} is lower triangular), so every unit spans MT + 1 K
+// tiles and the workgroups of a launch finish together instead of waiting on
+// the longest row tiles at the end.
+__device__ __attribute__((always_inline)) inline void transform_pair(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
+                                      unsigned char* __restrict__ buf1, unsigned char* __restrict__ buf2,
+                                      unsigned char* __restrict__ buf3) {
+  const int64_t mth = (p.MT + 1) >> 1, col = wu / mth, t = wu - col * mth;
+  transform_unit(p, col * p.MT + t, buf0, buf1, buf2, buf3);
+  if (p.MT - 1 - t != t) transform_unit(p, col * p.MT + (p.MT - 1 - t), buf0, buf1, buf2, buf3);
+}
+
 __global__ __launch_bounds__(512) void transform_kernel(TransformArgs p) {
   constexpr int SB = 32768;
   __shared__ __align__(16) unsigned char buf0[SB];
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(512) void transform_kernel(TransformArgs p) {
   for (int64_t k = slot; k < p.chunk; k += nslot) {
     const int64_t wu = xcd * p.chunk + k;
     if (wu >= p.nwu) break;
-    transform_unit(p, wu, buf0, buf1, buf2, buf3);
+    transform_pair(p, wu, buf0, buf1, buf2, buf3);
   }
 }
 
@@ -530,10 +543,17 @@ __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
 // Copy src -> dst (dst == src: check only); bit 4 of status on a non-finite value.
 using mg::CopyJob;
 
+// Grid-stride over [0, total) in chunks of 256 consecutive elements: the job
+// of a chunk's first element is found once for the workgroup (uniform, scalar
+// loads); each thread then steps past the jobs that end before its element.
 __global__ __launch_bounds__(256) void copy_check_group_kernel(const CopyJob* __restrict__ jobs, int njobs,
                                                                int64_t total) {
-  RIMP2_GRID_STRIDE(t, total) {
-    const CopyJob& J = jobs[job_find(jobs, njobs, t)];
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+    int j = job_find(jobs, njobs, base);
+    const int64_t t = base + threadIdx.x;
+    if (t >= total) continue;
+    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
+    const CopyJob& J = jobs[j];
     const int64_t e = t - J.first;
     const double v = J.src[e];
     if (J.dst != J.src) J.dst[e] = v;
@@ -562,8 +582,12 @@ using mg::LinvPackJob;
 
 __global__ __launch_bounds__(256) void pack_linv_group_kernel(const LinvPackJob* __restrict__ jobs, int njobs,
                                                               int64_t total) {
-  RIMP2_GRID_STRIDE(t, total) {
-    const LinvPackJob& J = jobs[job_find(jobs, njobs, t)];
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+    int j = job_find(jobs, njobs, base);
+    const int64_t t = base + threadIdx.x;
+    if (t >= total) continue;
+    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
+    const LinvPackJob& J = jobs[j];
     const int64_t e = t - J.first;
     const int lane = (int)(e & 63);
     const int64_t f = e >> 6, mb = f % J.MB, ks = f / J.MB;
@@ -572,23 +596,38 @@ __global__ __launch_bounds__(256) void pack_linv_group_kernel(const LinvPackJob*
   }
 }
 
-// X of one fragment into FP64 B-fragment order (the layout of pack.hip's
-// fp64 packing), bit 4 of status on a non-finite value.
+// One 64 (q) x 64 (column) tile of X per workgroup: coalesced column reads
+// (512 bytes per wave) into LDS, then the tile's 16 k-steps x 4 row blocks
+// written as 512-byte fragments of Xp[g][ks][rb][lane].
 using mg::XPackJob;
 
-__global__ __launch_bounds__(256) void pack_x_group_kernel(const XPackJob* __restrict__ jobs, int njobs,
-                                                           int64_t total) {
-  RIMP2_GRID_STRIDE(t, total) {
-    const XPackJob& J = jobs[job_find(jobs, njobs, t)];
-    const int64_t e = t - J.first;
-    const int lane = (int)(e & 63);
-    const int64_t f = e >> 6;
-    const int64_t rb = f % J.nrb, ks = (f / J.nrb) % J.nks, i = f / (J.nrb * J.nks);
-    const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-    const double v = (q < J.naux && a < J.nvirt) ? J.X[q + J.naux * (a + J.nvirt * i)] : 0.0;
-    J.Xp[e] = v;
-    if (!isfinite(v)) atomicOr(J.status, 4);
+__global__ __launch_bounds__(256) void pack_x_group_kernel(const XPackJob* __restrict__ jobs, int njobs) {
+  __shared__ double tile[64][65];  // [column][q]
+  const int64_t t = blockIdx.x;
+  const XPackJob& J = jobs[job_find(jobs, njobs, t)];
+  const int64_t u = t - J.first, kt = J.nks / 16, rt = J.nrb / 4;
+  const int64_t rbt = u % rt, kst = (u / rt) % kt, g = u / (rt * kt);
+  const int64_t q0 = kst * 64, c0 = rbt * 64;
+  const double* Xg = J.X + J.naux * J.cols * g;
+  const int tid = threadIdx.x;
+  bool bad = false;
+#pragma unroll 4
+  for (int r = 0; r < 16; r++) {
+    const int cl = (tid >> 6) + 4 * r;
+    const int64_t q = q0 + (tid & 63), c = c0 + cl;
+    const double v = (q < J.naux && c < J.cols) ? Xg[q + J.naux * c] : 0.0;
+    bad |= !isfinite(v);
+    tile[cl][tid & 63] = v;
   }
+  __syncthreads();
+  double* out = J.Xp + ((g * J.nks + kst * 16) * J.nrb + rbt * 4) * 64;
+#pragma unroll 4
+  for (int r = 0; r < 16; r++) {
+    const int e = 256 * r + tid, fr = e >> 6, lane = e & 63;
+    const int ksl = fr >> 2, rbl = fr & 3;
+    out[((int64_t)ksl * J.nrb + rbl) * 64 + lane] = tile[16 * rbl + (lane & 15)][4 * ksl + (lane >> 4)];
+  }
+  if (bad) atomicOr(J.status, 4);
 }
 
 using mg::TransformJob;
@@ -606,7 +645,7 @@ __global__ __launch_bounds__(512) void transform_group_kernel(const TransformJob
     if (wu >= total) break;
     const int j = job_find(jobs, njobs, wu);
     const TransformArgs a = jobs[j].a;
-    transform_unit(a, wu - jobs[j].first, buf0, buf1, buf2, buf3);
+    transform_pair(a, wu - jobs[j].first, buf0, buf1, buf2, buf3);
   }
 }
 
@@ -718,7 +757,7 @@ void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t noc
   if ((size_t)p.KS * p.MB * 512 > 0xFFFFFFFFull || (size_t)p.KS * p.NRB * 512 > 0xFFFFFFFFull)
     fail(RIMP2_ERR_ARG, "metric transform operands exceed 4 GiB per slab (naux=%lld nvirt=%lld)", (long long)n,
          (long long)nvirt);
-  p.nwu = p.MT * p.NT * nocc;
+  p.nwu = ((p.MT + 1) / 2) * p.NT * nocc;  // paired row tiles
   p.chunk = ceil_div(p.nwu, 8);
   PackGeom gx;
   gx.mode = RIMP2_MODE_FP64;
@@ -815,8 +854,9 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
         hipLaunchKernelGGL(pack_linv_group_kernel, dim3(eb), dim3(256), 0, s, (const LinvPackJob*)jp, st.njobs,
                            st.total);
         break;
-      case mg::GL_XPACK:
-        hipLaunchKernelGGL(pack_x_group_kernel, dim3(eb), dim3(256), 0, s, (const XPackJob*)jp, st.njobs, st.total);
+      case mg::GL_XPACK:  // one workgroup per 64 x 64 tile
+        if (st.total > kMaxGridBlocks) fail(RIMP2_ERR_ARG, "grouped X packing too large (%lld tiles)", (long long)st.total);
+        hipLaunchKernelGGL(pack_x_group_kernel, dim3((unsigned)st.total), dim3(256), 0, s, (const XPackJob*)jp, st.njobs);
         break;
       case mg::GL_TRANSFORM: {
         const int64_t chunk = ceil_div(st.total, (int64_t)8);

@@ -48,8 +48,26 @@ struct TransformArgs {
   int64_t MB, KS;           // L^{-1} row blocks (16) and k-steps (4)
   int64_t NRB;              // X row blocks (16 virtuals) per occupied
   int64_t MT, NT;           // row tiles, column tiles per occupied
-  int64_t nwu, chunk;
+  int64_t nwu, chunk;       // work units (row tiles t and MT-1-t of one column tile), per XCD
 };
+
+// Column grouping of the transform.  X and B are [n, nvirt, nocc] column-major,
+// i.e. [n, nvirt * nocc]: the transform may take its 128-wide column tiles
+// per occupied orbital (groups = nocc, cols = nvirt; each occupied orbital's
+// packed X is one slab addressed with 32-bit offsets) or over the flat range
+// (groups = 1, cols = nvirt * nocc), which pads each fragment by less than one
+// tile instead of up to 127 columns per occupied orbital.  Flat wherever the
+// one slab stays below 4 GiB.
+inline void transform_columns(int64_t n, int64_t nvirt, int64_t nocc, int64_t& cols, int64_t& groups) {
+  const size_t flat_slab = (size_t)(rup(n, kTrKQ) / 4) * (size_t)(rup(nvirt * nocc, kTrTN) / 16) * 512;
+  if (flat_slab <= 0xFFFFFFFFull) {
+    cols = nvirt * nocc;
+    groups = 1;
+  } else {
+    cols = nvirt;
+    groups = nocc;
+  }
+}
 
 inline TransformArgs transform_args(int64_t n, int64_t nvirt, int64_t nocc) {
   TransformArgs a{};
@@ -61,7 +79,7 @@ inline TransformArgs transform_args(int64_t n, int64_t nvirt, int64_t nocc) {
   a.NRB = rup(nvirt, kTrTN) / 16;
   a.MT = rup(n, kTrTM) / kTrTM;
   a.NT = a.NRB / 8;
-  a.nwu = a.MT * a.NT * nocc;
+  a.nwu = ((a.MT + 1) / 2) * a.NT * nocc;
   a.chunk = cdiv(a.nwu, 8);
   return a;
 }
@@ -102,12 +120,15 @@ struct LinvPackJob {
   int64_t n, MB;
   int64_t first;
 };
+// X of one fragment into B-fragment order Xp[g][ks][rb][lane] (lane l <->
+// q = 4 ks + l / 16, column 16 rb + l % 16 of group g), in tiles of 64 q x
+// 64 columns (16 k-steps x 4 row blocks); bit 4 of status on a non-finite value.
 struct XPackJob {
   const double* X;
   double* Xp;
-  int64_t naux, nvirt, nks, nrb;
+  int64_t naux, cols, nks, nrb;  // cols per group; nks % 16 == 0, nrb % 4 == 0
   int* status;
-  int64_t first;
+  int64_t first;  // first global tile; tiles = groups * (nks / 16) * (nrb / 4)
 };
 struct TransformJob {
   TransformArgs a;
@@ -167,8 +188,10 @@ inline Layout layout(const std::vector<Item>& items) {
     g.ln += (size_t)rup((int64_t)(it.n * it.n * sizeof(double)), 256);
     g.lpo[k] = g.lpn;
     g.lpn += (size_t)rup((int64_t)linv_packed_bytes(it.n), 256);
+    int64_t cols, groups;
+    transform_columns(it.n, it.nvirt, it.nocc, cols, groups);
     g.xpo[k] = g.xpn;
-    g.xpn += (size_t)rup((int64_t)x_packed_bytes(it.n, it.nvirt, it.nocc), 256);
+    g.xpn += (size_t)rup((int64_t)x_packed_bytes(it.n, cols, groups), 256);
     g.nmax = std::max(g.nmax, it.n);
   }
   return g;
@@ -275,15 +298,16 @@ inline void plan(const std::vector<Item>& items, const std::vector<Copy>& copies
     int64_t el = 0, ex = 0, eu = 0;
     for (size_t k : g.order) {
       const Item& it = items[k];
-      TransformArgs a = transform_args(it.n, it.nvirt, it.nocc);
+      int64_t cols, groups;
+      transform_columns(it.n, it.nvirt, it.nocc, cols, groups);
+      TransformArgs a = transform_args(it.n, cols, groups);
       a.Lp = ar.Lp + g.lpo[k];
       a.Xp = ar.Xp + g.xpo[k];
       a.B = it.B;
       lp.push_back(LinvPackJob{Lik(k), reinterpret_cast<double*>(ar.Lp + g.lpo[k]), it.n, a.MB, el});
       el += a.KS * a.MB * 64;
-      xp.push_back(XPackJob{it.X, reinterpret_cast<double*>(ar.Xp + g.xpo[k]), it.n, it.nvirt, a.KS, a.NRB, it.status,
-                            ex});
-      ex += it.nocc * a.KS * a.NRB * 64;
+      xp.push_back(XPackJob{it.X, reinterpret_cast<double*>(ar.Xp + g.xpo[k]), it.n, cols, a.KS, a.NRB, it.status, ex});
+      ex += groups * (a.KS / 16) * (a.NRB / 4);
       tr.push_back(TransformJob{a, eu});
       eu += a.nwu;
     }

@@ -93,25 +93,28 @@ __device__ inline double bget(const PackArgs& p, int64_t q, int64_t a, int64_t i
   return (q < p.naux && a < p.nvirt) ? p.B[q + p.naux * (a + p.nvirt * i)] : 0.0;
 }
 
-// one thread per (i, ks, rb, lane)
-__global__ __launch_bounds__(256) void pack_fp64_kernel(PackArgs p) {
-  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
-    const int lane = (int)(t & 63);
-    const int64_t f = t >> 6;  // fragment index = (i*nks + ks)*nrb + rb
-    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-    const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-    static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
-  }
+// One packed element t (fragment f = t >> 6 = (i * nks + ks) * nrb + rb, lane
+// t & 63) per mode; the split-BF16 and exact-slice modes write every plane of
+// the fragment.
+template <int MODE>
+__device__ inline void pack_one(const PackArgs& p, int64_t t, int64_t nvp);
+
+template <>
+__device__ inline void pack_one<RIMP2_MODE_FP64>(const PackArgs& p, int64_t t, int64_t) {
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
+  static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
 }
 
-__global__ __launch_bounds__(256) void pack_fp32_kernel(PackArgs p) {
-  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
-    const int lane = (int)(t & 63);
-    const int64_t f = t >> 6;
-    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-    const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
-    static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
-  }
+template <>
+__device__ inline void pack_one<RIMP2_MODE_FP32>(const PackArgs& p, int64_t t, int64_t) {
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
+  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
 }
 
 // bf16 round-to-nearest-even of a float (no NaN inputs reach here).
@@ -123,58 +126,138 @@ __device__ inline uint16_t f2bf(float x) {
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // planes: 0 = hi = bf16(b), 1 = lo = bf16(b - hi)
-__global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p) {
-  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
-    const int lane = (int)(t & 63);
-    const int64_t f = t >> 6;
-    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-    const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
-    uint16_t hi[8], lo[8];
-  #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      float b = (float)bget(p, q0 + e, a, i);
-      hi[e] = f2bf(b);
-      lo[e] = f2bf(b - bf2f(hi[e]));
-    }
-    uint4 vh, vl;
-    vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
-    vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
-    vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
-    vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
-    uint4* dst = static_cast<uint4*>(p.dst);
-    dst[(f * 2 + 0) * 64 + lane] = vh;
-    dst[(f * 2 + 1) * 64 + lane] = vl;
+template <>
+__device__ inline void pack_one<RIMP2_MODE_BF16X3>(const PackArgs& p, int64_t t, int64_t) {
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
+  uint16_t hi[8], lo[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    float b = (float)bget(p, q0 + e, a, i);
+    hi[e] = f2bf(b);
+    lo[e] = f2bf(b - bf2f(hi[e]));
   }
+  uint4 vh, vl;
+  vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
+  vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
+  vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
+  vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
+  uint4* dst = static_cast<uint4*>(p.dst);
+  dst[(f * 2 + 0) * 64 + lane] = vh;
+  dst[(f * 2 + 1) * 64 + lane] = vl;
 }
 
 template <int S>
-__global__ __launch_bounds__(256) void pack_i8_kernel(PackArgs p, int64_t nvp) {
-  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) {
-    const int lane = (int)(t & 63);
-    const int64_t f = t >> 6;
-    const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-    const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
-    const double sg = p.sigma[i * nvp + a];
-    const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
-    int8_t sl[S][16];
-  #pragma unroll
-    for (int e = 0; e < 16; e++) {
-      double r = bget(p, q0 + e, a, i) * inv;
-  #pragma unroll
-      for (int k = 0; k < S; k++) {
-        double s = rint(r);
-        s = fmin(127.0, fmax(-127.0, s));
-        sl[k][e] = (int8_t)s;
-        r = (r - s) * 254.0;
-      }
-    }
-    uint4* dst = static_cast<uint4*>(p.dst);
-  #pragma unroll
+__device__ inline void pack_i8_one(const PackArgs& p, int64_t t, int64_t nvp) {
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
+  const double sg = p.sigma[i * nvp + a];
+  const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
+  int8_t sl[S][16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    double r = bget(p, q0 + e, a, i) * inv;
+#pragma unroll
     for (int k = 0; k < S; k++) {
-      uint4 v;
-      memcpy(&v, sl[k], 16);
-      dst[(f * S + k) * 64 + lane] = v;
+      double s = rint(r);
+      s = fmin(127.0, fmax(-127.0, s));
+      sl[k][e] = (int8_t)s;
+      r = (r - s) * 254.0;
     }
+  }
+  uint4* dst = static_cast<uint4*>(p.dst);
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    uint4 v;
+    memcpy(&v, sl[k], 16);
+    dst[(f * S + k) * 64 + lane] = v;
+  }
+}
+template <>
+__device__ inline void pack_one<RIMP2_MODE_I8S4>(const PackArgs& p, int64_t t, int64_t nvp) {
+  pack_i8_one<4>(p, t, nvp);
+}
+template <>
+__device__ inline void pack_one<RIMP2_MODE_I8S5>(const PackArgs& p, int64_t t, int64_t nvp) {
+  pack_i8_one<5>(p, t, nvp);
+}
+
+// one thread per (i, ks, rb, lane)
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_kernel(PackArgs p, int64_t nvp) {
+  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) pack_one<MODE>(p, t, nvp);
+}
+
+// Grouped packing of many fragments (one launch per step for a batch).  Job j
+// owns the global items [first, next job's first); a chunk of 256 consecutive
+// items finds its first job once (uniform), each thread steps on from there.
+struct PackJob {
+  PackArgs p;
+  double* sigma;  // row scales written by rowmax_group_kernel (p.sigma reads them)
+  int64_t nvp;
+  int64_t first;
+};
+
+// Padded virtual energies of many fragments: out[a] = ev[a] (a < nvirt), 1.0
+// on padding rows (their B rows are zero); jobs own elements [first, next).
+struct PadJob {
+  const double* ev;
+  double* out;
+  int64_t nvirt;
+  int64_t first;
+};
+
+template <class Job>
+__device__ inline int pack_job_find(const Job* __restrict__ jobs, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_group_kernel(const PackJob* __restrict__ jobs, int njobs, int64_t total) {
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+    int j = pack_job_find(jobs, njobs, base);
+    const int64_t t = base + threadIdx.x;
+    if (t >= total) continue;
+    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
+    pack_one<MODE>(jobs[j].p, t - jobs[j].first, jobs[j].nvp);
+  }
+}
+
+// Row scales of the exact-slice modes for many fragments: one wave per row;
+// jobs own rows [first, next first) (row = i * nvp + a).
+__global__ __launch_bounds__(256) void rowmax_group_kernel(const PackJob* __restrict__ jobs, int njobs, int64_t total) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < total; row += (int64_t)gridDim.x * 4) {
+    const PackJob& J = jobs[pack_job_find(jobs, njobs, row)];
+    const int64_t r = row - J.first, i = r / J.nvp, a = r % J.nvp;
+    double m = 0.0;
+    if (a < J.p.nvirt) {
+      const double* col = J.p.B + J.p.naux * (a + J.p.nvirt * i);
+      for (int64_t q = lane; q < J.p.naux; q += 64) m = fmax(m, fabs(col[q]));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (lane == 0) J.sigma[r] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void pad_group_kernel(const PadJob* __restrict__ jobs, int njobs, int64_t total) {
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+    int j = pack_job_find(jobs, njobs, base);
+    const int64_t t = base + threadIdx.x;
+    if (t >= total) continue;
+    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
+    const int64_t a = t - jobs[j].first;
+    jobs[j].out[a] = (a < jobs[j].nvirt) ? jobs[j].ev[a] : 1.0;
   }
 }
 
@@ -187,22 +270,87 @@ void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc,
   const int64_t threads = nocc * g.nks * g.nrb * 64;
   const unsigned blocks = grid_blocks(threads, 256);
   switch (g.mode) {
-    case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_fp64_kernel, dim3(blocks), dim3(256), 0, s, p); break;
-    case RIMP2_MODE_FP32: hipLaunchKernelGGL(pack_fp32_kernel, dim3(blocks), dim3(256), 0, s, p); break;
-    case RIMP2_MODE_BF16X3: hipLaunchKernelGGL(pack_bf16x3_kernel, dim3(blocks), dim3(256), 0, s, p); break;
+    case RIMP2_MODE_FP64:
+      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP64>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      break;
+    case RIMP2_MODE_FP32:
+      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP32>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      break;
+    case RIMP2_MODE_BF16X3:
+      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_BF16X3>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      break;
     case RIMP2_MODE_I8S4:
     case RIMP2_MODE_I8S5: {
       hipLaunchKernelGGL(rowmax_kernel, dim3((unsigned)ceil_div(g.nvp * nocc, 4)), dim3(256), 0, s, dB, naux, nvirt,
                          nocc, g.nvp, dscales);
       RIMP2_HIP_CHECK(hipGetLastError());
       if (g.mode == RIMP2_MODE_I8S4)
-        hipLaunchKernelGGL(pack_i8_kernel<4>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+        hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_I8S4>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
       else
-        hipLaunchKernelGGL(pack_i8_kernel<5>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+        hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_I8S5>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
       break;
     }
     default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", g.mode);
   }
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, DevBuf& jobs,
+                  std::vector<unsigned char>& blob) {
+  if (items.empty()) return;
+  std::vector<PackJob> pj;
+  std::vector<PadJob> dj;
+  int64_t ne = 0, nr = 0, np = 0;
+  for (const PackItem& it : items) {
+    if (it.g.mode != mode) fail(RIMP2_ERR_ARG, "grouped packing: geometry of another mode");
+    PackJob j;
+    j.p = PackArgs{it.B, it.naux, it.nvirt, it.nocc, it.g.nks, it.g.nrb, it.sigma, it.dst};
+    j.sigma = it.sigma;
+    j.nvp = it.g.nvp;
+    j.first = ne;
+    pj.push_back(j);
+    ne += it.nocc * it.g.nks * it.g.nrb * 64;
+    dj.push_back(PadJob{it.ev, it.ev_padded, it.nvirt, np});
+    np += it.g.nvp;
+  }
+  // rowmax reuses the pack jobs with first counted in rows (i * nvp + a)
+  std::vector<PackJob> rj = pj;
+  for (size_t k = 0; k < rj.size(); k++) {
+    rj[k].first = nr;
+    nr += items[k].nocc * items[k].g.nvp;
+  }
+  blob.clear();
+  auto put = [&](const void* p, size_t bytes) {
+    const size_t off = (size_t)round_up((int64_t)blob.size(), 256);
+    blob.resize(off + bytes);
+    memcpy(blob.data() + off, p, bytes);
+    return off;
+  };
+  const size_t opj = put(pj.data(), pj.size() * sizeof(PackJob));
+  const size_t orj = put(rj.data(), rj.size() * sizeof(PackJob));
+  const size_t odj = put(dj.data(), dj.size() * sizeof(PadJob));
+  unsigned char* d = static_cast<unsigned char*>(jobs.ensure(blob.size()));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  const int nj = (int)items.size();
+  if (mode_slices(mode)) {
+    hipLaunchKernelGGL(rowmax_group_kernel, dim3(grid_blocks(nr * 64, 256)), dim3(256), 0, s,
+                       (const PackJob*)(d + orj), nj, nr);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  const dim3 gb(grid_blocks(ne, 256));
+  const PackJob* jp = (const PackJob*)(d + opj);
+  switch (mode) {
+    case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP64>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_FP32: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP32>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_BF16X3:
+      hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_BF16X3>, gb, dim3(256), 0, s, jp, nj, ne);
+      break;
+    case RIMP2_MODE_I8S4: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_I8S4>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_I8S5: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_I8S5>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
+  }
+  RIMP2_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(pad_group_kernel, dim3(grid_blocks(np, 256)), dim3(256), 0, s, (const PadJob*)(d + odj), nj, np);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

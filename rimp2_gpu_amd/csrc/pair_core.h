@@ -204,24 +204,28 @@ constexpr int64_t UNIT_SUPER = 8;
 // distinct operand slabs (46 on average in the flat order at nb = 32), so
 // more of the operand stream is served from that XCD's L2.
 // For nb <= UNIT_SUPER the order is the flat one.
-__device__ inline int unit_decode(int64_t u, int64_t nb, bool packed, int64_t& A, int64_t& B) {
-  const int64_t G = UNIT_SUPER;
-  const int64_t ng = (nb + G - 1) / G;
-  // units before full super-column g: sum_{c<g} (G*G*c + dG) = 32 g(g-1) + dG g
-  const int64_t dG = packed ? G * (G - 1) / 2 + G / 2 : G * (G + 1) / 2;
-  auto before = [&](int64_t g) { return (G * G / 2) * g * (g - 1) + dG * g; };
-  const double b = (double)(dG - G * G / 2);
-  int64_t gb = (int64_t)((-b + sqrt(b * b + 2.0 * G * G * (double)u)) / (double)(G * G));
+__device__ inline int unit_decode(int64_t u64, int64_t nb64, bool packed, int64_t& A, int64_t& B) {
+  if (nb64 <= UNIT_SUPER) return unit_decode_flat(u64, nb64, packed, A, B);
+  // 32-bit from here: units per pair < 2^31 (nb < 2^15)
+  constexpr uint32_t G = (uint32_t)UNIT_SUPER;
+  const uint32_t u = (uint32_t)u64, nb = (uint32_t)nb64;
+  const uint32_t ng = (nb + G - 1) / G;
+  // units before full super-column g: sum_{c<g} (G*G*c + dG) = (G*G/2) g(g-1) + dG g
+  const uint32_t dG = packed ? G * (G - 1) / 2 + G / 2 : G * (G + 1) / 2;
+  auto before = [&](uint32_t g) { return (G * G / 2) * g * (g - 1) + dG * g; };
+  const float b = (float)dG - (float)(G * G / 2);
+  int32_t gi = (int32_t)((-b + sqrtf(b * b + 2.0f * (float)(G * G) * (float)u)) / (float)(G * G));
+  uint32_t gb = gi < 0 ? 0u : (uint32_t)gi;
   if (gb > ng - 1) gb = ng - 1;
-  if (gb < 0) gb = 0;
   while (gb > 0 && before(gb) > u) gb--;
   while (gb + 1 < ng && before(gb + 1) <= u) gb++;
-  int64_t v = u - before(gb);
-  const int64_t gB = (nb - G * gb < G) ? nb - G * gb : G;
+  uint32_t v = u - before(gb);
+  const uint32_t gB = (nb - G * gb < G) ? nb - G * gb : G;
   if (v < gb * G * gB) {
-    const int64_t ga = v / (G * gB), w = v % (G * gB);
-    A = G * ga + w / gB;
-    B = G * gb + w % gB;
+    const uint32_t ga = v / (G * gB), w = v - ga * (G * gB);
+    const uint32_t wa = w / gB;
+    A = (int64_t)(G * ga + wa);
+    B = (int64_t)(G * gb + (w - wa * gB));
     return UNIT_TILE_PAIR;
   }
   v -= gb * G * gB;

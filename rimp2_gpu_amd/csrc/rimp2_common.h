@@ -147,6 +147,21 @@ PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile);
 void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
             double* dscales, hipStream_t s);
 int mode_slices(int mode);
+// One fragment of a grouped packing (device arrays).
+struct PackItem {
+  const double* B;      // FP64 B [naux, nvirt, nocc]
+  int64_t naux, nvirt, nocc;
+  PackGeom g;           // geometry of the mode and pair tile
+  void* dst;            // packed operands
+  double* sigma;        // exact-slice row scales [nocc * nvp] (else unused)
+  const double* ev;     // virtual energies [nvirt]
+  double* ev_padded;    // out: [nvp], 1.0 on padding rows
+};
+// Packs every item with three launches in all (row scales, packing, padded
+// virtual energies); jobs / blob hold the job tables (the caller synchronizes
+// s before the next call on them).
+void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, DevBuf& jobs,
+                  std::vector<unsigned char>& blob);
 
 // --- pair kernel (pair.hip) ---
 int choose_tile(int mode, int64_t nvirt, int64_t nocc);

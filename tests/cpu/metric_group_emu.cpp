@@ -333,18 +333,35 @@ int main() {
           break;
         }
         case GL_XPACK: {
+          // tiles of 64 q x 64 columns; each (q, column) of group g goes to
+          // fragment (ks = q / 4, rb = column / 16), lane (q % 4) 16 + column % 16;
+          // every element of the packed X written exactly once
           const XPackJob* J = reinterpret_cast<const XPackJob*>(jp);
           check_firsts(J, st.njobs, st.total);
-          for (int j = 0; j < st.njobs; j++)
-            for (int64_t e = 0; e < job_end(J, st.njobs, j, st.total) - J[j].first; e++) {
-              const XPackJob& x = J[j];
-              const int lane = (int)(e & 63);
-              const int64_t f = e >> 6, rb = f % x.nrb, ks = (f / x.nrb) % x.nks, i = f / (x.nrb * x.nks);
-              const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-              const double v = (q < x.naux && a < x.nvirt) ? rd(x.X + q + x.naux * (a + x.nvirt * i), "X") : 0.0;
-              wr(x.Xp + e, v, "Xp");
-              if (!std::isfinite(v)) wr(x.status, rd(x.status, "status") | 4, "status");
-            }
+          for (int j = 0; j < st.njobs; j++) {
+            const XPackJob& x = J[j];
+            const Item* it = nullptr;
+            for (const Item& c : items)
+              if (c.X == x.X) it = &c;
+            if (!it) die("X packing job of no fragment", j);
+            if (x.nks % 16 || x.nrb % 4) die("X packing tile shape", j);
+            const int64_t per = (x.nks / 16) * (x.nrb / 4), tiles = job_end(J, st.njobs, j, st.total) - x.first;
+            if (tiles % per) die("X packing tile count", j);
+            const int64_t groups = tiles / per;
+            if (x.cols * groups != it->nvirt * it->nocc || (groups != 1 && x.cols != it->nvirt))
+              die("X packing column grouping", j, groups);
+            std::vector<unsigned char> seen((size_t)(groups * x.nks * x.nrb * 64), 0);
+            for (int64_t g0 = 0; g0 < groups; g0++)
+              for (int64_t q = 0; q < x.nks * 4; q++)
+                for (int64_t c = 0; c < x.nrb * 16; c++) {
+                  const double v =
+                      (q < x.naux && c < x.cols) ? rd(x.X + x.naux * x.cols * g0 + q + x.naux * c, "X") : 0.0;
+                  const int64_t idx = ((g0 * x.nks + q / 4) * x.nrb + c / 16) * 64 + (q % 4) * 16 + c % 16;
+                  if (seen[idx]++) die("X packing writes an element twice", j);
+                  wr(x.Xp + idx, v, "Xp");
+                  if (!std::isfinite(v)) wr(x.status, rd(x.status, "status") | 4, "status");
+                }
+          }
           break;
         }
         case GL_TRANSFORM: {
@@ -359,7 +376,20 @@ int main() {
             const double* Xpd = reinterpret_cast<const double*>(a.Xp);
             chk(a.Lp, (size_t)a.KS * a.MB * 512, "transform Lp slab");
             chk(a.Xp, (size_t)a.nocc * a.KS * a.NRB * 512, "transform Xp slabs");
-            for (int64_t wu = 0; wu < a.nwu; wu++) {
+            // work unit: row tiles t and MT-1-t of one column tile; every
+            // (row tile, column tile) exactly once
+            std::vector<int64_t> units;
+            const int64_t mth = (a.MT + 1) / 2;
+            for (int64_t w2 = 0; w2 < a.nwu; w2++) {
+              const int64_t col = w2 / mth, t = w2 % mth;
+              units.push_back(col * a.MT + t);
+              if (a.MT - 1 - t != t) units.push_back(col * a.MT + a.MT - 1 - t);
+            }
+            std::vector<int> seen_u((size_t)(a.MT * a.NT * a.nocc), 0);
+            for (int64_t wu : units)
+              if (wu >= (int64_t)seen_u.size() || seen_u[wu]++) die("transform unit out of range or repeated", j, wu);
+            if ((int64_t)units.size() != a.MT * a.NT * a.nocc) die("transform units missing", j);
+            for (int64_t wu : units) {
               const int64_t mt = wu % a.MT, col = wu / a.MT, i = col / a.NT, nt = col % a.NT;
               const int64_t kend = std::min((mt + 1) * kTrTM, a.KS * 4);
               for (int64_t q = mt * kTrTM; q < std::min(a.naux, (mt + 1) * kTrTM); q++)
