@@ -265,11 +265,9 @@ __device__ inline void wave_lds_sync() {
 // wave's LDS accesses complete in order, so a wavefront-scope fence (no
 // instruction) is all the ordering needed: the whole dependency chain lives
 // in one wave.  The scaling by 1/sqrt(d_c) is applied once at the end.
-// Inverse, same wave: lane j forward-substitutes column j of X = L11^{-1},
-//     x_j[c] = (delta_cj - sum_{k<c} L[c][k] x_j[k]) / L[c][c],
-// with the row L[c][0..c) read by broadcast loads (every lane needs the same
-// entries), so the arithmetic is lane-local; X goes through LDS to be stored
-// with coalesced writes by all four waves.
+// Inverse: all four waves, in 16 x 16 blocks (diagonal blocks by forward
+// substitution, the blocks below by distance on f64 MFMA; see below); X goes
+// through LDS to be stored with coalesced writes by all four waves.
 // ---------------------------------------------------------------------------
 __device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A, int64_t lda, double* Linv, int64_t ldi, int64_t k0, int kb,
                                         int* status) {
@@ -277,6 +275,7 @@ __device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A
   __shared__ double Xs[NB][NB + 1];
   __shared__ __align__(16) double colb[2][NB];
   __shared__ __align__(16) double invd[NB];
+  __shared__ double Ts[3][16][17];  // per-wave 16 x 16 products of the inverse
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (wave == 0) {
     const int r = lane;
@@ -311,24 +310,55 @@ __device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A
       atomicCAS(status + 1, 0, (int)(k0 + bad_at));
       atomicOr(status, 2);
     }
-    // lane j: column j of L11^{-1}
-    const int j = lane;
-    double x[NB];
-#pragma unroll
-    for (int c = 0; c < NB; c++) {
-      double s0 = (c == j) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-      for (int k = 0; k + 1 < c; k += 2) {
-        s0 = fma(-Ls[c][k], x[k], s0);
-        s1 = fma(-Ls[c][k + 1], x[k + 1], s1);
-      }
-      if (c & 1) s0 = fma(-Ls[c][c - 1], x[c - 1], s0);
-      x[c] = (s0 + s1) * invd[c];
-    }
-#pragma unroll
-    for (int c = 0; c < NB; c++) Xs[c][j] = x[c];
   }
   __syncthreads();
+  // L11^{-1} on all four waves, in 16 x 16 blocks: wave w inverts the diagonal
+  // block D_w by forward substitution (lane j < 16: column j), then the
+  // blocks below the diagonal by distance d = i - j:
+  //     X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj
+  // (from (L X)_ij = 0), the 16 x 16 products on f64 MFMA 16x16x4.
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    if ((c >> 4) > (r >> 4)) Xs[r][c] = 0.0;
+  }
+  if (lane < 16) {
+    const int j = lane, o = 16 * wave;
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      double s = (c == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < c; k++) s = fma(-Ls[o + c][o + k], x[k], s);
+      x[c] = s * invd[o + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) Xs[o + c][o + j] = x[c];
+  }
+  __syncthreads();
+  for (int d = 1; d < 4; d++) {
+    if (wave < 4 - d) {
+      const int j = wave, i = wave + d;
+      // MFMA operands: A[l % 16][l / 16], B[l / 16][l % 16]; result D[4 r + l / 16][l % 16]
+      v4d t = {0.0, 0.0, 0.0, 0.0};
+      for (int k = j; k < i; k++)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++)
+          t = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[16 * i + (lane & 15)][16 * k + 4 * s4 + (lane >> 4)],
+                                                   Xs[16 * k + 4 * s4 + (lane >> 4)][16 * j + (lane & 15)], t, 0, 0,
+                                                   0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Ts[wave][4 * r + (lane >> 4)][lane & 15] = t[r];
+      wave_lds_sync();
+      v4d x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s4 = 0; s4 < 4; s4++)
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[16 * i + (lane & 15)][16 * i + 4 * s4 + (lane >> 4)],
+                                                 Ts[wave][4 * s4 + (lane >> 4)][lane & 15], x, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Xs[16 * i + 4 * r + (lane >> 4)][16 * j + (lane & 15)] = -x[r];
+    }
+    __syncthreads();
+  }
   {
     const int r = lane;
 #pragma unroll
