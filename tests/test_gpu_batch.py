@@ -35,7 +35,7 @@ def oracle_energies(oracle, frags):
     return out
 
 
-@pytest.mark.parametrize("mode", ["fp64", "fp32", "i8s5", "bf16x3"])
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "i8s5", "i8s4", "bf16x3"])
 def test_batch_matches_oracle(ctx, oracle, rimp2, mode):
     frags = make_batch(rimp2, 11, 9, scale=4)
     say("batch", mode, [f[0].shape for f in frags])
