@@ -47,8 +47,13 @@ void* DevBuf::ensure(size_t need) {
   return ptr;
 }
 
+// Frees after the device is idle: work queued on any stream (side streams
+// included) may still read the buffer.
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(ptr);
+  }
   ptr = nullptr;
   bytes = 0;
 }
@@ -182,6 +187,13 @@ static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc
 
 using namespace rimp2;
 
+static void destroy_streams(Ctx* c) {
+  for (hipEvent_t e : {c->ev_pack_go, c->ev_pack_done})
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t st : {c->stream, c->side_pack})
+    if (st) (void)hipStreamDestroy(st);
+}
+
 extern "C" {
 
 int rimp2_ctx_create(int device, void** ctx) {
@@ -195,9 +207,15 @@ int rimp2_ctx_create(int device, void** ctx) {
   Ctx* c = new Ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side_pack, hipStreamNonBlocking);
+  for (hipEvent_t* ev : {&c->ev_pack_go, &c->ev_pack_done})
+    if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+  // the "done" event starts recorded (on the idle side stream), so a first wait on it passes
+  if (e == hipSuccess) e = hipEventRecord(c->ev_pack_done, c->side_pack);
   if (e != hipSuccess) {
+    destroy_streams(c);
     delete c;
-    fail(RIMP2_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    fail(RIMP2_ERR_HIP, "stream or event creation failed: %s", hipGetErrorString(e));
   }
   *ctx = c;
   RIMP2_API_END(nullptr)
@@ -208,6 +226,7 @@ int rimp2_ctx_destroy(void* ctx) {
   if (!ctx) return RIMP2_OK;
   Ctx* c = as_ctx(ctx);
   (void)hipStreamSynchronize(c->stream);
+  if (c->side_pack) (void)hipStreamSynchronize(c->side_pack);
   resolve_timers(c);
   batch_destroy(c->batch);
   mlp_destroy(c->mlp);
@@ -216,7 +235,7 @@ int rimp2_ctx_destroy(void* ctx) {
                     &c->partials, &c->eps_pair, &c->etot, &c->status})
     b->release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(c->stream);
+  destroy_streams(c);
   delete c;
   RIMP2_API_END(nullptr)
 }
@@ -368,15 +387,28 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   validate_dims(naux, nvirt, nocc);
   validate_mode(mode);
   if (!d_Pia || !d_V || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
+  check_slice_bound(naux, mode);
   reset_status(c);
+  // (P|ia) depends on nothing computed here: its check and its packing into
+  // transform order run on a side stream, under the Cholesky
+  int* status = static_cast<int*>(c->status.ensure(64));
+  RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_go, c->stream));
+  RIMP2_HIP_CHECK(hipStreamWaitEvent(c->side_pack, c->ev_pack_go, 0));
+  check_finite_on(c->side_pack, status, d_Pia, naux * nvirt * nocc, 512);
+  metric_pack_x(ctx_metric_view(c), c->side_pack, naux, nvirt, nocc, d_Pia);
+  RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_done, c->side_pack));
   check_inputs_device(c, d_V, naux * naux, "V");
-  check_inputs_device(c, d_Pia, naux * nvirt * nocc, "Pia");
   check_inputs_device(c, d_eps_o, nocc, "eps_o");
   check_inputs_device(c, d_eps_v, nvirt, "eps_v");
   denom_check_device(c, d_eps_o, nocc, d_eps_v, nvirt);
   metric_factor(c, naux, d_V);
   c->have_metric = true;
-  transform_device(c, nvirt, nocc, d_Pia, mode);
+  RIMP2_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_pack_done, 0));
+  {
+    double* B = static_cast<double*>(c->B64.ensure((size_t)(naux * nvirt * nocc) * sizeof(double)));
+    metric_transform_packed(ctx_metric_view(c), naux, nvirt, nocc, B);
+  }
+  pack_current_B(c, nvirt, nocc, mode);
   pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,
                        d_e_tot, nullptr, 0);
   RIMP2_API_END(ctx)

@@ -707,14 +707,17 @@ void metric_factor(const MetricView& m, int64_t n, const double* dV_in) {
       const int kb = (int)((n - k0) < NB ? (n - k0) : NB);
       hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, L, n, Li, n, k0, kb, status);
       RIMP2_HIP_CHECK(hipGetLastError());
-      const int64_t r0 = k0 + kb, m = n - r0;
-      if (m <= 0) break;
+      const int64_t r0 = k0 + kb, mr = n - r0;
+      if (mr <= 0) break;
       // panel: L21 = A21 * L11^{-T}   (in place; one N tile covers kb <= 64)
-      gemm_f64<64, 64, true, false, false>(s, m, kb, kb, 1.0, L + r0 + n * k0, n, Li + k0 + n * k0, n, 0.0,
+      gemm_f64<64, 64, true, false, false>(s, mr, kb, kb, 1.0, L + r0 + n * k0, n, Li + k0 + n * k0, n, 0.0,
                                            L + r0 + n * k0, n);
       // trailing: A22 -= L21 L21^T  (lower tiles only; K = 64, so small
-      // tiles: the update is latency-bound and wants many workgroups)
-      gemm_f64<64, 64, true, false, true>(s, m, m, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
+      // tiles: the update is latency-bound and wants many workgroups).
+      // Splitting off the part beyond the next block column onto a second
+      // stream (to run under the next diagonal block) measured slower: the
+      // extra launch and stream wait per step cost more than the 15 us hidden.
+      gemm_f64<64, 64, true, false, true>(s, mr, mr, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
                                           L + r0 + n * r0, n);
     }
     hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_blocks(n * n, 256)), dim3(256), 0, s, L, n, n);
@@ -773,7 +776,30 @@ void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
 
 // dB = Linv * X  (X: n x (nvirt*nocc) device, column-major): X is packed
 // into B-fragment order, then one DMA-pipelined FP64 MFMA GEMM.
+void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t nocc, const double* dX) {
+  PackGeom gx;
+  gx.mode = RIMP2_MODE_FP64;
+  gx.rb = 16;
+  gx.kk = 4;
+  gx.planes = 1;
+  gx.lane_bytes = 8;
+  gx.nrb = round_up(nvirt, TR_TN) / 16;
+  gx.nks = round_up(n, TR_KQ) / 4;
+  gx.nvp = gx.nrb * 16;
+  gx.nqp = gx.nks * 4;
+  void* Xp = m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
+  // on a side stream, a capped grid: it runs beside the Cholesky's chain of
+  // small launches instead of competing with it for the whole GPU (512
+  // workgroups measured best: a full grid slows the Cholesky, 128 outlasts it)
+  pack_B(s == m.stream ? m.timer : nullptr, dX, n, nvirt, nocc, gx, Xp, nullptr, s, s == m.stream ? 0u : 512u);
+}
+
 void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* B) {
+  metric_pack_x(m, m.stream, n, nvirt, nocc, dX);
+  metric_transform_packed(m, n, nvirt, nocc, B);
+}
+
+void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, double* B) {
   TransformArgs p;
   p.naux = n;
   p.nvirt = nvirt;
@@ -789,21 +815,9 @@ void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t noc
          (long long)nvirt);
   p.nwu = ((p.MT + 1) / 2) * p.NT * nocc;  // paired row tiles
   p.chunk = ceil_div(p.nwu, 8);
-  PackGeom gx;
-  gx.mode = RIMP2_MODE_FP64;
-  gx.rb = 16;
-  gx.kk = 4;
-  gx.planes = 1;
-  gx.lane_bytes = 8;
-  gx.nvp = p.NRB * 16;
-  gx.nqp = p.KS * 4;
-  gx.nks = p.KS;
-  gx.nrb = p.NRB;
-  void* Xp = m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
   ScopedTimer t(m.timer, RIMP2_K_TRANSFORM);
-  pack_B(m.timer, dX, n, nvirt, nocc, gx, Xp, nullptr, m.stream);
   p.Lp = m.linv_packed.as<unsigned char>();
-  p.Xp = static_cast<const unsigned char*>(Xp);
+  p.Xp = m.x_packed.as<unsigned char>();
   p.B = B;
   const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
   hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, m.stream, p);

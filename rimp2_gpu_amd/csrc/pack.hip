@@ -264,11 +264,12 @@ __global__ __launch_bounds__(256) void pad_group_kernel(const PadJob* __restrict
 }  // namespace
 
 void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
-            double* dscales, hipStream_t s) {
+            double* dscales, hipStream_t s, unsigned max_blocks) {
   ScopedTimer tm(c, RIMP2_K_PACK);
   PackArgs p{dB, naux, nvirt, nocc, g.nks, g.nrb, dscales, dst};
   const int64_t threads = nocc * g.nks * g.nrb * 64;
-  const unsigned blocks = grid_blocks(threads, 256);
+  unsigned blocks = grid_blocks(threads, 256);
+  if (max_blocks && blocks > max_blocks) blocks = max_blocks;  // grid-stride kernels
   switch (g.mode) {
     case RIMP2_MODE_FP64:
       hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP64>, dim3(blocks), dim3(256), 0, s, p, g.nvp);

@@ -232,8 +232,8 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
-void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n) {
-  const unsigned blocks = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(1, ceil_div(n, 256 * 8)));
+void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n, unsigned max_blocks) {
+  const unsigned blocks = (unsigned)std::min<int64_t>(max_blocks, std::max<int64_t>(1, ceil_div(n, 256 * 8)));
   hipLaunchKernelGGL(finite_check_kernel, dim3(blocks), dim3(256), 0, s, p, n, status);
   RIMP2_HIP_CHECK(hipGetLastError());
 }

@@ -74,6 +74,10 @@ struct MlpState;
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Side stream (default priority) for the X packing that overlaps the
+  // Cholesky; two events hand work over (stream -> side, side -> stream).
+  hipStream_t side_pack = nullptr;
+  hipEvent_t ev_pack_go = nullptr, ev_pack_done = nullptr;
   std::string last_error;
   // metric state
   int64_t naux = 0;
@@ -122,6 +126,10 @@ MetricView ctx_metric_view(Ctx* c);
 void metric_factor(const MetricView& m, int64_t n, const double* dV_in);
 // dB[n, nvirt, nocc] = L^{-1} X  (after metric_factor on the same view)
 void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* dB);
+// The two halves of metric_transform: X into B-fragment order (m.x_packed) on
+// stream s, then the product from the packed X on m.stream.
+void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t nocc, const double* dX);
+void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, double* dB);
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
 void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
 // Grouped metric pipeline: the Cholesky, inverse and transform of many
@@ -144,8 +152,10 @@ void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
 
 // --- packing (pack.hip) ---
 PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile);
+// max_blocks (0: no limit) caps the grid, for packing that shares the GPU
+// with a latency-bound chain on another stream.
 void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
-            double* dscales, hipStream_t s);
+            double* dscales, hipStream_t s, unsigned max_blocks = 0);
 int mode_slices(int mode);
 // One fragment of a grouped packing (device arrays).
 struct PackItem {
@@ -173,7 +183,7 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
 // Status bits on a given stream: 4 = a non-finite value among p[0..n);
 // 8 = max(eo) >= min(ev) (some pair denominator >= 0).
-void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n);
+void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n, unsigned max_blocks = 8192);
 void check_denom_on(hipStream_t s, int* status, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
 
 // --- batch / mlp cleanup ---
