@@ -33,7 +33,10 @@ def check_E(mode, E, Eref):
         assert abs(E - Eref) <= TOL_REL[mode] * abs(Eref), (mode, E, Eref, (E - Eref) / Eref)
 
 
-@pytest.mark.parametrize("naux", [1, 5, 64, 100, 130, 512])
+# 64-wide Cholesky steps with ragged last blocks of 2, 8, 1 and 60 rows
+# (130, 200, 257, 700), and several levels of the inverse's doubling with a
+# ragged last group
+@pytest.mark.parametrize("naux", [1, 5, 64, 100, 128, 130, 200, 257, 512, 700])
 def test_metric_factor_and_inverse(ctx, oracle, rimp2, naux):
     say("metric naux", naux)
     _, V, _, _ = rimp2.gen_fragment(100 + naux, naux, 2, 1)
