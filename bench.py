@@ -219,6 +219,7 @@ def workload_batch(r, seed, nfrag, steps):
 
     for mode in MODES:
         t_first = timed(lambda: ctx.batch_prepare(frags, mode))
+        tuning = ctx.batch_tuning()
         t_host = timed(lambda: ctx.batch_prepare(frags, mode))
         ctx.batch_run()
         ef_host, _ = ctx.batch_result(want_pairs=False)
@@ -235,9 +236,10 @@ def workload_batch(r, seed, nfrag, steps):
                          prepare_device_ms=t_dev * 1e3, prepare_host_s=t_host, prepare_first_call_s=t_first,
                          time_to_energy_device_ms=(t_dev + t_run) * 1e3,
                          device_equals_host_input=bool(np.array_equal(ef, ef_host)),
-                         tile=ctx.batch_tile(), E_sum=float(np.sum(ef)), e_frag=ef)
+                         tile=ctx.batch_tile(), autotune_ms_by_tile={str(t): m for t, m in tuning},
+                         E_sum=float(np.sum(ef)), e_frag=ef)
         log(f"[bench] c3 {mode}: pair stage {t_run * 1e3:.3f} ms, prepare device {t_dev * 1e3:.1f} ms, "
-            f"host {t_host:.3f} s, first {t_first:.3f} s, tile {ctx.batch_tile()}")
+            f"host {t_host:.3f} s, first {t_first:.3f} s, tile {ctx.batch_tile()}, tuning {tuning}")
     for p in ptrs:
         ctx.free(p)
     import oracle

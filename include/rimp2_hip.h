@@ -156,6 +156,10 @@ int rimp2_batch_run(void* ctx, double* d_e_frag);
 int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed);
 /* Tile size chosen by the autotuner for the prepared batch (32 or 64). */
 int rimp2_batch_tile(void* ctx, int64_t* tile);
+/* The autotuner's measurements for the prepared batch: *n (<= 4) candidate
+ * tiles and the time of one pair-stage launch with each (ms).  *n = 0 when
+ * the choice came from the per-(mode, size class) cache. */
+int rimp2_batch_tuning(void* ctx, int* n, int64_t* tiles, double* ms);
 
 /* ---- pair-screening surrogate (BF16 MLP 16 -> 64 -> 64 -> 1, GELU) ---- */
 #define RIMP2_NFEAT 16

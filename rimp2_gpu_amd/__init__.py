@@ -66,6 +66,7 @@ SIGNATURES = {
     "rimp2_batch_run": (ctypes.c_int, [_vp, _vp]),
     "rimp2_batch_result": (ctypes.c_int, [_vp, _dp, _dp]),
     "rimp2_batch_tile": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
+    "rimp2_batch_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_i64), _dp]),
     "rimp2_pair_features": (ctypes.c_int, [_vp, _i64, _i64, _dp, _dp, _fp]),
     "rimp2_mlp_init": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "rimp2_mlp_set_norm": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_float, ctypes.c_float]),
@@ -353,6 +354,14 @@ class Context:
         pp = np.zeros(self._batch_npp) if want_pairs else None
         self._chk(lib().rimp2_batch_result(self.h, _ptr(ef), _ptr(pp)))
         return ef, pp
+
+    def batch_tuning(self):
+        """The autotuner's last measurements: [(tile, ms), ...] ([] when cached)."""
+        n = ctypes.c_int()
+        tiles = (_i64 * 4)()
+        ms = (ctypes.c_double * 4)()
+        self._chk(lib().rimp2_batch_tuning(self.h, ctypes.byref(n), tiles, ms))
+        return [(int(tiles[k]), float(ms[k])) for k in range(n.value)]
 
     def batch_tile(self):
         t = _i64()

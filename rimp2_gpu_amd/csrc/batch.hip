@@ -166,6 +166,7 @@ void launch_batch(int mode, int tile, hipStream_t s, const BatchParams& p, int n
       if (tile == 64) return launch_batch_t<PolF64, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_FP32:
+      if (tile == 32) return launch_batch_t<PolF32R16, 32>(s, p, ncu);
       if (tile == 64) return launch_batch_t<PolF32, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_BF16X3:
@@ -183,8 +184,8 @@ void launch_batch(int mode, int tile, hipStream_t s, const BatchParams& p, int n
 
 std::vector<int> tile_candidates(int mode) {
   switch (mode) {
-    case RIMP2_MODE_FP64: return {32, 64};
-    case RIMP2_MODE_FP32:
+    case RIMP2_MODE_FP64:
+    case RIMP2_MODE_FP32: return {32, 64};
     case RIMP2_MODE_BF16X3: return {64};
     default: return {64};
   }
@@ -535,6 +536,20 @@ int rimp2_batch(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode
   st = rimp2_batch_run(ctx, nullptr);
   if (st) return st;
   return rimp2_batch_result(ctx, e_frag, eps_pair_packed);
+}
+
+int rimp2_batch_tuning(void* ctx, int* n, int64_t* tiles, double* ms) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (!n || !tiles || !ms) fail(RIMP2_ERR_ARG, "null output pointer");
+  if (!c->batch || !c->batch->ready) fail(RIMP2_ERR_STATE, "rimp2_batch_prepare must run first");
+  const BatchState* b = c->batch;
+  *n = b->ntune;
+  for (int k = 0; k < b->ntune; k++) {
+    tiles[k] = b->tune_tiles[k];
+    ms[k] = b->tune_ms[k];
+  }
+  RIMP2_API_END(ctx)
 }
 
 int rimp2_batch_tile(void* ctx, int64_t* tile) {

@@ -10,6 +10,7 @@
 // the MFMA A[m][k] and B[k][n] lane maps coincide (m <-> n):
 //   fp64 16x16x4 : lane l holds (row l%16, k l/16)              8 bytes
 //   fp32 32x32x2 : lane l holds (row l%32, k l/32)              4 bytes
+//   fp32 16x16x4 : lane l holds (row l%16, k l/16)              4 bytes (32-wide tile)
 //   bf16 32x32x16: lane l holds (row l%32, k 8*(l/32)+e, e<8)  16 bytes
 //   i8   32x32x32: lane l holds (row l%32, k 16*(l/32)+e, e<16) 16 bytes
 // Padding rows (virtual a >= nvirt) and padding k (Q >= naux) are zero.
@@ -46,7 +47,9 @@ PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile) {
   g.mode = mode;
   switch (mode) {
     case RIMP2_MODE_FP64: g.rb = 16; g.kk = 4; g.planes = 1; g.lane_bytes = 8; break;
-    case RIMP2_MODE_FP32: g.rb = 32; g.kk = 2; g.planes = 1; g.lane_bytes = 4; break;
+    case RIMP2_MODE_FP32:  // 32-wide tile: 16x16x4 operands; else 32x32x2
+      if (tile == 32) { g.rb = 16; g.kk = 4; } else { g.rb = 32; g.kk = 2; }
+      g.planes = 1; g.lane_bytes = 4; break;
     case RIMP2_MODE_BF16X3: g.rb = 32; g.kk = 16; g.planes = 2; g.lane_bytes = 16; break;
     case RIMP2_MODE_I8S4:
     case RIMP2_MODE_I8S5: g.rb = 32; g.kk = 32; g.planes = mode_slices(mode); g.lane_bytes = 16; break;
@@ -114,6 +117,17 @@ __device__ inline void pack_one<RIMP2_MODE_FP32>(const PackArgs& p, int64_t t, i
   const int64_t f = t >> 6;
   const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
   const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
+  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
+}
+
+// FP32 in 16x16x4 operand order (the fp64 fragment shape with 4-byte lanes).
+constexpr int PACK_F32R16 = 16;  // packer id beside the mode numbers
+template <>
+__device__ inline void pack_one<PACK_F32R16>(const PackArgs& p, int64_t t, int64_t) {
+  const int lane = (int)(t & 63);
+  const int64_t f = t >> 6;
+  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
+  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
   static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
 }
 
@@ -275,7 +289,8 @@ void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc,
       hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP64>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
       break;
     case RIMP2_MODE_FP32:
-      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP32>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      if (g.rb == 16) hipLaunchKernelGGL(pack_kernel<PACK_F32R16>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      else hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP32>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
       break;
     case RIMP2_MODE_BF16X3:
       hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_BF16X3>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
@@ -303,7 +318,8 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
   std::vector<PadJob> dj;
   int64_t ne = 0, nr = 0, np = 0;
   for (const PackItem& it : items) {
-    if (it.g.mode != mode) fail(RIMP2_ERR_ARG, "grouped packing: geometry of another mode");
+    if (it.g.mode != mode || it.g.rb != items[0].g.rb)
+      fail(RIMP2_ERR_ARG, "grouped packing: geometries of another mode or tile");
     PackJob j;
     j.p = PackArgs{it.B, it.naux, it.nvirt, it.nocc, it.g.nks, it.g.nrb, it.sigma, it.dst};
     j.sigma = it.sigma;
@@ -342,7 +358,10 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
   const PackJob* jp = (const PackJob*)(d + opj);
   switch (mode) {
     case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP64>, gb, dim3(256), 0, s, jp, nj, ne); break;
-    case RIMP2_MODE_FP32: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP32>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_FP32:
+      if (items[0].g.rb == 16) hipLaunchKernelGGL(pack_group_kernel<PACK_F32R16>, gb, dim3(256), 0, s, jp, nj, ne);
+      else hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP32>, gb, dim3(256), 0, s, jp, nj, ne);
+      break;
     case RIMP2_MODE_BF16X3:
       hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_BF16X3>, gb, dim3(256), 0, s, jp, nj, ne);
       break;

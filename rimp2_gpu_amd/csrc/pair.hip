@@ -144,8 +144,8 @@ void launch_pair(hipStream_t s, const PairParams& p) {
 int choose_tile(int mode, int64_t nvirt, int64_t nocc) {
   (void)nocc;
   switch (mode) {
-    case RIMP2_MODE_FP64: return nvirt <= 48 ? 32 : 64;
-    case RIMP2_MODE_FP32:
+    case RIMP2_MODE_FP64:
+    case RIMP2_MODE_FP32: return nvirt <= 48 ? 32 : 64;
     case RIMP2_MODE_BF16X3: return 64;  // a 128 tile's split wave groups would spill at two waves per SIMD
     default: return 64;
   }
@@ -155,7 +155,7 @@ int choose_tile(int mode, int64_t nvirt, int64_t nocc) {
 bool pair_packs_diagonal(int mode, int tile) {
   switch (mode) {
     case RIMP2_MODE_FP64: return tile == 32 ? packs_diagonal<PolF64, 32>() : packs_diagonal<PolF64, 64>();
-    case RIMP2_MODE_FP32: return packs_diagonal<PolF32, 64>();
+    case RIMP2_MODE_FP32: return tile == 32 ? packs_diagonal<PolF32R16, 32>() : packs_diagonal<PolF32, 64>();
     case RIMP2_MODE_BF16X3: return packs_diagonal<PolBF16x3, 64>();
     case RIMP2_MODE_I8S4: return packs_diagonal<PolI8<4, 4>, 64>();
     default: return packs_diagonal<PolI8<5, 4>, 64>();
@@ -169,6 +169,7 @@ static void dispatch_pair(int mode, int tile, hipStream_t s, const PairParams& p
       if (tile == 64) return launch_pair<PolF64, 64>(s, p);
       break;
     case RIMP2_MODE_FP32:
+      if (tile == 32) return launch_pair<PolF32R16, 32>(s, p);
       if (tile == 64) return launch_pair<PolF32, 64>(s, p);
       break;
     case RIMP2_MODE_BF16X3:

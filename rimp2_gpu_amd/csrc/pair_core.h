@@ -64,6 +64,22 @@ struct PolF32 {
   __device__ static int col(int lane, int r) { return lane & 31; }
 };
 
+// FP32 on 16x16x4 MFMAs (the 32-wide tile): operands as PolF64 (lane l holds
+// row l % 16, k l / 16), result D[4 (l / 16) + r][l % 16] (the f32 layout,
+// not the f64 interleave).
+struct PolF32R16 {
+  static constexpr int RB = 16, KK = 4, P = 1, LB = 4, ND = 1, NE = 4, KSTEPS = 4, MMA_PER_CALL = 1;
+  using frag_t = float;
+  using acc_t = v4f;
+  __device__ static void zero(acc_t& a) { a = v4f{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static void mma(acc_t (&acc)[ND], const frag_t (&a)[P], const frag_t (&b)[P]) {
+    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc[0], 0, 0, 0);
+  }
+  __device__ static double value(const acc_t (&acc)[ND], int r) { return (double)acc[0][r]; }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static int col(int lane, int r) { return lane & 15; }
+};
+
 struct PolBF16x3 {
   static constexpr int RB = 32, KK = 16, P = 2, LB = 16, ND = 1, NE = 16, KSTEPS = 1, MMA_PER_CALL = 3;
   using frag_t = v8bf;

@@ -202,3 +202,24 @@ def test_batch_prepare_device_errors(ctx, rimp2):
         assert "fragment 1" in str(ei.value)
     ef, _ = ctx.batch([good], "fp64")
     assert ef[0] < 0
+
+
+def test_batch_autotuner_times_every_candidate(oracle, rimp2):
+    """A fresh context times both fp32 tiles (32 and 64) on the batch and keeps
+    the faster; a second prepare of the same size class uses the cached choice
+    (no measurements).  Both tiles give the oracle's fragment energies within
+    the fp32 tolerance (the 32-wide tile runs 16x16x4 MFMAs)."""
+    frags = make_batch(rimp2, 57, 5, scale=4)
+    with rimp2.Context(0) as c:
+        c.batch_prepare(frags, "fp32")
+        tuning = c.batch_tuning()
+        say("autotune fp32", tuning, "chosen", c.batch_tile())
+        assert sorted(t for t, _ in tuning) == [32, 64]
+        assert all(ms > 0 for _, ms in tuning)
+        assert c.batch_tile() == min(tuning, key=lambda x: x[1])[0]
+        c.batch_run()
+        ef, _ = c.batch_result(want_pairs=False)
+        c.batch_prepare(frags, "fp32")
+        assert c.batch_tuning() == []
+    for k, (Eref, _) in enumerate(oracle_energies(oracle, frags)):
+        assert abs(ef[k] - Eref) <= TOL_REL["fp32"] * abs(Eref), (k, ef[k], Eref)
