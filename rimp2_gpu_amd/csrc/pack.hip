@@ -200,6 +200,58 @@ __device__ inline void pack_one<RIMP2_MODE_I8S5>(const PackArgs& p, int64_t t, i
   pack_i8_one<5>(p, t, nvp);
 }
 
+// Exact-slice packing by tiles: one workgroup per (i, rb, 4 k-steps) = 32 rows
+// x 128 auxiliary values.  The tile is read with coalesced loads (one row of
+// 1 KB per wave instruction) into LDS, then thread (k-step f, lane) quantises the
+// 16 values of its fragment element from LDS and writes the S planes (each
+// wave: 1 KB per plane).  The per-element path reads 16 strided bytes from 64
+// rows per load instead.
+template <int S>
+__global__ __launch_bounds__(256) void pack_i8_tile_kernel(PackArgs p, int64_t nvp) {
+  __shared__ double tile[32][129];
+  const int64_t kt = p.nks / 4, ntiles = p.nocc * kt * p.nrb;
+  const int tid = threadIdx.x;
+  for (int64_t u = blockIdx.x; u < ntiles; u += gridDim.x) {
+    const int64_t rb = u % p.nrb, kq = (u / p.nrb) % kt, i = u / (p.nrb * kt);
+    const int64_t q0 = kq * 128;
+    {  // wave w reads rows 8w .. 8w+7, one row (128 q = 1 KB) per instruction, lane l q = 2l, 2l+1
+      const int w = tid >> 6, l = tid & 63;
+#pragma unroll
+      for (int rr = 0; rr < 8; rr++) {
+        const int row = 8 * w + rr;
+        tile[row][2 * l] = bget(p, q0 + 2 * l, rb * 32 + row, i);
+        tile[row][2 * l + 1] = bget(p, q0 + 2 * l + 1, rb * 32 + row, i);
+      }
+    }
+    __syncthreads();
+    const int f = tid >> 6, lane = tid & 63, row = lane & 31;
+    const int64_t a = rb * 32 + row;
+    const double sg = p.sigma[i * nvp + a];
+    const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
+    int8_t sl[S][16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      double x = tile[row][f * 32 + 16 * (lane >> 5) + e] * inv;
+#pragma unroll
+      for (int k = 0; k < S; k++) {
+        double sv = rint(x);
+        sv = fmin(127.0, fmax(-127.0, sv));
+        sl[k][e] = (int8_t)sv;
+        x = (x - sv) * 254.0;
+      }
+    }
+    const int64_t frag = (i * p.nks + kq * 4 + f) * p.nrb + rb;
+    uint4* dst = static_cast<uint4*>(p.dst);
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      uint4 v;
+      memcpy(&v, sl[k], 16);
+      dst[(frag * S + k) * 64 + lane] = v;
+    }
+    __syncthreads();  // the tile is reused by the next iteration
+  }
+}
+
 // one thread per (i, ks, rb, lane)
 template <int MODE>
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs p, int64_t nvp) {
@@ -300,10 +352,13 @@ void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc,
       hipLaunchKernelGGL(rowmax_kernel, dim3((unsigned)ceil_div(g.nvp * nocc, 4)), dim3(256), 0, s, dB, naux, nvirt,
                          nocc, g.nvp, dscales);
       RIMP2_HIP_CHECK(hipGetLastError());
+      // tiles of 4 k-steps (the auxiliary padding is a multiple of four stages of 32)
+      if (g.nks % 4) fail(RIMP2_ERR_ARG, "exact-slice packing: %lld k-steps not a multiple of 4", (long long)g.nks);
+      const unsigned tb = grid_blocks(nocc * (g.nks / 4) * g.nrb * 64, 64);  // tiles, capped
       if (g.mode == RIMP2_MODE_I8S4)
-        hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_I8S4>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+        hipLaunchKernelGGL(pack_i8_tile_kernel<4>, dim3(tb), dim3(256), 0, s, p, g.nvp);
       else
-        hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_I8S5>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+        hipLaunchKernelGGL(pack_i8_tile_kernel<5>, dim3(tb), dim3(256), 0, s, p, g.nvp);
       break;
     }
     default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", g.mode);
