@@ -419,7 +419,11 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // Issue is in order per wave: a DMA or LDS read that waits for a full queue
   // also holds back every MFMA behind it.  Interleaving one DMA and a share of
   // the LDS reads between groups of MFMAs keeps both queues short and the
-  // matrix pipe fed while the stage transfer proceeds.
+  // matrix pipe fed while the stage transfer proceeds.  Group order: one MFMA,
+  // the group's LDS reads, its other MFMAs, then the DMA (measured against
+  // six other orders on one box: 2% faster for the exact-slice kernel than
+  // DMA-reads-MFMAs, no worse for FP64; front-loading all LDS reads or all
+  // DMAs was slower).
   constexpr int NLDS = 2 * KS * FW * Pol::P;
   constexpr int NMMA = KS * FW * FW * Pol::MMA_PER_CALL;
   constexpr int GL = (NLDS + NI - 1) / NI, GM = (NMMA + NI - 1) / NI;
@@ -431,9 +435,10 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     mma(cur);
 #pragma unroll
     for (int g = 0; g < NI; g++) {
-      __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);   // VMEM read (the DMA)
-      __builtin_amdgcn_sched_group_barrier(0x0100, GL, 0);  // LDS reads
-      __builtin_amdgcn_sched_group_barrier(0x0008, GM, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);       // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x0100, GL, 0);      // LDS reads
+      __builtin_amdgcn_sched_group_barrier(0x0008, GM - 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);       // VMEM read (the DMA)
     }
   };
 
