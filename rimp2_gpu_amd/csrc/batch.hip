@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchFrag* frag
   cc[threadIdx.x] = c;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+    if ((int)threadIdx.x < w) {
       double a = ss[threadIdx.x], b = ss[threadIdx.x + w], u = a + b;
       double e = (fabs(a) >= fabs(b)) ? (a - u) + b : (b - u) + a;
       ss[threadIdx.x] = u;

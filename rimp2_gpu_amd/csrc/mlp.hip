@@ -96,7 +96,7 @@ __global__ void rownorm_kernel(const double* B, int64_t naux, int64_t nvirt, int
 }
 
 // one workgroup per unique pair; 8 slabs; raw (unstandardised) features
-__global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int64_t naux, int64_t nocc,
+__global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int64_t naux, int64_t /*nocc*/,
                                                             const double* eo, const double* ev, int64_t nvirt,
                                                             float* F) {
   const int64_t p = blockIdx.x;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void pair_features_kernel(const float* r, int6
   evs[1][threadIdx.x] = esum;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+    if ((int)threadIdx.x < w) {
       for (int s = 0; s < 8; s++) sl[s][threadIdx.x] += sl[s][threadIdx.x + w];
       ni[threadIdx.x] += ni[threadIdx.x + w];
       nj[threadIdx.x] += nj[threadIdx.x + w];

@@ -45,7 +45,7 @@ struct PolF64 {
   }
   __device__ static double value(const acc_t (&acc)[ND], int r) { return acc[0][r]; }
   __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
-  __device__ static int col(int lane, int r) { return lane & 15; }
+  __device__ static int col(int lane, int /*r*/) { return lane & 15; }
 };
 
 struct PolF32 {
@@ -61,7 +61,7 @@ struct PolF32 {
   }
   __device__ static double value(const acc_t (&acc)[ND], int r) { return (double)acc[0][r]; }
   __device__ static int row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
-  __device__ static int col(int lane, int r) { return lane & 31; }
+  __device__ static int col(int lane, int /*r*/) { return lane & 31; }
 };
 
 // FP32 on 16x16x4 MFMAs (the 32-wide tile): operands as PolF64 (lane l holds
@@ -77,7 +77,7 @@ struct PolF32R16 {
   }
   __device__ static double value(const acc_t (&acc)[ND], int r) { return (double)acc[0][r]; }
   __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-  __device__ static int col(int lane, int r) { return lane & 15; }
+  __device__ static int col(int lane, int /*r*/) { return lane & 15; }
 };
 
 struct PolBF16x3 {
@@ -93,7 +93,7 @@ struct PolBF16x3 {
   }
   __device__ static double value(const acc_t (&acc)[ND], int r) { return (double)acc[0][r]; }
   __device__ static int row(int lane, int r) { return PolF32::row(lane, r); }
-  __device__ static int col(int lane, int r) { return lane & 31; }
+  __device__ static int col(int lane, int /*r*/) { return lane & 31; }
 };
 
 // Exact-slice int8: S slices, diagonals d = s + t <= DMAX accumulate into
@@ -127,7 +127,7 @@ struct PolI8 {
     return v * (1.0 / (127.0 * 127.0));
   }
   __device__ static int row(int lane, int r) { return PolF32::row(lane, r); }
-  __device__ static int col(int lane, int r) { return lane & 31; }
+  __device__ static int col(int lane, int /*r*/) { return lane & 31; }
 };
 
 template <class Pol>
