@@ -12,6 +12,7 @@ energy difference in microhartree.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode i8s5] [--no-cpu]
   python bench.py --workload c1|c3|c4|c5     (the other configurations, every mode)
+  python bench.py --workload eos             (DEC local-basis 4-index arrays, Steps 5-8)
 
 Prints ONE JSON line (plus progress on stderr).
 """
@@ -279,6 +280,90 @@ def workload_surrogate(r, seed):
     return out
 
 
+def ref_fixture(nvirt, nocc, noccEOS, nvirtEOS, nba):
+    """The reference's fixture inputs (generate.F90:77-128): every Calpha
+    element A*0.00124235636/A + B*0.01010101/B + C*0.014321/C, constant orbital
+    energies and U matrices of the form a*x/a + b*y/b, and the closed-form value
+    each of the four output arrays then takes in every element."""
+    c = 0.00124235636 + 0.01010101 + 0.014321
+    d = 2 * 1.934321 - 2 * 1.91321
+    uoE, uv, uo, uvE = 0.030321 + 0.101121, 0.021321 + 0.01321, 0.032321 + 0.01521, 0.023321 + 0.01421
+    args = (np.full((nba, nvirt, nocc), c, order="F"), np.full(nocc, 1.934321), np.full(nvirt, 1.91321),
+            np.full((nocc, noccEOS), uoE, order="F"), np.full((nvirt, nvirt), uv, order="F"),
+            np.full((nocc, nocc), uo, order="F"), np.full((nvirt, nvirtEOS), uvE, order="F"))
+    expect = (nba * c * c / d * (nocc * uoE) ** 2 * (nvirt * uv) ** 2,
+              nba * c * c / d * (nvirt * uvE) ** 2 * (nocc * uo) ** 2,
+              nba * (nvirt * uv * nocc * uoE * c) ** 2, nba * (nvirt * uvE * nocc * uo * c) ** 2)
+    return args, expect
+
+
+def workload_eos(r, steps, warmup, cpu=True):
+    """DEC local-basis 4-index arrays (Major Steps 5-8, RIMP2.F90:67-170) at the
+    larger fragment size written in generate.F90:47-52 (nvirt=1041, nocc=99,
+    noccEOS=14, nvirtEOS=45, NBA=2173), inputs resident in HBM, FP64 MFMA.
+    Inputs are the reference's fixture formula, so every output element has a
+    closed form (checked to 1e-12 relative; the reference's 1e-10 absolute
+    check is meant for its small fixtures).  The CPU leg times the oracle on
+    the reference's own fixture size (generate.F90:58-63), where the GPU is
+    timed as well."""
+    dims = (1041, 99, 14, 45, 2173)
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    F, Fa = r.eos_flops(*dims)
+    args, expect = ref_fixture(*dims)
+    ctx = r.Context(0)
+    d_in = [ctx.to_device(x) for x in args]
+    shapes = [(nvirt, noccEOS, nvirt, noccEOS), (nvirtEOS, nocc, nvirtEOS, nocc)] * 2
+    d_out = [ctx.alloc(int(np.prod(s)) * 8) for s in shapes]
+    log(f"[bench] eos {dims}: warmup {warmup}, steps {steps}")
+    for _ in range(warmup):
+        ctx.eos_arrays_device(*dims, d_in, d_out)
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.eos_arrays_device(*dims, d_in, d_out)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / steps
+    ampl_ms, ampl_n = ctx.timer("eos_ampl")
+    eos_ms, _ = ctx.timer("eos")
+    ctx.timing(False)
+    outs = [ctx.from_device(p, s) for p, s in zip(d_out, shapes)]
+    rel = [float(np.max(np.abs(o - e)) / abs(e)) for o, e in zip(outs, expect)]
+    for p in d_in + d_out:
+        ctx.free(p)
+    per_launch_s = ampl_ms / max(ampl_n, 1) / 1e3
+    launches = ampl_n / steps
+    res = dict(workload="eos", dims=dict(nvirt=nvirt, nocc=nocc, noccEOS=noccEOS, nvirtEOS=nvirtEOS, nba=nba),
+               ms_per_call=t * 1e3, ms_per_call_events=eos_ms / steps, flops=F, tflops=F / t / 1e12,
+               amplitude=dict(launches_per_call=launches, ms_per_launch=per_launch_s * 1e3,
+                              tflops=Fa / launches / per_launch_s / 1e12,
+                              frac_fp64_peak=Fa / launches / per_launch_s / 1e12 / PEAK["fp64_tflops"],
+                              share_of_call=ampl_ms / max(eos_ms, 1e-9)),
+               max_rel_err_vs_closed_form=max(rel))
+    log(f"[bench] eos: {t * 1e3:.2f} ms/call, {F / t / 1e12:.2f} TFLOP/s; amplitude GEMM "
+        f"{per_launch_s * 1e3:.3f} ms x {launches:.0f}, {res['amplitude']['tflops']:.2f} TFLOP/s; rel err {rel}")
+    # the reference's own fixture size: GPU and the oracle on the host cores
+    small = (104, 20, 4, 15, 373)
+    sargs, sexp = ref_fixture(*small)
+    ctx.eos_arrays(*sargs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        souts = ctx.eos_arrays(*sargs)
+    t_small = (time.perf_counter() - t0) / steps
+    res["fixture_104_20_4_15_373"] = dict(gpu_ms_host_arrays=t_small * 1e3,
+                                          max_abs_err=max(float(np.max(np.abs(o - e))) for o, e in zip(souts, sexp)))
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.steps5to8(*sargs)
+        t_cpu = time.perf_counter() - t0
+        res["fixture_104_20_4_15_373"].update(oracle_s=t_cpu, oracle_threads=oracle.num_threads(),
+                                              gpu_speedup=t_cpu / t_small)
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,7 +377,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-memory (PCIe-inclusive) run")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5"],
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "eos"],
                     help="run one of the other configurations in every mode (one JSON line) instead of the "
                          "headline step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -314,6 +399,8 @@ def main():
             res = workload_single(r, "c5", 5, 6144, 2048, 256, 1, 0, 4)
         elif a.workload == "c3":
             res = workload_batch(r, 3, 256, a.steps)
+        elif a.workload == "eos":
+            res = workload_eos(r, min(a.steps, 3), 1, cpu=not a.no_cpu)
         else:
             res = workload_surrogate(r, 4)
         print(json.dumps(res), flush=True)

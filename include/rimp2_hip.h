@@ -56,7 +56,9 @@ extern "C" {
 #define RIMP2_K_BATCH 6
 #define RIMP2_K_MLP 7
 #define RIMP2_K_FEATURES 8
-#define RIMP2_NUM_KERNELS 9
+#define RIMP2_K_EOS 9        /* a whole rimp2_eos_arrays call */
+#define RIMP2_K_EOS_AMPL 10  /* its amplitude GEMMs (1/Delta epilogue) */
+#define RIMP2_NUM_KERNELS 11
 
 /* ---- context, device memory query, errors ---- */
 
@@ -223,6 +225,41 @@ int rimp2_gen_fragment_device(void* ctx, uint64_t seed, int64_t naux, int64_t nv
                               double* d_Pia, double* d_V, double* d_eps_o, double* d_eps_v);
 /* nocc ~ U{4..32}, nvirt ~ U{32..256}, naux = 4 nvirt per fragment. */
 int rimp2_gen_batch_shapes(uint64_t seed, int64_t nfrag, int64_t* nocc, int64_t* nvirt, int64_t* naux);
+
+/* ---- general FP64 GEMM and the DEC local-basis 4-index arrays ---- */
+/* Replaces cublasDgemm_v2 as bound in gpu_interfaces.F90:24-32 and called at
+ * RIMP2.GPU.F90:119,135,172,186,213,244: C = alpha op(A) op(B) + beta C,
+ * column-major, device pointers, asynchronous on the context stream (the
+ * binding's handle + cublasSetStream pair becomes the context).  transa /
+ * transb: 0 = N, 1 = T (2 = C, the same for real data; 'N'/'T'/'C' also
+ * accepted).  alpha and beta are host pointers (cuBLAS host pointer mode);
+ * beta = 0 does not read C.  Leading dimensions are checked as BLAS does. */
+int rimp2_dgemm(void* ctx, int transa, int transb, int64_t m, int64_t n, int64_t k, const double* alpha,
+                const double* d_A, int64_t lda, const double* d_B, int64_t ldb, const double* beta, double* d_C,
+                int64_t ldc);
+/* Replaces Major Steps 5-8 of the GPU host (RIMP2.GPU.F90:103-260; CPU form
+ * RIMP2.F90:67-170): from Calpha(NBA,nvirt,nocc), EVocc(nocc), EVvirt(nvirt),
+ * UoccEOST(nocc,noccEOS), UvirtT(nvirt,nvirt), UoccT(nocc,nocc) and
+ * UvirtEOST(nvirt,nvirtEOS) forms toccEOS(nvirt,noccEOS,nvirt,noccEOS),
+ * tvirtEOS(nvirtEOS,nocc,nvirtEOS,nocc), goccEOS(nvirt,noccEOS,nvirt,noccEOS)
+ * and gvirtEOS(nvirtEOS,nocc,nvirtEOS,nocc), all FP64 on MFMA.  As in the
+ * reference the denominators e_I + e_J - e_A - e_B are not sign-checked (its
+ * own fixture has them > 0).  Blocking host-pointer form; the _device form
+ * takes device pointers and is asynchronous on the context stream. */
+int rimp2_eos_arrays(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                     const double* Calpha, const double* EVocc, const double* EVvirt, const double* UoccEOST,
+                     const double* UvirtT, const double* UoccT, const double* UvirtEOST, double* toccEOS,
+                     double* tvirtEOS, double* goccEOS, double* gvirtEOS);
+int rimp2_eos_arrays_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                            const double* d_Calpha, const double* d_EVocc, const double* d_EVvirt,
+                            const double* d_UoccEOST, const double* d_UvirtT, const double* d_UoccT,
+                            const double* d_UvirtEOST, double* d_toccEOS, double* d_tvirtEOS, double* d_goccEOS,
+                            double* d_gvirtEOS);
+/* Occupied orbitals per amplitude chunk of rimp2_eos_arrays (0 = automatic:
+ * the chunk's amplitude slab, nvirt^2 nocc doubles per orbital, and its
+ * partial transforms stay within 8 GiB).  Bounds the work memory; has no
+ * counterpart in the reference, which holds every amplitude at once. */
+int rimp2_eos_set_chunk(void* ctx, int64_t nocc_per_chunk);
 
 #ifdef __cplusplus
 }

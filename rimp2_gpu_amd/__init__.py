@@ -20,7 +20,8 @@ ERR_ARG, ERR_HIP, ERR_NOT_SPD, ERR_STATE, ERR_NOMEM, ERR_NUMERIC = 1, 2, 3, 4, 5
 MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2, "i8s4": 3, "i8s5": 4}
 MODE_NAMES = {v: k for k, v in MODES.items()}
 KERNELS = {"pair": 0, "transform": 1, "pack": 2, "cholesky": 3, "trtri": 4, "reduce": 5,
-           "batch": 6, "mlp": 7, "features": 8}
+           "batch": 6, "mlp": 7, "features": 8, "eos": 9,
+           "eos_ampl": 10}
 NFEAT = 16
 
 _i64 = ctypes.c_int64
@@ -93,6 +94,11 @@ SIGNATURES = {
                                                  _vp, _vp]),
     "rimp2_gen_batch_shapes": (ctypes.c_int, [ctypes.c_uint64, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                               ctypes.POINTER(_i64)]),
+    "rimp2_dgemm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i64, _i64, _i64, _dp, _vp, _i64, _vp, _i64, _dp,
+                                   _vp, _i64]),
+    "rimp2_eos_arrays": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_dp] * 11),
+    "rimp2_eos_arrays_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_vp] * 11),
+    "rimp2_eos_set_chunk": (ctypes.c_int, [_vp, _i64]),
 }
 
 
@@ -368,6 +374,43 @@ class Context:
         self._chk(lib().rimp2_batch_tile(self.h, ctypes.byref(t)))
         return t.value
 
+    # ---- DEC local-basis 4-index arrays (RIMP2.F90:67-170) ----
+    def eos_arrays(self, Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST):
+        """Major Steps 5-8 on the GPU: returns (toccEOS, tvirtEOS, goccEOS,
+        gvirtEOS) as Fortran-ordered arrays of the reference's shapes."""
+        C = _f64(Calpha)
+        if C.ndim != 3:
+            raise ValueError("Calpha must be (NBA, nvirt, nocc)")
+        nba, nvirt, nocc = C.shape
+        UoE, Uv, Uo, UvE = _f64(UoccEOST), _f64(UvirtT), _f64(UoccT), _f64(UvirtEOST)
+        noccEOS, nvirtEOS = UoE.shape[1], UvE.shape[1]
+        if (UoE.shape[0] != nocc or Uv.shape != (nvirt, nvirt) or Uo.shape != (nocc, nocc)
+                or UvE.shape[0] != nvirt):
+            raise ValueError("U matrix shapes do not match (nocc, nvirt)")
+        eo, ev = _f64(EVocc, (nocc,)), _f64(EVvirt, (nvirt,))
+        to = np.zeros((nvirt, noccEOS, nvirt, noccEOS), order="F")
+        tv = np.zeros((nvirtEOS, nocc, nvirtEOS, nocc), order="F")
+        go = np.zeros((nvirt, noccEOS, nvirt, noccEOS), order="F")
+        gv = np.zeros((nvirtEOS, nocc, nvirtEOS, nocc), order="F")
+        self._chk(lib().rimp2_eos_arrays(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, _ptr(C), _ptr(eo), _ptr(ev),
+                                         _ptr(UoE), _ptr(Uv), _ptr(Uo), _ptr(UvE), _ptr(to), _ptr(tv), _ptr(go),
+                                         _ptr(gv)))
+        return to, tv, go, gv
+
+    def eos_arrays_device(self, nvirt, nocc, noccEOS, nvirtEOS, nba, d_in, d_out):
+        """d_in: 7 device pointers (Calpha, EVocc, EVvirt, UoccEOST, UvirtT,
+        UoccT, UvirtEOST); d_out: 4 (toccEOS, tvirtEOS, goccEOS, gvirtEOS)."""
+        self._chk(lib().rimp2_eos_arrays_device(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, *d_in, *d_out))
+
+    def eos_set_chunk(self, nocc_per_chunk):
+        self._chk(lib().rimp2_eos_set_chunk(self.h, int(nocc_per_chunk)))
+
+    def dgemm(self, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc):
+        """cublasDgemm_v2 semantics on device pointers (asynchronous)."""
+        a, b = ctypes.c_double(alpha), ctypes.c_double(beta)
+        self._chk(lib().rimp2_dgemm(self.h, int(transa), int(transb), m, n, k, ctypes.byref(a), d_A, lda, d_B, ldb,
+                                    ctypes.byref(b), d_C, ldc))
+
     # ---- surrogate ----
     def pair_features(self, eps_o, eps_v):
         eo, ev = _f64(eps_o), _f64(eps_v)
@@ -431,6 +474,20 @@ class Context:
 def pair_flops(nocc, nvirt, naux):
     """Algorithmic FLOPs of the pair contraction over unique pairs i <= j."""
     return 2.0 * naux * nvirt * nvirt * nocc * (nocc + 1) / 2.0
+
+
+def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
+    """Algorithmic FLOPs of rimp2_eos_arrays in its order of work: the
+    amplitude build once for both amplitude arrays, the transforms of each
+    array (the index that shrinks most first), and Steps 7-8 as C3^T C3.
+    Returns (total, amplitude build)."""
+    nv, no, nE, vE = float(nvirt), float(nocc), float(noccEOS), float(nvirtEOS)
+    ampl = 2 * nba * nv * nv * no * no
+    tocc = 2 * nv * nv * no * no * nE + 2 * nv * nv * nE * no * nE + 4 * nv ** 3 * nE * nE
+    tvirt = 2 * vE * nv * nv * no * no + 2 * vE * vE * nv * no * no + 4 * vE * vE * no ** 3
+    gocc = 2 * nba * nv * no * nE + 2 * nba * nv * nv * nE + 2 * (nv * nE) ** 2 * nba
+    gvirt = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * (vE * no) ** 2 * nba
+    return ampl + tocc + tvirt + gocc + gvirt, ampl
 
 
 def metric_flops(nocc, nvirt, naux):

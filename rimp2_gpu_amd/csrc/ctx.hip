@@ -230,6 +230,7 @@ int rimp2_ctx_destroy(void* ctx) {
   resolve_timers(c);
   batch_destroy(c->batch);
   mlp_destroy(c->mlp);
+  eos_destroy(c->eos);
   for (DevBuf* b : {&c->V, &c->Linv, &c->work, &c->linv_packed, &c->X, &c->B64, &c->packed, &c->scales, &c->x_packed,
                     &c->eps_o, &c->eps_v,
                     &c->partials, &c->eps_pair, &c->etot, &c->status})

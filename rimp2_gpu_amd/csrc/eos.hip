@@ -1,0 +1,750 @@
+// DEC local-basis 4-index arrays (toccEOS, tvirtEOS, goccEOS, gvirtEOS) and
+// the general FP64 GEMM behind them.
+//
+// Reference: RIMP2.F90:67-170 (Major Steps 5-8, CPU) and RIMP2.GPU.F90:103-260
+// (the same steps as OpenACC kernels + cublasDgemm_v2).  Every output is a
+// 4-index transform of the canonical amplitude or integral:
+//   t(I,J,A,B) = g(A,I,B,J) / (e_I + e_J - e_B - e_A),  g = sum_alpha C(alpha,A,I) C(alpha,B,J)
+//   toccEOS(Al,Il,Bl,Jl) = sum UvirtT(A,Al) UoccEOST(I,Il) UvirtT(B,Bl) UoccEOST(J,Jl) t(I,J,A,B)
+//   tvirtEOS(Al,Il,Bl,Jl) = sum UvirtEOST(A,Al) UoccT(I,Il) UvirtEOST(B,Bl) UoccT(J,Jl) t(I,J,A,B)
+//   goccEOS / gvirtEOS: the same transforms of g, formed as C3^T C3 of the
+//   transformed 3-index tensor (RIMP2.F90:136-170).
+//
+// Order of work here (not the reference's): both amplitude arrays share one
+// t, built chunk by chunk over I as a batched GEMM with the 1/Delta applied in
+// its epilogue (t never exists whole); from each chunk the occupied pair of
+// toccEOS and the virtual pair of tvirtEOS are contracted at once (the index
+// of each array that shrinks most goes first), and the remaining two indices
+// of each are transformed after the last chunk.  Every step is one launch of
+// the FP64 MFMA GEMM below with two-level batch strides, so the reorders of
+// toccB / tvirtB (RIMP2.F90:456-508) are just output strides.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "rimp2_common.h"
+
+namespace rimp2 {
+
+struct EosState {
+  DevBuf in, work, out;
+};
+void eos_destroy(EosState* s) {
+  if (!s) return;
+  s->in.release();
+  s->work.release();
+  s->out.release();
+  delete s;
+}
+
+namespace {
+
+using v4d = double __attribute__((ext_vector_type(4)));
+
+enum { EPI_PLAIN = 0, EPI_AMPL = 1 };
+
+// C[z] = alpha op(A[z]) op(B[z]) + beta C[z] for batch members z = z1 + Z1 z2,
+// member offsets z1 * s?1 + z2 * s?2 (elements).  EPI_AMPL: C(m, n) of member
+// z is the amplitude of occupied pair (I = i0 + z, J = n / nv), virtuals
+// (A = m, B = n % nv): acc / (e_I + e_J - e_B - e_A).
+struct DgemmArgs {
+  int64_t M, N, K;
+  double alpha, beta;
+  const double* A;
+  int64_t lda, sA1, sA2;
+  const double* B;
+  int64_t ldb, sB1, sB2;
+  double* C;
+  int64_t ldc, sC1, sC2;
+  int64_t Z1, zbase;
+  int64_t tm, tiles;  // row tiles, tiles per member
+  int a_vec, b_vec;   // 16-byte loads allowed (even strides, aligned bases)
+  const double* eo;
+  const double* ev;
+  int64_t nv, i0;
+};
+
+// 256 threads = 4 waves in 2x2, wave tile 32x32 of 16x16 MFMA blocks
+// (v_mfma_f64_16x16x4f64; lane l holds A[l%16][l/16], B[l/16][l%16], result
+// d[r] = D[4r + l/16][l%16]).  K tiles of 16 go through LDS with the next
+// tile fetched into registers while the MFMAs run.  LDS layouts keep every
+// fragment read free of bank conflicts: a stored-untransposed A as As[k][m]
+// (pitch BM+16), a transposed A as As[m][k] (pitch 17); B likewise.
+// Tiles are numbered row tile fastest and laid out so that each XCD (blocks
+// b with the same b % 8) runs one contiguous range of them: the row tiles
+// that share a column panel of op(B) share that XCD's L2.
+template <int BM, int BN, bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void dgemm_kernel(const DgemmArgs p) {
+  constexpr int BK = 16;
+  constexpr int PA = TA ? BK + 1 : BM + 16;
+  constexpr int PB = TB ? BN + 16 : BK + 1;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int NA = BM * BK / 512, NB = BN * BK / 512;
+  __shared__ double As[TA ? BM * PA : BK * PA];
+  __shared__ double Bs[TB ? BK * PB : BN * PB];
+
+  const int64_t G = gridDim.x;  // a multiple of 8
+  const int64_t b = blockIdx.x;
+  const int64_t t = (b & 7) * (G >> 3) + (b >> 3);
+  if (t >= p.tiles) return;
+  const int64_t z = p.zbase + blockIdx.y;
+  const int64_t z1 = z % p.Z1, z2 = z / p.Z1;
+  const double* __restrict__ A = p.A + z1 * p.sA1 + z2 * p.sA2;
+  const double* __restrict__ B = p.B + z1 * p.sB1 + z2 * p.sB2;
+  double* __restrict__ C = p.C + z1 * p.sC1 + z2 * p.sC2;
+  const int64_t M = p.M, N = p.N, K = p.K, lda = p.lda, ldb = p.ldb;
+  const int64_t m0 = (t % p.tm) * BM, n0 = (t / p.tm) * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+
+  v4d acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  double2 ra[NA], rb[NB];
+  // element (m, k) of op(A): A[m + lda k] (TA: A[k + lda m]); pairs run along
+  // the contiguous index
+  auto load_a = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < NA; s++) {
+      const int e = tid + 256 * s;
+      double2 v = {0.0, 0.0};
+      if (!TA) {
+        const int k = e / (BM / 2), m = (e % (BM / 2)) * 2;
+        const int64_t gm = m0 + m, gk = k0 + k;
+        if (gk < K) {
+          const double* q = A + gm + lda * gk;
+          if (p.a_vec && gm + 1 < M) v = *reinterpret_cast<const double2*>(q);
+          else {
+            if (gm < M) v.x = q[0];
+            if (gm + 1 < M) v.y = q[1];
+          }
+        }
+      } else {
+        const int m = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        const int64_t gm = m0 + m, gk = k0 + k;
+        if (gm < M) {
+          const double* q = A + gk + lda * gm;
+          if (p.a_vec && gk + 1 < K) v = *reinterpret_cast<const double2*>(q);
+          else {
+            if (gk < K) v.x = q[0];
+            if (gk + 1 < K) v.y = q[1];
+          }
+        }
+      }
+      ra[s] = v;
+    }
+  };
+  // element (k, n) of op(B): B[k + ldb n] (TB: B[n + ldb k])
+  auto load_b = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < NB; s++) {
+      const int e = tid + 256 * s;
+      double2 v = {0.0, 0.0};
+      if (!TB) {
+        const int n = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        const int64_t gn = n0 + n, gk = k0 + k;
+        if (gn < N) {
+          const double* q = B + gk + ldb * gn;
+          if (p.b_vec && gk + 1 < K) v = *reinterpret_cast<const double2*>(q);
+          else {
+            if (gk < K) v.x = q[0];
+            if (gk + 1 < K) v.y = q[1];
+          }
+        }
+      } else {
+        const int k = e / (BN / 2), n = (e % (BN / 2)) * 2;
+        const int64_t gn = n0 + n, gk = k0 + k;
+        if (gk < K) {
+          const double* q = B + gn + ldb * gk;
+          if (p.b_vec && gn + 1 < N) v = *reinterpret_cast<const double2*>(q);
+          else {
+            if (gn < N) v.x = q[0];
+            if (gn + 1 < N) v.y = q[1];
+          }
+        }
+      }
+      rb[s] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int s = 0; s < NA; s++) {
+      const int e = tid + 256 * s;
+      if (!TA) {
+        const int k = e / (BM / 2), m = (e % (BM / 2)) * 2;
+        As[k * PA + m] = ra[s].x;
+        As[k * PA + m + 1] = ra[s].y;
+      } else {
+        const int m = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        As[m * PA + k] = ra[s].x;
+        As[m * PA + k + 1] = ra[s].y;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NB; s++) {
+      const int e = tid + 256 * s;
+      if (!TB) {
+        const int n = e / (BK / 2), k = (e % (BK / 2)) * 2;
+        Bs[n * PB + k] = rb[s].x;
+        Bs[n * PB + k + 1] = rb[s].y;
+      } else {
+        const int k = e / (BN / 2), n = (e % (BN / 2)) * 2;
+        Bs[k * PB + n] = rb[s].x;
+        Bs[k * PB + n + 1] = rb[s].y;
+      }
+    }
+  };
+
+  if (K > 0) {
+    load_a(0);
+    load_b(0);
+  }
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    store();
+    __syncthreads();
+    if (k0 + BK < K) {
+      load_a(k0 + BK);
+      load_b(k0 + BK);
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kq = ks + (lane >> 4);
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; i++) {
+        const int m = wm + 16 * i + (lane & 15);
+        af[i] = TA ? As[m * PA + kq] : As[kq * PA + m];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; j++) {
+        const int n = wn + 16 * j + (lane & 15);
+        bf[j] = TB ? Bs[kq * PB + n] : Bs[n * PB + kq];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++) {
+      const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+      if (gn >= N) continue;
+      double dn = 0.0;
+      if (EPI == EPI_AMPL) {
+        const int64_t J = gn / p.nv, Bv = gn - J * p.nv;
+        dn = (p.eo[p.i0 + z] + p.eo[J]) - p.ev[Bv];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t gm = m0 + wm + 16 * i + 4 * r + (lane >> 4);
+        if (gm >= M) continue;
+        double* q = C + gm + p.ldc * gn;
+        if (EPI == EPI_AMPL) {
+          *q = acc[i][j][r] / (dn - p.ev[gm]);
+        } else {
+          double v = p.alpha * acc[i][j][r];
+          if (p.beta != 0.0) v += p.beta * *q;
+          *q = v;
+        }
+      }
+    }
+}
+
+constexpr int kBM = 64, kBN = 64;
+
+// Host description of one batched GEMM (element strides).
+struct Gemm {
+  bool ta = false, tb = false;
+  int64_t M = 0, N = 0, K = 0;
+  double alpha = 1.0, beta = 0.0;
+  const double* A = nullptr;
+  int64_t lda = 1, sA1 = 0, sA2 = 0;
+  const double* B = nullptr;
+  int64_t ldb = 1, sB1 = 0, sB2 = 0;
+  double* C = nullptr;
+  int64_t ldc = 1, sC1 = 0, sC2 = 0;
+  int64_t Z1 = 1, Z2 = 1;
+  // amplitude epilogue
+  bool ampl = false;
+  const double* eo = nullptr;
+  const double* ev = nullptr;
+  int64_t nv = 1, i0 = 0;
+};
+
+bool even_aligned(const double* base, int64_t ld, int64_t s1, int64_t s2) {
+  return (reinterpret_cast<uintptr_t>(base) & 15) == 0 && (ld & 1) == 0 && (s1 & 1) == 0 && (s2 & 1) == 0;
+}
+
+template <bool TA, bool TB, int EPI>
+void launch_one(hipStream_t s, const DgemmArgs& a, unsigned gx, unsigned gy) {
+  hipLaunchKernelGGL((dgemm_kernel<kBM, kBN, TA, TB, EPI>), dim3(gx, gy), dim3(256), 0, s, a);
+}
+
+void run_gemm(hipStream_t s, const Gemm& g) {
+  if (g.M <= 0 || g.N <= 0 || g.Z1 <= 0 || g.Z2 <= 0) return;
+  DgemmArgs a;
+  a.M = g.M;
+  a.N = g.N;
+  a.K = g.K;
+  a.alpha = g.alpha;
+  a.beta = g.beta;
+  a.A = g.A;
+  a.lda = g.lda;
+  a.sA1 = g.sA1;
+  a.sA2 = g.sA2;
+  a.B = g.B;
+  a.ldb = g.ldb;
+  a.sB1 = g.sB1;
+  a.sB2 = g.sB2;
+  a.C = g.C;
+  a.ldc = g.ldc;
+  a.sC1 = g.sC1;
+  a.sC2 = g.sC2;
+  a.Z1 = g.Z1;
+  a.tm = ceil_div(g.M, kBM);
+  a.tiles = a.tm * ceil_div(g.N, kBN);
+  a.a_vec = even_aligned(g.A, g.lda, g.sA1, g.sA2);
+  a.b_vec = even_aligned(g.B, g.ldb, g.sB1, g.sB2);
+  a.eo = g.eo;
+  a.ev = g.ev;
+  a.nv = g.nv;
+  a.i0 = g.i0;
+  const int64_t gx = round_up(a.tiles, 8);
+  // one dispatch counts blocks x 256 work-items in 32 bits; grid.y <= 65535
+  if (gx > kMaxGridBlocks) fail(RIMP2_ERR_ARG, "GEMM of %lld x %lld exceeds the launch limit", (long long)g.M,
+                                (long long)g.N);
+  const int64_t zmax = std::max<int64_t>(1, std::min<int64_t>(65535, kMaxGridBlocks / gx));
+  const int64_t Z = g.Z1 * g.Z2;
+  for (int64_t z0 = 0; z0 < Z; z0 += zmax) {
+    a.zbase = z0;
+    const unsigned gy = (unsigned)std::min<int64_t>(zmax, Z - z0);
+    if (g.ampl) launch_one<true, false, EPI_AMPL>(s, a, (unsigned)gx, gy);
+    else if (!g.ta && !g.tb) launch_one<false, false, EPI_PLAIN>(s, a, (unsigned)gx, gy);
+    else if (g.ta && !g.tb) launch_one<true, false, EPI_PLAIN>(s, a, (unsigned)gx, gy);
+    else if (!g.ta && g.tb) launch_one<false, true, EPI_PLAIN>(s, a, (unsigned)gx, gy);
+    else launch_one<true, true, EPI_PLAIN>(s, a, (unsigned)gx, gy);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+}
+
+// Sizes (elements) of the work arena of the EOS pipeline with ci occupied
+// orbitals per amplitude chunk.
+struct EosPlan {
+  int64_t nv, no, nE, vE, nba, ci;
+  int64_t T, P, S, Q, R, c2, c3;  // element counts
+  size_t off[8];                  // byte offsets: T P S S' Q R R' c2 (c3 after)
+  size_t bytes;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, int64_t ci) {
+  EosPlan p;
+  p.nv = nv;
+  p.no = no;
+  p.nE = nE;
+  p.vE = vE;
+  p.nba = nba;
+  p.ci = ci;
+  p.T = nv * nv * no * ci;
+  p.P = nv * nv * nE * ci;
+  p.S = nv * nv * nE * nE;
+  p.Q = vE * nv * no * ci;
+  p.R = vE * vE * no * no;
+  p.c2 = nba * nv * std::max(nE, no);
+  p.c3 = nba * std::max(nv * nE, vE * no);
+  const int64_t sizes[9] = {p.T, p.P, p.S, p.S, p.Q, p.R, p.R, p.c2, p.c3};
+  size_t o = 0;
+  for (int i = 0; i < 9; i++) {
+    if (i < 8) p.off[i] = o;
+    o += align256((size_t)sizes[i] * sizeof(double));
+  }
+  p.bytes = o;
+  return p;
+}
+
+// Chunk of occupied orbitals per amplitude slab: the t slab (nv^2 no doubles
+// per orbital) and its two partial transforms stay within 8 GiB.
+int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE) {
+  const double per = (double)(nv * nv * no + nv * nv * nE + vE * nv * no) * sizeof(double);
+  const double budget = 8.0 * (1ull << 30);
+  int64_t ci = (int64_t)(budget / per);
+  return std::max<int64_t>(1, std::min<int64_t>(no, ci));
+}
+
+}  // namespace
+
+int64_t eos_chunk_size(int64_t nv, int64_t no, int64_t nE, int64_t vE) { return eos_chunk(nv, no, nE, vE); }
+
+// Queues Steps 5-8 on c->stream (device arrays; outputs in the reference's
+// layouts, RIMP2.F90:92,123,146,169).
+void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, const double* Cal,
+                       const double* eo, const double* ev, const double* Uo5, const double* Uv5, const double* Uo6,
+                       const double* Uv6, double* toccEOS, double* tvirtEOS, double* goccEOS, double* gvirtEOS,
+                       int64_t ci) {
+  if (!c->eos) c->eos = new EosState();
+  if (ci <= 0) ci = eos_chunk(nv, no, nE, vE);
+  ci = std::min(ci, no);
+  const EosPlan pl = eos_plan(nv, no, nE, vE, nba, ci);
+  unsigned char* w = static_cast<unsigned char*>(c->eos->work.ensure(pl.bytes));
+  double* T = reinterpret_cast<double*>(w + pl.off[0]);
+  double* P = reinterpret_cast<double*>(w + pl.off[1]);
+  double* S = reinterpret_cast<double*>(w + pl.off[2]);
+  double* S2 = reinterpret_cast<double*>(w + pl.off[3]);
+  double* Q = reinterpret_cast<double*>(w + pl.off[4]);
+  double* R = reinterpret_cast<double*>(w + pl.off[5]);
+  double* R2 = reinterpret_cast<double*>(w + pl.off[6]);
+  double* c2 = reinterpret_cast<double*>(w + pl.off[7]);
+  double* c3 = reinterpret_cast<double*>(w + pl.off[7] + align256((size_t)pl.c2 * sizeof(double)));
+  hipStream_t s = c->stream;
+  ScopedTimer tm(c, RIMP2_K_EOS);
+  const int64_t nv2 = nv * nv;
+
+  for (int64_t i0 = 0; i0 < no; i0 += ci) {
+    const int64_t cz = std::min(ci, no - i0);
+    {  // t(A, B, J, I) for I in [i0, i0 + cz):  C_I^T C / Delta   (RIMP2.F90:399-404)
+      Gemm g;
+      g.ampl = true;
+      g.ta = true;
+      g.M = nv;
+      g.N = nv * no;
+      g.K = nba;
+      g.A = Cal + nba * nv * i0;
+      g.lda = nba;
+      g.sA1 = nba * nv;
+      g.B = Cal;
+      g.ldb = nba;
+      g.C = T;
+      g.ldc = nv;
+      g.sC1 = nv2 * no;
+      g.Z1 = cz;
+      g.eo = eo;
+      g.ev = ev;
+      g.nv = nv;
+      g.i0 = i0;
+      ScopedTimer ta(c, RIMP2_K_EOS_AMPL);
+      run_gemm(s, g);
+    }
+    {  // toccEOS, occupied J: P(A,B,Jl,I) = sum_J t(A,B,J,I) UoccEOST(J,Jl)
+      Gemm g;
+      g.M = nv2;
+      g.N = nE;
+      g.K = no;
+      g.A = T;
+      g.lda = nv2;
+      g.sA1 = nv2 * no;
+      g.B = Uo5;
+      g.ldb = no;
+      g.C = P;
+      g.ldc = nv2;
+      g.sC1 = nv2 * nE;
+      g.Z1 = cz;
+      run_gemm(s, g);
+    }
+    {  // toccEOS, occupied I: S(A,B,Jl,Il) += sum_I P(A,B,Jl,I) UoccEOST(I,Il)
+      Gemm g;
+      g.M = nv2 * nE;
+      g.N = nE;
+      g.K = cz;
+      g.A = P;
+      g.lda = nv2 * nE;
+      g.B = Uo5 + i0;
+      g.ldb = no;
+      g.beta = i0 > 0 ? 1.0 : 0.0;
+      g.C = S;
+      g.ldc = nv2 * nE;
+      run_gemm(s, g);
+    }
+    {  // tvirtEOS, virtual A: Q(Al, B, J, I) = sum_A UvirtEOST(A,Al) t(A,B,J,I)
+      Gemm g;
+      g.ta = true;
+      g.M = vE;
+      g.N = nv * no * cz;
+      g.K = nv;
+      g.A = Uv6;
+      g.lda = nv;
+      g.B = T;
+      g.ldb = nv;
+      g.C = Q;
+      g.ldc = vE;
+      run_gemm(s, g);
+    }
+    {  // tvirtEOS, virtual B: R(Al,Bl,J,I) = sum_B Q(Al,B,J,I) UvirtEOST(B,Bl)
+      Gemm g;
+      g.M = vE;
+      g.N = vE;
+      g.K = nv;
+      g.A = Q;
+      g.lda = vE;
+      g.sA1 = vE * nv;
+      g.B = Uv6;
+      g.ldb = nv;
+      g.C = R + vE * vE * no * i0;
+      g.ldc = vE;
+      g.sC1 = vE * vE;
+      g.Z1 = no * cz;
+      run_gemm(s, g);
+    }
+  }
+  {  // toccEOS, virtual A: S2(Al,B,Jl,Il) = sum_A UvirtT(A,Al) S(A,B,Jl,Il)
+    Gemm g;
+    g.ta = true;
+    g.M = nv;
+    g.N = nv * nE * nE;
+    g.K = nv;
+    g.A = Uv5;
+    g.lda = nv;
+    g.B = S;
+    g.ldb = nv;
+    g.C = S2;
+    g.ldc = nv;
+    run_gemm(s, g);
+  }
+  {  // toccEOS(Al,Il,Bl,Jl) = sum_B S2(Al,B,Jl,Il) UvirtT(B,Bl)   (toccB reorder, RIMP2.F90:456-481)
+    Gemm g;
+    g.M = nv;
+    g.N = nv;
+    g.K = nv;
+    g.A = S2;
+    g.lda = nv;
+    g.sA1 = nv2;
+    g.sA2 = nv2 * nE;
+    g.B = Uv5;
+    g.ldb = nv;
+    g.C = toccEOS;
+    g.ldc = nv * nE;
+    g.sC1 = nv2 * nE;
+    g.sC2 = nv;
+    g.Z1 = nE;
+    g.Z2 = nE;
+    run_gemm(s, g);
+  }
+  {  // tvirtEOS, occupied I: R2(Al,Bl,J,Il) = sum_I R(Al,Bl,J,I) UoccT(I,Il)
+    Gemm g;
+    g.M = vE * vE * no;
+    g.N = no;
+    g.K = no;
+    g.A = R;
+    g.lda = vE * vE * no;
+    g.B = Uo6;
+    g.ldb = no;
+    g.C = R2;
+    g.ldc = vE * vE * no;
+    run_gemm(s, g);
+  }
+  {  // tvirtEOS(Al,Il,Bl,Jl) = sum_J R2(Al,Bl,J,Il) UoccT(J,Jl)   (tvirtB reorder, RIMP2.F90:483-508)
+    Gemm g;
+    g.M = vE;
+    g.N = no;
+    g.K = no;
+    g.A = R2;
+    g.lda = vE * vE;
+    g.sA1 = vE;
+    g.sA2 = vE * vE * no;
+    g.B = Uo6;
+    g.ldb = no;
+    g.C = tvirtEOS;
+    g.ldc = vE * no * vE;
+    g.sC1 = vE * no;
+    g.sC2 = vE;
+    g.Z1 = vE;
+    g.Z2 = no;
+    run_gemm(s, g);
+  }
+  // Step 7 (RIMP2.F90:136-147): goccEOS = C3^T C3 with
+  // C3(alpha,Bl,Jl) = sum_B UvirtT(B,Bl) sum_J C(alpha,B,J) UoccEOST(J,Jl)
+  // Step 8 (RIMP2.F90:158-170): gvirtEOS likewise with UoccT and UvirtEOST.
+  for (int step = 7; step <= 8; step++) {
+    const int64_t nO = step == 7 ? nE : no, nV = step == 7 ? nv : vE;
+    const double* Uo = step == 7 ? Uo5 : Uo6;
+    const double* Uv = step == 7 ? Uv5 : Uv6;
+    double* out = step == 7 ? goccEOS : gvirtEOS;
+    {
+      Gemm g;
+      g.M = nba * nv;
+      g.N = nO;
+      g.K = no;
+      g.A = Cal;
+      g.lda = nba * nv;
+      g.B = Uo;
+      g.ldb = no;
+      g.C = c2;
+      g.ldc = nba * nv;
+      run_gemm(s, g);
+    }
+    {
+      Gemm g;
+      g.M = nba;
+      g.N = nV;
+      g.K = nv;
+      g.A = c2;
+      g.lda = nba;
+      g.sA1 = nba * nv;
+      g.B = Uv;
+      g.ldb = nv;
+      g.C = c3;
+      g.ldc = nba;
+      g.sC1 = nba * nV;
+      g.Z1 = nO;
+      run_gemm(s, g);
+    }
+    {
+      Gemm g;
+      g.ta = true;
+      g.M = nV * nO;
+      g.N = nV * nO;
+      g.K = nba;
+      g.A = c3;
+      g.lda = nba;
+      g.B = c3;
+      g.ldb = nba;
+      g.C = out;
+      g.ldc = nV * nO;
+      run_gemm(s, g);
+    }
+  }
+}
+
+void dgemm_device(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                  int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  Gemm g;
+  g.ta = ta;
+  g.tb = tb;
+  g.M = m;
+  g.N = n;
+  g.K = k;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  run_gemm(s, g);
+}
+
+}  // namespace rimp2
+
+using namespace rimp2;
+
+namespace {
+
+int parse_op(int op, const char* which) {
+  switch (op) {
+    case 0: case 'N': case 'n': return 0;
+    case 1: case 2: case 'T': case 't': case 'C': case 'c': return 1;
+    default: fail(RIMP2_ERR_ARG, "%s = %d is not N (0) or T (1, 2)", which, op);
+  }
+}
+
+void eos_validate(int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba) {
+  if (nvirt < 1 || nocc < 1 || noccEOS < 1 || nvirtEOS < 1 || nba < 1)
+    fail(RIMP2_ERR_ARG, "dimensions must be >= 1 (nvirt=%lld nocc=%lld noccEOS=%lld nvirtEOS=%lld nba=%lld)",
+         (long long)nvirt, (long long)nocc, (long long)noccEOS, (long long)nvirtEOS, (long long)nba);
+  if (noccEOS > nocc || nvirtEOS > nvirt)
+    fail(RIMP2_ERR_ARG, "noccEOS=%lld > nocc=%lld or nvirtEOS=%lld > nvirt=%lld", (long long)noccEOS,
+         (long long)nocc, (long long)nvirtEOS, (long long)nvirt);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rimp2_dgemm(void* ctx, int transa, int transb, int64_t m, int64_t n, int64_t k, const double* alpha,
+                const double* d_A, int64_t lda, const double* d_B, int64_t ldb, const double* beta, double* d_C,
+                int64_t ldc) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  const int ta = parse_op(transa, "transa"), tb = parse_op(transb, "transb");
+  if (m < 0 || n < 0 || k < 0) fail(RIMP2_ERR_ARG, "negative dimension (m=%lld n=%lld k=%lld)", (long long)m,
+                                    (long long)n, (long long)k);
+  if (!alpha || !beta) fail(RIMP2_ERR_ARG, "null alpha or beta");
+  const int64_t ra = ta ? k : m, rb = tb ? n : k;
+  if (lda < std::max<int64_t>(1, ra) || ldb < std::max<int64_t>(1, rb) || ldc < std::max<int64_t>(1, m))
+    fail(RIMP2_ERR_ARG, "leading dimension too small (lda=%lld ldb=%lld ldc=%lld)", (long long)lda, (long long)ldb,
+         (long long)ldc);
+  if (m == 0 || n == 0) return RIMP2_OK;
+  if (!d_C || (k > 0 && (!d_A || !d_B))) fail(RIMP2_ERR_ARG, "null device pointer");
+  dgemm_device(c->stream, ta, tb, m, n, k, *alpha, d_A, lda, d_B, ldb, *beta, d_C, ldc);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_eos_arrays_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                            const double* d_Calpha, const double* d_EVocc, const double* d_EVvirt,
+                            const double* d_UoccEOST, const double* d_UvirtT, const double* d_UoccT,
+                            const double* d_UvirtEOST, double* d_toccEOS, double* d_tvirtEOS, double* d_goccEOS,
+                            double* d_gvirtEOS) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  eos_validate(nvirt, nocc, noccEOS, nvirtEOS, nba);
+  if (!d_Calpha || !d_EVocc || !d_EVvirt || !d_UoccEOST || !d_UvirtT || !d_UoccT || !d_UvirtEOST || !d_toccEOS ||
+      !d_tvirtEOS || !d_goccEOS || !d_gvirtEOS)
+    fail(RIMP2_ERR_ARG, "null device pointer");
+  eos_arrays_device(c, nvirt, nocc, noccEOS, nvirtEOS, nba, d_Calpha, d_EVocc, d_EVvirt, d_UoccEOST, d_UvirtT,
+                    d_UoccT, d_UvirtEOST, d_toccEOS, d_tvirtEOS, d_goccEOS, d_gvirtEOS, c->eos_chunk);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_eos_arrays(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                     const double* Calpha, const double* EVocc, const double* EVvirt, const double* UoccEOST,
+                     const double* UvirtT, const double* UoccT, const double* UvirtEOST, double* toccEOS,
+                     double* tvirtEOS, double* goccEOS, double* gvirtEOS) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  eos_validate(nvirt, nocc, noccEOS, nvirtEOS, nba);
+  if (!Calpha || !EVocc || !EVvirt || !UoccEOST || !UvirtT || !UoccT || !UvirtEOST || !toccEOS || !tvirtEOS ||
+      !goccEOS || !gvirtEOS)
+    fail(RIMP2_ERR_ARG, "null pointer");
+  if (!c->eos) c->eos = new EosState();
+  const int64_t nv = nvirt, no = nocc, nE = noccEOS, vE = nvirtEOS;
+  // inputs, in order, and outputs
+  const int64_t nin[7] = {nba * nv * no, no, nv, no * nE, nv * nv, no * no, nv * vE};
+  const double* hin[7] = {Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST};
+  const int64_t nout[4] = {nv * nE * nv * nE, vE * no * vE * no, nv * nE * nv * nE, vE * no * vE * no};
+  double* hout[4] = {toccEOS, tvirtEOS, goccEOS, gvirtEOS};
+  size_t offi[7], offo[4], bi = 0, bo = 0;
+  for (int i = 0; i < 7; i++) {
+    offi[i] = bi;
+    bi += align256((size_t)nin[i] * sizeof(double));
+  }
+  for (int i = 0; i < 4; i++) {
+    offo[i] = bo;
+    bo += align256((size_t)nout[i] * sizeof(double));
+  }
+  unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
+  unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
+  for (int i = 0; i < 7; i++)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(din + offi[i], hin[i], (size_t)nin[i] * sizeof(double), hipMemcpyHostToDevice,
+                                   c->stream));
+  const double* d[7];
+  for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
+  double* o[4];
+  for (int i = 0; i < 4; i++) o[i] = reinterpret_cast<double*>(dout + offo[i]);
+  eos_arrays_device(c, nv, no, nE, vE, nba, d[0], d[1], d[2], d[3], d[4], d[5], d[6], o[0], o[1], o[2], o[3], c->eos_chunk);
+  for (int i = 0; i < 4; i++)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(hout[i], o[i], (size_t)nout[i] * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_eos_set_chunk(void* ctx, int64_t nocc_per_chunk) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (nocc_per_chunk < 0) fail(RIMP2_ERR_ARG, "nocc_per_chunk = %lld < 0", (long long)nocc_per_chunk);
+  c->eos_chunk = nocc_per_chunk;
+  RIMP2_API_END(ctx)
+}
+
+}  // extern "C"

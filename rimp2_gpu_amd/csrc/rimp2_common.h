@@ -70,6 +70,7 @@ struct PendingTimer {
 
 struct BatchState;
 struct MlpState;
+struct EosState;
 
 struct Ctx {
   int device = 0;
@@ -100,6 +101,10 @@ struct Ctx {
   // batch + surrogate
   BatchState* batch = nullptr;
   MlpState* mlp = nullptr;
+  // DEC local-basis arrays: work buffers, occupied orbitals per amplitude
+  // chunk (0: sized automatically)
+  EosState* eos = nullptr;
+  int64_t eos_chunk = 0;
 };
 
 // Records hipEvents around a launch on ctx->stream when timing is on (no-op
@@ -189,6 +194,7 @@ void check_denom_on(hipStream_t s, int* status, const double* eo, int64_t nocc, 
 // --- batch / mlp cleanup ---
 void batch_destroy(BatchState* b);
 void mlp_destroy(MlpState* m);
+void eos_destroy(EosState* s);
 
 // helpers
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -1,0 +1,177 @@
+"""GPU parity of the DEC local-basis 4-index arrays (Major Steps 5-8,
+RIMP2.F90:67-170) and of the general FP64 GEMM (cublasDgemm_v2 semantics,
+gpu_interfaces.F90:24-32), through the C ABI.
+
+Tolerances: the reference's own check, |test - got| <= 1e-10 absolute for
+every element (RIMP2.F90:262 and its three siblings), and in addition
+<= 1e-12 of the largest oracle entry (summation order is the only
+difference: FP64 MFMA against the oracle's scalar loops).  GEMM: 1e-13 of the
+largest |A||B| product sum.
+"""
+import numpy as np
+import pytest
+
+from test_oracle import FIXTURES, ref_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def orth(rng, n, k):
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return np.asfortranarray(q[:, :k])
+
+
+def random_inputs(seed, nvirt, nocc, noccEOS, nvirtEOS, nba):
+    """Physically shaped: Calpha ~ 0.1 N(0,1), occupied below virtual
+    energies, orthonormal columns in every U (orbital rotations)."""
+    rng = np.random.default_rng(seed)
+    C = np.asfortranarray(0.1 * rng.standard_normal((nba, nvirt, nocc)))
+    eo = rng.uniform(-2.0, -0.3, nocc)
+    ev = rng.uniform(0.1, 3.0, nvirt)
+    return C, eo, ev, orth(rng, nocc, noccEOS), orth(rng, nvirt, nvirt), orth(rng, nocc, nocc), \
+        orth(rng, nvirt, nvirtEOS)
+
+
+def check_arrays(got, exp):
+    names = ("toccEOS", "tvirtEOS", "goccEOS", "gvirtEOS")
+    for name, g, e in zip(names, got, exp):
+        assert g.shape == e.shape, (name, g.shape, e.shape)
+        err = float(np.max(np.abs(g - e)))
+        scale = float(np.max(np.abs(e)))
+        say(f"  {name}: max|d| = {err:.3e} (max|ref| {scale:.3e})")
+        assert err <= 1e-10, (name, err)
+        assert err <= 1e-12 * scale, (name, err, scale)
+
+
+# the reference's own fixtures (generate.F90:41-46, 58-63), fed through the
+# GPU: every element equals the closed form within the reference's 1e-10
+@pytest.mark.parametrize("dims,expect", FIXTURES)
+def test_eos_reference_fixture(ctx, dims, expect):
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    say(f"eos reference fixture {dims}")
+    ctx.eos_set_chunk(0)
+    outs = ctx.eos_arrays(*ref_fixture(nvirt, nocc, noccEOS, nvirtEOS, nba))
+    for arr, val in zip(outs, expect):
+        assert np.all(np.isfinite(arr))
+        assert float(np.max(np.abs(arr - val))) <= 1e-10
+
+
+# shapes: the reference's two fixture sizes, a one-orbital case, ragged
+# tiles (65 = 64 + 1 virtuals), full EOS spaces (noccEOS = nocc,
+# nvirtEOS = nvirt); chunk = occupied orbitals per amplitude slab (0: auto)
+SHAPES = [
+    ((74, 10, 4, 15, 273), 0),
+    ((74, 10, 4, 15, 273), 1),
+    ((74, 10, 4, 15, 273), 3),
+    ((104, 20, 4, 15, 373), 0),
+    ((1, 1, 1, 1, 1), 0),
+    ((65, 3, 2, 64, 17), 2),
+    ((33, 7, 7, 33, 130), 0),
+]
+
+
+@pytest.mark.parametrize("dims,chunk", SHAPES)
+def test_eos_random_vs_oracle(ctx, oracle, dims, chunk):
+    say(f"eos random {dims} chunk={chunk}")
+    args = random_inputs(17 + sum(dims), *dims)
+    ctx.eos_set_chunk(chunk)
+    try:
+        got = ctx.eos_arrays(*args)
+    finally:
+        ctx.eos_set_chunk(0)
+    check_arrays(got, oracle.steps5to8(*args))
+
+
+def test_eos_device_entry_matches_host_entry(ctx):
+    dims = (74, 10, 4, 15, 273)
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    args = random_inputs(5, *dims)
+    say("eos device entry")
+    host = ctx.eos_arrays(*args)
+    d_in = [ctx.to_device(x) for x in args]
+    shapes = [(nvirt, noccEOS, nvirt, noccEOS), (nvirtEOS, nocc, nvirtEOS, nocc)] * 2
+    d_out = [ctx.alloc(int(np.prod(s)) * 8) for s in shapes]
+    try:
+        ctx.eos_arrays_device(nvirt, nocc, noccEOS, nvirtEOS, nba, d_in, d_out)
+        ctx.sync()
+        dev = [ctx.from_device(p, s) for p, s in zip(d_out, shapes)]
+    finally:
+        for p in d_in + d_out:
+            ctx.free(p)
+    for h, d in zip(host, dev):
+        np.testing.assert_array_equal(h, d)
+
+
+def test_eos_rejects_bad_dimensions(ctx, rimp2):
+    args = random_inputs(1, 8, 3, 2, 4, 5)
+    C, eo, ev, UoE, Uv, Uo, UvE = args
+    with pytest.raises(rimp2.Rimp2Error) as e:
+        ctx.eos_arrays(C, eo, ev, np.asfortranarray(np.ones((3, 4))), Uv, Uo, UvE)  # noccEOS 4 > nocc 3
+    assert e.value.code == rimp2.ERR_ARG
+    with pytest.raises(rimp2.Rimp2Error) as e:
+        ctx.eos_set_chunk(-1)
+    assert e.value.code == rimp2.ERR_ARG
+
+
+# ---------------------------------------------------------------------------
+# rimp2_dgemm: all four op combinations, ragged sizes, odd and even leading
+# dimensions (scalar and 16-byte load paths), alpha / beta, k = 0, and beta = 0
+# not reading C (NaN there stays unseen)
+# ---------------------------------------------------------------------------
+GEMM_CASES = [(1, 1, 1, 0), (70, 33, 129, 3), (128, 64, 16, 0), (65, 130, 47, 1), (5, 200, 0, 2), (200, 5, 300, 4)]
+
+
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("tb", [0, 1])
+@pytest.mark.parametrize("m,n,k,pad", GEMM_CASES)
+def test_dgemm_matches_numpy(ctx, ta, tb, m, n, k, pad):
+    rng = np.random.default_rng(m * 7 + n * 3 + k + ta + 2 * tb)
+    ra, ca = (k, m) if ta else (m, k)
+    rb, cb = (n, k) if tb else (k, n)
+    lda, ldb, ldc = max(1, ra) + pad, max(1, rb) + pad, max(1, m) + pad
+    A = np.asfortranarray(rng.standard_normal((lda, max(ca, 1))))
+    B = np.asfortranarray(rng.standard_normal((ldb, max(cb, 1))))
+    C0 = np.asfortranarray(rng.standard_normal((ldc, n)))
+    opA = A[:ra, :ca].T if ta else A[:ra, :ca]
+    opB = B[:rb, :cb].T if tb else B[:rb, :cb]
+    say(f"dgemm ta={ta} tb={tb} m={m} n={n} k={k} pad={pad}")
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    try:
+        for alpha, beta in ((1.0, 0.0), (1.5, -0.5)):
+            C = C0.copy(order="F")
+            if beta == 0.0:
+                C[:m, :] = np.nan
+            dC = ctx.to_device(C)
+            try:
+                ctx.dgemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+                ctx.sync()
+                got = ctx.from_device(dC, C.shape)
+            finally:
+                ctx.free(dC)
+            exp = C0.copy(order="F")
+            exp[:m, :] = alpha * (opA @ opB) + (beta * C0[:m, :] if beta != 0.0 else 0.0)
+            scale = max(1.0, float(np.max(np.abs(opA) @ np.abs(opB))) if k else 1.0)
+            assert np.all(np.isfinite(got[:m, :]))
+            assert float(np.max(np.abs(got[:m, :] - exp[:m, :]))) <= 1e-13 * scale * max(1.0, abs(alpha))
+            # rows below m (the padding of ldc) are untouched
+            np.testing.assert_array_equal(got[m:, :], C0[m:, :])
+    finally:
+        ctx.free(dA)
+        ctx.free(dB)
+
+
+def test_dgemm_rejects_bad_arguments(ctx, rimp2):
+    p = ctx.alloc(64 * 8)
+    try:
+        for args in ((5, 0, 4, 4, 4, 4, 4, 4), (0, 0, 4, 4, 4, 3, 4, 4), (0, 0, 4, 4, 4, 4, 3, 4),
+                     (0, 0, 4, 4, 4, 4, 4, 3), (0, 0, -1, 4, 4, 4, 4, 4), (1, 0, 4, 4, 5, 4, 4, 4)):
+            ta, tb, m, n, k, lda, ldb, ldc = args
+            with pytest.raises(rimp2.Rimp2Error) as e:
+                ctx.dgemm(ta, tb, m, n, k, 1.0, p, lda, p, ldb, 0.0, p, ldc)
+            assert e.value.code == rimp2.ERR_ARG, args
+    finally:
+        ctx.free(p)
