@@ -308,7 +308,8 @@ def workload_eos(r, steps, warmup, cpu=True):
     timed as well."""
     dims = (1041, 99, 14, 45, 2173)
     nvirt, nocc, noccEOS, nvirtEOS, nba = dims
-    F, Fa = r.eos_flops(*dims)
+    chunk = r.eos_chunk(nvirt, nocc, noccEOS, nvirtEOS)
+    F, Fa = r.eos_flops(*dims, chunk)
     args, expect = ref_fixture(*dims)
     ctx = r.Context(0)
     d_in = [ctx.to_device(x) for x in args]
@@ -334,9 +335,10 @@ def workload_eos(r, steps, warmup, cpu=True):
         ctx.free(p)
     per_launch_s = ampl_ms / max(ampl_n, 1) / 1e3
     launches = ampl_n / steps
+    assert launches == -(-nocc // chunk), (launches, chunk)
     res = dict(workload="eos", dims=dict(nvirt=nvirt, nocc=nocc, noccEOS=noccEOS, nvirtEOS=nvirtEOS, nba=nba),
                ms_per_call=t * 1e3, ms_per_call_events=eos_ms / steps, flops=F, tflops=F / t / 1e12,
-               amplitude=dict(launches_per_call=launches, ms_per_launch=per_launch_s * 1e3,
+               amplitude=dict(launches_per_call=launches, chunk=chunk, ms_per_launch=per_launch_s * 1e3,
                               tflops=Fa / launches / per_launch_s / 1e12,
                               frac_fp64_peak=Fa / launches / per_launch_s / 1e12 / PEAK["fp64_tflops"],
                               share_of_call=ampl_ms / max(eos_ms, 1e-9)),

@@ -476,18 +476,27 @@ def pair_flops(nocc, nvirt, naux):
     return 2.0 * naux * nvirt * nvirt * nocc * (nocc + 1) / 2.0
 
 
-def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
-    """Algorithmic FLOPs of rimp2_eos_arrays in its order of work: the
-    amplitude build once for both amplitude arrays, the transforms of each
-    array (the index that shrinks most first), and Steps 7-8 as C3^T C3.
-    Returns (total, amplitude build)."""
+def eos_chunk(nvirt, nocc, noccEOS, nvirtEOS):
+    """Occupied orbitals per amplitude slab chosen by rimp2_eos_arrays when no
+    chunk is set: the slab and its partial transforms within 8 GiB."""
+    per = (nvirt * nvirt * nocc + nvirt * nvirt * noccEOS + nvirtEOS * nvirt * nocc) * 8
+    return max(1, min(nocc, int((8 << 30) // per)))
+
+
+def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba, chunk):
+    """FLOPs of rimp2_eos_arrays.  Returns (algorithmic, amplitude GEMMs as
+    launched): the algorithmic count builds each unordered occupied pair once
+    (nocc (nocc + 1) / 2 pairs) and transforms it; the launched amplitude
+    GEMMs of a chunk of c orbitals starting at i0 cover c (nocc - i0) pairs."""
     nv, no, nE, vE = float(nvirt), float(nocc), float(noccEOS), float(nvirtEOS)
-    ampl = 2 * nba * nv * nv * no * no
-    tocc = 2 * nv * nv * no * no * nE + 2 * nv * nv * nE * no * nE + 4 * nv ** 3 * nE * nE
-    tvirt = 2 * vE * nv * nv * no * no + 2 * vE * vE * nv * no * no + 4 * vE * vE * no ** 3
+    npair = no * (no + 1) / 2
+    ampl = 2 * nba * nv * nv * npair
+    tocc = 2 * nv * nv * npair * nE + 2 * nv * nv * nE * no * nE + 4 * nv ** 3 * nE * nE
+    tvirt = 2 * vE * nv * nv * npair + 2 * vE * vE * nv * npair + 4 * vE * vE * no ** 3
     gocc = 2 * nba * nv * no * nE + 2 * nba * nv * nv * nE + 2 * (nv * nE) ** 2 * nba
     gvirt = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * (vE * no) ** 2 * nba
-    return ampl + tocc + tvirt + gocc + gvirt, ampl
+    launched = sum(2.0 * nba * nv * nv * (nocc - i0) * min(chunk, nocc - i0) for i0 in range(0, nocc, chunk))
+    return ampl + tocc + tvirt + gocc + gvirt, launched
 
 
 def metric_flops(nocc, nvirt, naux):

@@ -12,10 +12,12 @@
 //
 // Order of work here (not the reference's): both amplitude arrays share one
 // t, built chunk by chunk over I as a batched GEMM with the 1/Delta applied in
-// its epilogue (t never exists whole); from each chunk the occupied pair of
+// its epilogue (t never exists whole) and for each unordered occupied pair
+// once (t(A,B,J,I) = t(B,A,I,J)); from each chunk the occupied pair of
 // toccEOS and the virtual pair of tvirtEOS are contracted at once (the index
-// of each array that shrinks most goes first), and the remaining two indices
-// of each are transformed after the last chunk.  Every step is one launch of
+// of each array that shrinks most goes first), the remaining two indices of
+// each are transformed after the last chunk, and a symmetrization restores
+// the pairs J < I.  Every step is one launch of
 // the FP64 MFMA GEMM below with two-level batch strides, so the reorders of
 // toccB / tvirtB (RIMP2.F90:456-508) are just output strides.
 #include <hip/hip_runtime.h>
@@ -45,8 +47,9 @@ enum { EPI_PLAIN = 0, EPI_AMPL = 1 };
 
 // C[z] = alpha op(A[z]) op(B[z]) + beta C[z] for batch members z = z1 + Z1 z2,
 // member offsets z1 * s?1 + z2 * s?2 (elements).  EPI_AMPL: C(m, n) of member
-// z is the amplitude of occupied pair (I = i0 + z, J = n / nv), virtuals
-// (A = m, B = n % nv): acc / (e_I + e_J - e_B - e_A).
+// z is the weighted amplitude of occupied pair (I = i0 + z, J = j0 + n / nv),
+// virtuals (A = m, B = n % nv): w_IJ acc / (e_I + e_J - e_B - e_A) with
+// w_IJ = 1 (J > I), 1/2 (J = I), 0 (J < I): each unordered pair once.
 struct DgemmArgs {
   int64_t M, N, K;
   double alpha, beta;
@@ -61,20 +64,23 @@ struct DgemmArgs {
   int a_vec, b_vec;   // 16-byte loads allowed (even strides, aligned bases)
   const double* eo;
   const double* ev;
-  int64_t nv, i0;
+  int64_t nv, i0, j0;
 };
 
-// 256 threads = 4 waves in 2x2, wave tile 32x32 of 16x16 MFMA blocks
+// 256 threads = 4 waves in 2x2, wave tile (BM/2)x(BN/2) of 16x16 MFMA blocks
 // (v_mfma_f64_16x16x4f64; lane l holds A[l%16][l/16], B[l/16][l%16], result
 // d[r] = D[4r + l/16][l%16]).  K tiles of 16 go through LDS with the next
 // tile fetched into registers while the MFMAs run.  LDS layouts keep every
 // fragment read free of bank conflicts: a stored-untransposed A as As[k][m]
 // (pitch BM+16), a transposed A as As[m][k] (pitch 17); B likewise.
+// At least two waves per SIMD: the compiler then keeps the accumulators of
+// the larger tiles in VGPRs with no spills (measured 70 ms against 99 ms for
+// the one-wave build, 76 ms for 64x64, on the 1041/99/2173 amplitude slab).
 // Tiles are numbered row tile fastest and laid out so that each XCD (blocks
 // b with the same b % 8) runs one contiguous range of them: the row tiles
 // that share a column panel of op(B) share that XCD's L2.
 template <int BM, int BN, bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void dgemm_kernel(const DgemmArgs p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_kernel(const DgemmArgs p) {
   constexpr int BK = 16;
   constexpr int PA = TA ? BK + 1 : BM + 16;
   constexpr int PB = TB ? BN + 16 : BK + 1;
@@ -239,10 +245,11 @@ __global__ __launch_bounds__(256) void dgemm_kernel(const DgemmArgs p) {
     for (int j = 0; j < FN; j++) {
       const int64_t gn = n0 + wn + 16 * j + (lane & 15);
       if (gn >= N) continue;
-      double dn = 0.0;
+      double dn = 0.0, w = 1.0;
       if (EPI == EPI_AMPL) {
-        const int64_t J = gn / p.nv, Bv = gn - J * p.nv;
-        dn = (p.eo[p.i0 + z] + p.eo[J]) - p.ev[Bv];
+        const int64_t Jr = gn / p.nv, Bv = gn - Jr * p.nv, J = p.j0 + Jr, I = p.i0 + z;
+        dn = (p.eo[I] + p.eo[J]) - p.ev[Bv];
+        w = J > I ? 1.0 : (J == I ? 0.5 : 0.0);
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(256) void dgemm_kernel(const DgemmArgs p) {
         if (gm >= M) continue;
         double* q = C + gm + p.ldc * gn;
         if (EPI == EPI_AMPL) {
-          *q = acc[i][j][r] / (dn - p.ev[gm]);
+          *q = w == 0.0 ? 0.0 : w * (acc[i][j][r] / (dn - p.ev[gm]));
         } else {
           double v = p.alpha * acc[i][j][r];
           if (p.beta != 0.0) v += p.beta * *q;
@@ -259,8 +266,6 @@ __global__ __launch_bounds__(256) void dgemm_kernel(const DgemmArgs p) {
       }
     }
 }
-
-constexpr int kBM = 64, kBN = 64;
 
 // Host description of one batched GEMM (element strides).
 struct Gemm {
@@ -278,16 +283,44 @@ struct Gemm {
   bool ampl = false;
   const double* eo = nullptr;
   const double* ev = nullptr;
-  int64_t nv = 1, i0 = 0;
+  int64_t nv = 1, i0 = 0, j0 = 0;
 };
 
 bool even_aligned(const double* base, int64_t ld, int64_t s1, int64_t s2) {
   return (reinterpret_cast<uintptr_t>(base) & 15) == 0 && (ld & 1) == 0 && (s1 & 1) == 0 && (s2 & 1) == 0;
 }
 
-template <bool TA, bool TB, int EPI>
-void launch_one(hipStream_t s, const DgemmArgs& a, unsigned gx, unsigned gy) {
-  hipLaunchKernelGGL((dgemm_kernel<kBM, kBN, TA, TB, EPI>), dim3(gx, gy), dim3(256), 0, s, a);
+template <int BM, int BN>
+void launch_tiles(hipStream_t s, const Gemm& g, DgemmArgs a) {
+  a.tm = ceil_div(g.M, BM);
+  a.tiles = a.tm * ceil_div(g.N, BN);
+  const int64_t gx = round_up(a.tiles, 8);
+  // one dispatch counts blocks x 256 work-items in 32 bits; grid.y <= 65535
+  if (gx > kMaxGridBlocks) fail(RIMP2_ERR_ARG, "GEMM of %lld x %lld exceeds the launch limit", (long long)g.M,
+                                (long long)g.N);
+  const int64_t zmax = std::max<int64_t>(1, std::min<int64_t>(65535, kMaxGridBlocks / gx));
+  const int64_t Z = g.Z1 * g.Z2;
+  for (int64_t z0 = 0; z0 < Z; z0 += zmax) {
+    a.zbase = z0;
+    const dim3 grid((unsigned)gx, (unsigned)std::min<int64_t>(zmax, Z - z0));
+    if (g.ampl) hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, false, EPI_AMPL>), grid, dim3(256), 0, s, a);
+    else if (!g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, false, EPI_PLAIN>), grid, dim3(256), 0, s, a);
+    else if (g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, false, EPI_PLAIN>), grid, dim3(256), 0, s, a);
+    else if (!g.ta && g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, true, EPI_PLAIN>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, true, EPI_PLAIN>), grid, dim3(256), 0, s, a);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+}
+
+// Block tile of one GEMM.  An FP64 64x64 tile moves 8 flops per operand byte
+// through L2; 128x128 doubles that, and 96x128 (13.7) pads an awkward M less
+// (M = 1041: 1.4 % against 10.7 % at 128).  Small or skinny problems keep
+// 64x64 so that their grids stay large.
+enum TileShape { TILE_64x64, TILE_96x128, TILE_128x128 };
+TileShape pick_tile(const Gemm& g) {
+  if (g.N < 128 || g.M < 96 || g.K < 64) return TILE_64x64;
+  const double w128 = (double)round_up(g.M, 128) / (double)g.M, w96 = (double)round_up(g.M, 96) / (double)g.M;
+  return (w96 < w128 - 0.03) ? TILE_96x128 : TILE_128x128;
 }
 
 void run_gemm(hipStream_t s, const Gemm& g) {
@@ -311,29 +344,48 @@ void run_gemm(hipStream_t s, const Gemm& g) {
   a.sC1 = g.sC1;
   a.sC2 = g.sC2;
   a.Z1 = g.Z1;
-  a.tm = ceil_div(g.M, kBM);
-  a.tiles = a.tm * ceil_div(g.N, kBN);
+  a.zbase = 0;
   a.a_vec = even_aligned(g.A, g.lda, g.sA1, g.sA2);
   a.b_vec = even_aligned(g.B, g.ldb, g.sB1, g.sB2);
   a.eo = g.eo;
   a.ev = g.ev;
   a.nv = g.nv;
   a.i0 = g.i0;
-  const int64_t gx = round_up(a.tiles, 8);
-  // one dispatch counts blocks x 256 work-items in 32 bits; grid.y <= 65535
-  if (gx > kMaxGridBlocks) fail(RIMP2_ERR_ARG, "GEMM of %lld x %lld exceeds the launch limit", (long long)g.M,
-                                (long long)g.N);
-  const int64_t zmax = std::max<int64_t>(1, std::min<int64_t>(65535, kMaxGridBlocks / gx));
-  const int64_t Z = g.Z1 * g.Z2;
-  for (int64_t z0 = 0; z0 < Z; z0 += zmax) {
-    a.zbase = z0;
-    const unsigned gy = (unsigned)std::min<int64_t>(zmax, Z - z0);
-    if (g.ampl) launch_one<true, false, EPI_AMPL>(s, a, (unsigned)gx, gy);
-    else if (!g.ta && !g.tb) launch_one<false, false, EPI_PLAIN>(s, a, (unsigned)gx, gy);
-    else if (g.ta && !g.tb) launch_one<true, false, EPI_PLAIN>(s, a, (unsigned)gx, gy);
-    else if (!g.ta && g.tb) launch_one<false, true, EPI_PLAIN>(s, a, (unsigned)gx, gy);
-    else launch_one<true, true, EPI_PLAIN>(s, a, (unsigned)gx, gy);
-    RIMP2_HIP_CHECK(hipGetLastError());
+  a.j0 = g.j0;
+  switch (pick_tile(g)) {
+    case TILE_64x64: launch_tiles<64, 64>(s, g, a); break;
+    case TILE_96x128: launch_tiles<96, 128>(s, g, a); break;
+    case TILE_128x128: launch_tiles<128, 128>(s, g, a); break;
+  }
+}
+
+__global__ void zero_kernel(double* x, int64_t n) {
+  RIMP2_GRID_STRIDE(t, n) x[t] = 0.0;
+}
+
+// In place: x(Al,Il,Bl,Jl) <- x(Al,Il,Bl,Jl) + x(Bl,Jl,Al,Il) over an
+// [nv, no, nv, no] array (each pair of elements read and written by one thread).
+__global__ void sym_pairs_kernel(double* x, int64_t nv, int64_t no) {
+  const int64_t n1 = nv * no, n = n1 * n1;
+  RIMP2_GRID_STRIDE(t, n) {
+    const int64_t u = t % n1, v = t / n1;  // u = (Al, Il), v = (Bl, Jl)
+    const int64_t q = v + n1 * u;
+    if (t < q) {
+      const double a = x[t], b = x[q];
+      x[t] = a + b;
+      x[q] = a + b;
+    } else if (t == q) {
+      x[t] = 2.0 * x[t];
+    }
+  }
+}
+
+// y(Al,Bl,J,I) = x(Al,Bl,J,I) + x(Bl,Al,I,J) over [vE, vE, no, no] arrays.
+__global__ void sym_swap_kernel(const double* __restrict__ x, double* __restrict__ y, int64_t vE, int64_t no) {
+  const int64_t n = vE * vE * no * no;
+  RIMP2_GRID_STRIDE(t, n) {
+    const int64_t al = t % vE, bl = (t / vE) % vE, j = (t / (vE * vE)) % no, i = t / (vE * vE * no);
+    y[t] = x[t] + x[bl + vE * (al + vE * (i + no * j))];
   }
 }
 
@@ -410,28 +462,38 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
   ScopedTimer tm(c, RIMP2_K_EOS);
   const int64_t nv2 = nv * nv;
 
+  // Each unordered occupied pair once: t(A,B,J,I) = t(B,A,I,J), so a chunk
+  // of I builds J >= i0 only, weighted by w_IJ (1 above the diagonal, 1/2 on
+  // it, 0 below).  Any 4-index transform X of the weighted half satisfies
+  // full(Al,Bl,J,I) = X(Al,Bl,J,I) + X(Bl,Al,I,J), applied at the end.
+  {
+    const int64_t nR = vE * vE * no * no;
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_blocks(nR, 256)), dim3(256), 0, s, R, nR);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
   for (int64_t i0 = 0; i0 < no; i0 += ci) {
-    const int64_t cz = std::min(ci, no - i0);
-    {  // t(A, B, J, I) for I in [i0, i0 + cz):  C_I^T C / Delta   (RIMP2.F90:399-404)
+    const int64_t cz = std::min(ci, no - i0), nj = no - i0;
+    {  // t(A, B, J - i0, I - i0), J >= i0:  w C_I^T C_J / Delta   (RIMP2.F90:399-404)
       Gemm g;
       g.ampl = true;
       g.ta = true;
       g.M = nv;
-      g.N = nv * no;
+      g.N = nv * nj;
       g.K = nba;
       g.A = Cal + nba * nv * i0;
       g.lda = nba;
       g.sA1 = nba * nv;
-      g.B = Cal;
+      g.B = Cal + nba * nv * i0;
       g.ldb = nba;
       g.C = T;
       g.ldc = nv;
-      g.sC1 = nv2 * no;
+      g.sC1 = nv2 * nj;
       g.Z1 = cz;
       g.eo = eo;
       g.ev = ev;
       g.nv = nv;
       g.i0 = i0;
+      g.j0 = i0;
       ScopedTimer ta(c, RIMP2_K_EOS_AMPL);
       run_gemm(s, g);
     }
@@ -439,11 +501,11 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       Gemm g;
       g.M = nv2;
       g.N = nE;
-      g.K = no;
+      g.K = nj;
       g.A = T;
       g.lda = nv2;
-      g.sA1 = nv2 * no;
-      g.B = Uo5;
+      g.sA1 = nv2 * nj;
+      g.B = Uo5 + i0;
       g.ldb = no;
       g.C = P;
       g.ldc = nv2;
@@ -469,7 +531,7 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       Gemm g;
       g.ta = true;
       g.M = vE;
-      g.N = nv * no * cz;
+      g.N = nv * nj * cz;
       g.K = nv;
       g.A = Uv6;
       g.lda = nv;
@@ -487,12 +549,15 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.A = Q;
       g.lda = vE;
       g.sA1 = vE * nv;
+      g.sA2 = vE * nv * nj;
       g.B = Uv6;
       g.ldb = nv;
-      g.C = R + vE * vE * no * i0;
+      g.C = R + vE * vE * (i0 + no * i0);
       g.ldc = vE;
       g.sC1 = vE * vE;
-      g.Z1 = no * cz;
+      g.sC2 = vE * vE * no;
+      g.Z1 = nj;
+      g.Z2 = cz;
       run_gemm(s, g);
     }
   }
@@ -510,7 +575,7 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
     g.ldc = nv;
     run_gemm(s, g);
   }
-  {  // toccEOS(Al,Il,Bl,Jl) = sum_B S2(Al,B,Jl,Il) UvirtT(B,Bl)   (toccB reorder, RIMP2.F90:456-481)
+  {  // X(Al,Bl,Jl,Il) = sum_B S2(Al,B,Jl,Il) UvirtT(B,Bl), stored at toccEOS(Al,Il,Bl,Jl)
     Gemm g;
     g.M = nv;
     g.N = nv;
@@ -529,25 +594,35 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
     g.Z2 = nE;
     run_gemm(s, g);
   }
-  {  // tvirtEOS, occupied I: R2(Al,Bl,J,Il) = sum_I R(Al,Bl,J,I) UoccT(I,Il)
+  {  // toccEOS(Al,Il,Bl,Jl) = X(Al,Bl,Jl,Il) + X(Bl,Al,Il,Jl)   (toccB reorder, RIMP2.F90:456-481)
+    const int64_t n = nv2 * nE * nE;
+    hipLaunchKernelGGL(sym_pairs_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, s, toccEOS, nv, nE);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  {  // R2(Al,Bl,J,I) = R(Al,Bl,J,I) + R(Bl,Al,I,J): the full virtual transform of every ordered pair
+    const int64_t n = vE * vE * no * no;
+    hipLaunchKernelGGL(sym_swap_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, s, R, R2, vE, no);
+    RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  {  // tvirtEOS, occupied I: R(Al,Bl,J,Il) = sum_I R2(Al,Bl,J,I) UoccT(I,Il)
     Gemm g;
     g.M = vE * vE * no;
     g.N = no;
     g.K = no;
-    g.A = R;
+    g.A = R2;
     g.lda = vE * vE * no;
     g.B = Uo6;
     g.ldb = no;
-    g.C = R2;
+    g.C = R;
     g.ldc = vE * vE * no;
     run_gemm(s, g);
   }
-  {  // tvirtEOS(Al,Il,Bl,Jl) = sum_J R2(Al,Bl,J,Il) UoccT(J,Jl)   (tvirtB reorder, RIMP2.F90:483-508)
+  {  // tvirtEOS(Al,Il,Bl,Jl) = sum_J R(Al,Bl,J,Il) UoccT(J,Jl)   (tvirtB reorder, RIMP2.F90:483-508)
     Gemm g;
     g.M = vE;
     g.N = no;
     g.K = no;
-    g.A = R2;
+    g.A = R;
     g.lda = vE * vE;
     g.sA1 = vE;
     g.sA2 = vE * vE * no;
