@@ -10,6 +10,14 @@
 !>       header (nvirt,nocc,noccEOS,nvirtEOS,NBA,nocctot), Calpha(NBA,nvirt,nocc),
 !>       EVocc(nocc), EVvirt(nvirt).  Calpha is already fitted (B = Calpha) and
 !>       goes through rimp2_energy_fitted / rimp2_load_fitted, no metric.
+!>       When the file goes on with UoccEOST, UvirtT, UoccT, UvirtEOST
+!>       (RIMP2.F90:55-63) the local-basis arrays toccEOS, tvirtEOS, goccEOS,
+!>       gvirtEOS are formed on the GPU (Major Steps 5-8); when it then holds
+!>       the expected arrays as generate.F90:333-336 writes them (toccEOS,
+!>       tvirtEOS, gvirtEOS, goccEOS) every element is checked as
+!>       RIMP2.F90:253-323 does (|test - got| > 1e-10 stops with ERROR3).
+!>       The energy runs only when every denominator e_i + e_j - e_a - e_b is
+!>       negative (the reference's fixtures have them positive).
 !> mode: 0 fp64, 1 fp32, 2 bf16x3, 3 i8s4, 4 i8s5.
 !> Prints "E_MP2 <mode> <E>" and "dE_uEh <mode-vs-fp64>"; stops with
 !> 'ERROR3' when the energy is not finite (the reference's failure word,
@@ -26,9 +34,15 @@ program rimp2_driver
   real(C_DOUBLE), allocatable :: Pia(:,:,:), V(:,:), eo(:), ev(:), eps(:,:), eps64(:,:)
   real(C_DOUBLE) :: E, E64
   integer(C_SIZE_T) :: total, free
-  logical :: fitted
+  logical :: fitted, have_u, have_ref, energy_ok
+  integer(C_INT64_T) :: noccEOS, nvirtEOS
+  real(C_DOUBLE), allocatable :: UoccEOST(:,:), UvirtT(:,:), UoccT(:,:), UvirtEOST(:,:)
+  real(C_DOUBLE), allocatable :: toccEOS(:,:,:,:), tvirtEOS(:,:,:,:), goccEOS(:,:,:,:), gvirtEOS(:,:,:,:)
+  real(C_DOUBLE), allocatable :: tref(:,:,:,:), vref(:,:,:,:), gvref(:,:,:,:), goref(:,:,:,:)
 
   fitted = .false.
+  have_u = .false.
+  have_ref = .false.
   call get_command_argument(1, arg)
   if (trim(arg) == 'gen') then
      call get_command_argument(2, arg); read(arg, *) seed
@@ -60,6 +74,22 @@ program rimp2_driver
      if (ios /= 0) error stop 'ERROR3'
      read(lu, iostat=ios) ev
      if (ios /= 0) error stop 'ERROR3'
+     noccEOS = hdr(3); nvirtEOS = hdr(4)
+     allocate(UoccEOST(nocc, noccEOS), UvirtT(nvirt, nvirt), UoccT(nocc, nocc), UvirtEOST(nvirt, nvirtEOS))
+     read(lu, iostat=ios) UoccEOST
+     if (ios == 0) read(lu, iostat=ios) UvirtT
+     if (ios == 0) read(lu, iostat=ios) UoccT
+     if (ios == 0) read(lu, iostat=ios) UvirtEOST
+     have_u = (ios == 0)
+     if (have_u) then
+        allocate(tref(nvirt, noccEOS, nvirt, noccEOS), vref(nvirtEOS, nocc, nvirtEOS, nocc))
+        allocate(gvref(nvirtEOS, nocc, nvirtEOS, nocc), goref(nvirt, noccEOS, nvirt, noccEOS))
+        read(lu, iostat=ios) tref
+        if (ios == 0) read(lu, iostat=ios) vref
+        if (ios == 0) read(lu, iostat=ios) gvref
+        if (ios == 0) read(lu, iostat=ios) goref
+        have_ref = (ios == 0)
+     endif
      close(lu)
   else
      print *, 'usage: rimp2_driver.x gen <seed> <naux> <nvirt> <nocc> <mode> | file <path> <mode>'
@@ -73,6 +103,29 @@ program rimp2_driver
   call rimp2_check(stat, ctx, 'rimp2_get_dev_mem')
   print '(A,I0,A,I0)', 'device memory total ', total, ' free ', free
 
+  if (have_u) then
+     allocate(toccEOS(nvirt, noccEOS, nvirt, noccEOS), tvirtEOS(nvirtEOS, nocc, nvirtEOS, nocc))
+     allocate(goccEOS(nvirt, noccEOS, nvirt, noccEOS), gvirtEOS(nvirtEOS, nocc, nvirtEOS, nocc))
+     stat = rimp2_eos_arrays(ctx, nvirt, nocc, noccEOS, nvirtEOS, naux, Pia, eo, ev, UoccEOST, UvirtT, UoccT, &
+          & UvirtEOST, toccEOS, tvirtEOS, goccEOS, gvirtEOS)
+     call rimp2_check(stat, ctx, 'rimp2_eos_arrays')
+     print '(A,4ES25.16)', 'EOS_sums ', sum(toccEOS), sum(tvirtEOS), sum(goccEOS), sum(gvirtEOS)
+     if (have_ref) then
+        call verify('toccEOS', tref, toccEOS)
+        call verify('tvirtEOS', vref, tvirtEOS)
+        call verify('gvirtEOS', gvref, gvirtEOS)
+        call verify('goccEOS', goref, goccEOS)
+        print '(A)', 'EOS arrays match the file within 1e-10'
+     endif
+  endif
+
+  energy_ok = maxval(eo) < minval(ev)
+  if (.not. energy_ok) then
+     print '(A)', 'E_MP2 skipped: some denominator e_i + e_j - e_a - e_b >= 0'
+     stat = rimp2_ctx_destroy(ctx)
+     call rimp2_check(stat, C_NULL_PTR, 'rimp2_ctx_destroy')
+     stop
+  endif
   allocate(eps(nocc, nocc), eps64(nocc, nocc))
   if (fitted) then
      stat = rimp2_energy_fitted(ctx, naux, nvirt, nocc, Pia, eo, ev, RIMP2_MODE_FP64, eps64, E64)
@@ -95,4 +148,29 @@ program rimp2_driver
   print '(A,ES25.16)', 'eps_11 ', eps(1, 1)
   stat = rimp2_ctx_destroy(ctx)
   call rimp2_check(stat, C_NULL_PTR, 'rimp2_ctx_destroy')
+
+contains
+
+  !> The reference's element check (RIMP2.F90:257-271): the first element
+  !> with |test - got| > 1e-10 is printed and the run stops with ERROR3.
+  subroutine verify(what, test, got)
+    character(len=*), intent(in) :: what
+    real(C_DOUBLE), intent(in) :: test(:,:,:,:), got(:,:,:,:)
+    integer :: a, b, c, d
+    do d = 1, size(test, 4)
+       do c = 1, size(test, 3)
+          do b = 1, size(test, 2)
+             do a = 1, size(test, 1)
+                if (abs(test(a, b, c, d) - got(a, b, c, d)) > 1.0e-10_C_DOUBLE) then
+                   print *, 'Test(A,B,C,D)', test(a, b, c, d)
+                   print *, what, '(A,B,C,D)', got(a, b, c, d)
+                   print *, 'Error ', a, b, c, d, 'DIFF', abs(test(a, b, c, d) - got(a, b, c, d))
+                   error stop 'ERROR3'
+                endif
+             enddo
+          enddo
+       enddo
+    enddo
+  end subroutine verify
+
 end program rimp2_driver

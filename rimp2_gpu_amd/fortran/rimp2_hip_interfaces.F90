@@ -116,6 +116,27 @@ module rimp2_hip_interfaces
        real(C_DOUBLE) :: Pia(*), V(*), eps_o(*), eps_v(*)
      end function rimp2_gen_fragment
 
+     !> Major Steps 5-8 (RIMP2.F90:67-170) on the GPU, host arrays, blocking.
+     integer(C_INT) function rimp2_eos_arrays(ctx, nvirt, nocc, noccEOS, nvirtEOS, nba, Calpha, EVocc, EVvirt, &
+          & UoccEOST, UvirtT, UoccT, UvirtEOST, toccEOS, tvirtEOS, goccEOS, gvirtEOS) bind(C, name="rimp2_eos_arrays")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nvirt, nocc, noccEOS, nvirtEOS, nba
+       real(C_DOUBLE) :: Calpha(*), EVocc(*), EVvirt(*), UoccEOST(*), UvirtT(*), UoccT(*), UvirtEOST(*)
+       real(C_DOUBLE) :: toccEOS(*), tvirtEOS(*), goccEOS(*), gvirtEOS(*)
+     end function rimp2_eos_arrays
+
+     !> cublasDgemm_v2 as bound in gpu_interfaces.F90:24-32: device pointers,
+     !> asynchronous on the context stream, alpha/beta by reference.
+     integer(C_INT) function rimp2_dgemm(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc) &
+          & bind(C, name="rimp2_dgemm")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx, A, B, C
+       integer(C_INT), value :: transa, transb
+       integer(C_INT64_T), value :: m, n, k, lda, ldb, ldc
+       real(C_DOUBLE) :: alpha, beta
+     end function rimp2_dgemm
+
   end interface
 
 contains

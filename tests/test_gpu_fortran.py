@@ -2,10 +2,13 @@
 
 The driver reads a fragment file in the reference's record order
 (RIMP2.F90:45-63: header nvirt,nocc,noccEOS,nvirtEOS,NBA,nocctot as default
-integers; Calpha(NBA,nvirt,nocc); EVocc; EVvirt; then the U matrices, which
-the energy path does not read) or generates a seeded fragment, runs the FP64
-energy and one other precision mode, and prints both.  Files are written here
-as sequential unformatted records with 4-byte length markers.
+integers; Calpha(NBA,nvirt,nocc); EVocc; EVvirt; then the U matrices and,
+as generate.F90:333-336 writes them, the expected toccEOS, tvirtEOS,
+gvirtEOS, goccEOS) or generates a seeded fragment.  It forms the four
+local-basis arrays on the GPU and checks them element by element at the
+reference's 1e-10 (RIMP2.F90:253-323), then runs the FP64 energy and one other
+precision mode when the denominators allow it.  Files are written here as
+sequential unformatted records with 4-byte length markers.
 
 Parity: the FP64 energy against the oracle (metric transform + pair loops) to
 1e-12 relative; the other mode against the FP64 energy within its accuracy.
@@ -111,3 +114,69 @@ def test_reference_record_file(tmp_path, oracle, mode):
     assert abs(v["E_MP2_fp64"] - E_ref) <= 1e-12 * abs(E_ref)
     rel = {0: 0.0, 2: 2e-5, 4: 1e-9}[mode]
     assert abs(v["E"] - E_ref) <= rel * abs(E_ref) + 1e-12 * abs(E_ref)
+
+
+def reference_fixture_records(oracle, dims):
+    """A fragment file exactly as generate.F90 writes it (first_order off,
+    generate.F90:75-128, 333-336): header, the fixture inputs, and the four
+    arrays its CPU Steps 5-8 produce (here: the oracle's)."""
+    from test_oracle import ref_fixture
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    args = ref_fixture(*dims)
+    to, tv, go, gv = oracle.steps5to8(*args)
+    hdr = np.array([nvirt, nocc, noccEOS, nvirtEOS, nba, nocc], dtype=np.int32)
+    return [hdr, *args, to, tv, gv, go], (to, tv, go, gv)
+
+
+@needs_exe
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(74, 10, 4, 15, 273), (104, 20, 4, 15, 373)])
+def test_reference_fixture_file_steps5to8(tmp_path, oracle, dims):
+    recs, _ = reference_fixture_records(oracle, dims)
+    path = tmp_path / "Calpha2.data"
+    write_records(path, recs)
+    say("fortran steps 5-8 on the reference fixture", dims)
+    rc, out = run("file", path, 4)
+    assert rc == 0, out
+    assert "EOS arrays match the file within 1e-10" in out
+    assert "E_MP2 skipped" in out  # the fixture's denominators are positive
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_reference_fixture_file_mismatch_stops_with_error3(tmp_path, oracle):
+    dims = (74, 10, 4, 15, 273)
+    recs, _ = reference_fixture_records(oracle, dims)
+    tv = recs[9].copy(order="F")
+    tv[3, 2, 1, 4] += 5e-10  # beyond the reference's 1e-10
+    recs[9] = tv
+    path = tmp_path / "Calpha1.data"
+    write_records(path, recs)
+    say("fortran steps 5-8 mismatch")
+    rc, out = run("file", path, 0)
+    assert rc != 0 and "ERROR3" in out
+    assert "tvirtEOS" in out
+    assert "4 3 2 5" in " ".join(out.split())  # 1-based index of the changed element
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_record_file_eos_sums(tmp_path, oracle):
+    """Random inputs with U records but no expected arrays: the driver forms
+    the four arrays and prints their sums, checked against the oracle."""
+    rng = np.random.default_rng(5)
+    nvirt, nocc, noccEOS, nvirtEOS, naux = 40, 9, 3, 5, 160
+    C = np.asfortranarray(rng.standard_normal((naux, nvirt, nocc)) * 0.05)
+    eo = np.sort(rng.uniform(-2.0, -0.3, nocc))
+    ev = np.sort(rng.uniform(0.1, 3.0, nvirt))
+    U = [np.asfortranarray(rng.standard_normal(s)) for s in
+         ((nocc, noccEOS), (nvirt, nvirt), (nocc, nocc), (nvirt, nvirtEOS))]
+    write_records(tmp_path / "f.data", [np.array([nvirt, nocc, noccEOS, nvirtEOS, naux, nocc], dtype=np.int32),
+                                        C, eo, ev, *U])
+    rc, out = run("file", tmp_path / "f.data", 0)
+    assert rc == 0, out
+    line = [ln for ln in out.splitlines() if ln.startswith("EOS_sums")][0]
+    got = [float(x) for x in line.split()[1:]]
+    exp = oracle.steps5to8(C, eo, ev, *U)
+    for g, e in zip(got, exp):
+        assert abs(g - float(np.sum(e))) <= 1e-12 * float(np.sum(np.abs(e))), (g, float(np.sum(e)))
