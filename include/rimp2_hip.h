@@ -255,6 +255,25 @@ int rimp2_eos_arrays_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t nocc
                             const double* d_UoccEOST, const double* d_UvirtT, const double* d_UoccT,
                             const double* d_UvirtEOST, double* d_toccEOS, double* d_tvirtEOS, double* d_goccEOS,
                             double* d_gvirtEOS);
+/* Replaces the first-order block of the reference host (RIMP2.F90:172-251;
+ * the GPU build skips it, RIMP2.GPU.F90:100): from Calpha(NBA,nvirt,nocc),
+ * CalphaOcc(NBA,nocc,nocctot), CalphaVV(NBA,nvirt,nvirt) and the four U
+ * matrices forms djik(nvirt,noccEOS,noccEOS,nocctot) and
+ * blad(nvirtEOS,nocc,nvirtEOS,nvirt) (RIMP2_calc_gen4DimFO, RIMP2.F90:616-641)
+ * in FP64 on MFMA, with the reference's own dgemm arguments: the CalphaVV
+ * product has K = ldb = nocc (RIMP2.F90:231-235), so it sums the first nocc
+ * virtual columns of CalphaVV against UvirtT's storage read as an nocc x nvirt
+ * matrix.  The reference is in bounds only for nocctot = nocc and nocc <=
+ * nvirt; other sizes return RIMP2_ERR_ARG.  Blocking host form; _device is
+ * asynchronous on the context stream. */
+int rimp2_first_order(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                      int64_t nocctot, const double* Calpha, const double* CalphaOcc, const double* CalphaVV,
+                      const double* UoccEOST, const double* UvirtT, const double* UoccT, const double* UvirtEOST,
+                      double* djik, double* blad);
+int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS,
+                              int64_t nba, int64_t nocctot, const double* d_Calpha, const double* d_CalphaOcc,
+                              const double* d_CalphaVV, const double* d_UoccEOST, const double* d_UvirtT,
+                              const double* d_UoccT, const double* d_UvirtEOST, double* d_djik, double* d_blad);
 /* Occupied orbitals per amplitude chunk of rimp2_eos_arrays (0 = automatic:
  * the chunk's amplitude slab, nvirt^2 nocc doubles per orbital, and its
  * partial transforms stay within 8 GiB).  Bounds the work memory; has no

@@ -16,8 +16,13 @@ Parity pins (tests/test_oracle.py):
          s^4 scaling, orbital-permutation invariance, eps_ij <= 0.
   cholesky / trsm -> scipy.linalg (LAPACK dpotrf / dtrtrs), L L^T = V,
          B^T B = X^T V^-1 X, V = I gives B = X.
-  gen4dimfo -> numpy einsum (no reference fixture is reachable: parity of the
-         first-order K=nocc quirk is not exercised by this path).
+  gen4dimfo -> numpy einsum.
+  first_order -> an independent numpy einsum that spells the reference's
+         K = ldb = nocc product (RIMP2.F90:231-235) out as a reinterpretation
+         of UvirtT's storage, closed forms on constant inputs, and a check
+         that the quirk changes the result (so a "corrected" call fails).  The
+         reference's generator writes no first-order records
+         (generate.F90:130), so no reference fixture reaches this block.
   mlp_* -> finite differences and a closed-form GELU (erf form).
 """
 import ctypes
@@ -57,6 +62,7 @@ def lib():
             "oracle_gocc": [ci, ci, ci, _dp, _dp],
             "oracle_gvirt": [ci, ci, ci, _dp, _dp],
             "oracle_gen4dimfo": [ci, _dp, ci, ci, _dp, ci, ci, _dp],
+            "oracle_first_order": [ci, ci, ci, ci, ci, ci] + [_dp] * 9,
             "oracle_steps5to8": [ci, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
             "oracle_pair_energies": [_i64, _i64, _i64, _dp, _dp, _dp, _dp, _dp],
             "oracle_pair_energies_subset": [_i64, _i64, _dp, _dp, _dp, _i64, ctypes.POINTER(_i64),
@@ -180,6 +186,22 @@ def steps5to8(Calpha, EVocc, EVvirt, UoccEOST, UvirtT, UoccT, UvirtEOST):
     lib().oracle_steps5to8(nvirt, nocc, noccEOS, nvirtEOS, nba, *[_p(a) for a in args], _p(toccEOS), _p(tvirtEOS),
                            _p(goccEOS), _p(gvirtEOS))
     return toccEOS, tvirtEOS, goccEOS, gvirtEOS
+
+
+def first_order(Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST):
+    """RIMP2.F90:172-251: returns (djik, blad) with the reference's own call
+    arguments (nocctot = nocc required)."""
+    nba, nvirt, nocc = Calpha.shape
+    nocctot = CalphaOcc.shape[2]
+    if nocctot != nocc:
+        raise ValueError("the reference's first-order block is in bounds only for nocctot == nocc")
+    noccEOS, nvirtEOS = UoccEOST.shape[1], UvirtEOST.shape[1]
+    args = [_F(x) for x in (Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST)]
+    djik = np.zeros((nvirt, noccEOS, noccEOS, nocctot), order="F")
+    blad = np.zeros((nvirtEOS, nocc, nvirtEOS, nvirt), order="F")
+    lib().oracle_first_order(nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, *[_p(a) for a in args], _p(djik),
+                             _p(blad))
+    return djik, blad
 
 
 def pair_energies(B, eps_o, eps_v):

@@ -16,6 +16,7 @@
  *   RIMP2.F90:589-614  RIMP2_calc_gvirt   -> oracle_gvirt
  *   RIMP2.F90:616-641  RIMP2_calc_gen4DimFO -> oracle_gen4dimfo
  *   RIMP2.F90:67-170   Major Steps 5-8    -> oracle_steps5to8
+ *   RIMP2.F90:172-251  first-order djik / blad -> oracle_first_order
  * plus the canonical RI-MP2 pair energy built from toccA (t = g/Delta,
  * Delta = e_I+e_J-e_B-e_A, RIMP2.F90:399-404) and gocc (g = sum_alpha C C,
  * RIMP2.F90:577-581) with U = identity:
@@ -255,6 +256,53 @@ void oracle_steps5to8(int nvirt, int nocc, int noccEOS, int nvirtEOS, int nba, c
   oracle_gvirt(nvirtEOS, nocc, nba, c3, gvirtEOS);
   free(c2);
   free(c3);
+}
+
+/* RIMP2.F90:172-251: the first-order property intermediates.
+ *   djik(nvirt,noccEOS,noccEOS,nocctot) and blad(nvirtEOS,nocc,nvirtEOS,nvirt)
+ * from Calpha(NBA,nvirt,nocc), CalphaOcc(NBA,nocc,nocctot) and
+ * CalphaVV(NBA,nvirt,nvirt), every call with the reference's own arguments:
+ * RIMP2.F90:198 multiplies CalphaOcc by UoccT as an nocctot x nocctot matrix
+ * (UoccT is nocc x nocc, so only nocctot = nocc is in bounds; the caller
+ * guarantees it), and RIMP2.F90:231-235 multiplies CalphaVV by UvirtT with
+ * K = ldb = nocc, i.e. UvirtT's storage read as an nocc x nvirt matrix and
+ * only the first nocc virtual columns of CalphaVV summed. */
+void oracle_first_order(int nvirt, int nocc, int noccEOS, int nvirtEOS, int nba, int nocctot, const double* Calpha,
+                        const double* CalphaOcc, const double* CalphaVV, const double* UoccEOST, const double* UvirtT,
+                        const double* UoccT, const double* UvirtEOST, double* djik, double* blad) {
+  size_t nv = nvirt, no = nocc, noE = noccEOS, nvE = nvirtEOS, nt = nocctot, q = nba;
+  /* djik: RIMP2.F90:178-210 */
+  double* c2 = malloc(sizeof(double) * q * nv * noE);
+  oracle_dgemm('N', 'N', (int64_t)nba * nvirt, noccEOS, nocc, 1.0, Calpha, (int64_t)nba * nvirt, UoccEOST, nocc, 0.0,
+               c2, (int64_t)nba * nvirt);
+  double* c3 = malloc(sizeof(double) * q * nv * noE);
+  oracle_transalpha2(noccEOS, nvirt, nvirt, nba, UvirtT, c2, c3);
+  free(c2);
+  double* o2 = malloc(sizeof(double) * q * no * nt);
+  oracle_dgemm('N', 'N', (int64_t)nba * nocc, nocctot, nocctot, 1.0, CalphaOcc, (int64_t)nba * nocc, UoccT, nocctot,
+               0.0, o2, (int64_t)nba * nocc);
+  double* c4 = malloc(sizeof(double) * q * noE * nt);
+  oracle_transalpha2(nocctot, nocc, noccEOS, nba, UoccEOST, o2, c4);
+  free(o2);
+  oracle_gen4dimfo(nba, c3, nvirt, noccEOS, c4, noccEOS, nocctot, djik);
+  free(c3);
+  free(c4);
+  /* blad: RIMP2.F90:216-250 */
+  double* b2 = malloc(sizeof(double) * q * nv * no);
+  oracle_dgemm('N', 'N', (int64_t)nba * nvirt, nocc, nocc, 1.0, Calpha, (int64_t)nba * nvirt, UoccT, nocc, 0.0, b2,
+               (int64_t)nba * nvirt);
+  double* b3 = malloc(sizeof(double) * q * nvE * no);
+  oracle_transalpha2(nocc, nvirt, nvirtEOS, nba, UvirtEOST, b2, b3);
+  free(b2);
+  double* v2 = malloc(sizeof(double) * q * nv * nv);
+  oracle_dgemm('N', 'N', (int64_t)nba * nvirt, nvirt, nocc, 1.0, CalphaVV, (int64_t)nba * nvirt, UvirtT, nocc, 0.0,
+               v2, (int64_t)nba * nvirt);
+  double* b4 = malloc(sizeof(double) * q * nvE * nv);
+  oracle_transalpha2(nvirt, nvirt, nvirtEOS, nba, UvirtEOST, v2, b4);
+  free(v2);
+  oracle_gen4dimfo(nba, b3, nvirtEOS, nocc, b4, nvirtEOS, nvirt, blad);
+  free(b3);
+  free(b4);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -99,6 +99,8 @@ SIGNATURES = {
     "rimp2_eos_arrays": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_dp] * 11),
     "rimp2_eos_arrays_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_vp] * 11),
     "rimp2_eos_set_chunk": (ctypes.c_int, [_vp, _i64]),
+    "rimp2_first_order": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64, _i64] + [_dp] * 9),
+    "rimp2_first_order_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64, _i64] + [_vp] * 9),
 }
 
 
@@ -401,6 +403,26 @@ class Context:
         """d_in: 7 device pointers (Calpha, EVocc, EVvirt, UoccEOST, UvirtT,
         UoccT, UvirtEOST); d_out: 4 (toccEOS, tvirtEOS, goccEOS, gvirtEOS)."""
         self._chk(lib().rimp2_eos_arrays_device(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, *d_in, *d_out))
+
+    def first_order(self, Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST):
+        """First-order intermediates (RIMP2.F90:172-251) on the GPU: returns
+        (djik, blad) as Fortran-ordered arrays of the reference's shapes."""
+        C, Co, Cvv = _f64(Calpha), _f64(CalphaOcc), _f64(CalphaVV)
+        nba, nvirt, nocc = C.shape
+        nocctot = Co.shape[2]
+        UoE, Uv, Uo, UvE = _f64(UoccEOST), _f64(UvirtT), _f64(UoccT), _f64(UvirtEOST)
+        noccEOS, nvirtEOS = UoE.shape[1], UvE.shape[1]
+        if Co.shape[:2] != (nba, nocc) or Cvv.shape != (nba, nvirt, nvirt):
+            raise ValueError("CalphaOcc must be (NBA, nocc, nocctot) and CalphaVV (NBA, nvirt, nvirt)")
+        if (UoE.shape[0] != nocc or Uv.shape != (nvirt, nvirt) or Uo.shape != (nocc, nocc)
+                or UvE.shape[0] != nvirt):
+            raise ValueError("U matrix shapes do not match (nocc, nvirt)")
+        djik = np.zeros((nvirt, noccEOS, noccEOS, nocctot), order="F")
+        blad = np.zeros((nvirtEOS, nocc, nvirtEOS, nvirt), order="F")
+        self._chk(lib().rimp2_first_order(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, _ptr(C), _ptr(Co),
+                                          _ptr(Cvv), _ptr(UoE), _ptr(Uv), _ptr(Uo), _ptr(UvE), _ptr(djik),
+                                          _ptr(blad)))
+        return djik, blad
 
     def eos_set_chunk(self, nocc_per_chunk):
         self._chk(lib().rimp2_eos_set_chunk(self.h, int(nocc_per_chunk)))

@@ -29,13 +29,14 @@
 namespace rimp2 {
 
 struct EosState {
-  DevBuf in, work, out;
+  DevBuf in, work, out, fo;
 };
 void eos_destroy(EosState* s) {
   if (!s) return;
   s->in.release();
   s->work.release();
   s->out.release();
+  s->fo.release();
   delete s;
 }
 
@@ -690,6 +691,66 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
   }
 }
 
+// First-order intermediates (RIMP2.F90:172-251), with the reference's own
+// call arguments: UoccT multiplies CalphaOcc as an nocctot x nocctot matrix
+// (nocctot = nocc, checked by the caller), and CalphaVV is multiplied by
+// UvirtT with K = ldb = nocc (RIMP2.F90:231-235): the first nocc virtual
+// columns of CalphaVV against UvirtT's storage read as an nocc x nvirt matrix.
+//   djik(a,Il,Jl,K) = sum_alpha C3(alpha,a,Il) C4(alpha,Jl,K)   (RIMP2_calc_gen4DimFO, RIMP2.F90:616-641)
+//   blad(Al,J,Bl,b) = sum_alpha B3(alpha,Al,J) B4(alpha,Bl,b)
+void first_order_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, const double* Cal,
+                        const double* Cocc, const double* Cvv, const double* Uo5, const double* Uv5,
+                        const double* Uo6, const double* Uv6, double* djik, double* blad) {
+  if (!c->eos) c->eos = new EosState();
+  const int64_t nt = no;
+  const int64_t n_c2 = nba * nv * std::max(nE, no), n_c3 = nba * std::max(nv * nE, vE * no);
+  const int64_t n_o2 = nba * no * nt, n_c4 = nba * nE * nt, n_v2 = nba * nv * nv, n_b4 = nba * vE * nv;
+  const size_t o_c3 = align256((size_t)n_c2 * 8), o_o2 = o_c3 + align256((size_t)n_c3 * 8),
+               o_c4 = o_o2 + align256((size_t)n_o2 * 8), o_v2 = o_c4 + align256((size_t)n_c4 * 8),
+               o_b4 = o_v2 + align256((size_t)n_v2 * 8), total = o_b4 + align256((size_t)n_b4 * 8);
+  unsigned char* w = static_cast<unsigned char*>(c->eos->fo.ensure(total));
+  double* c2 = reinterpret_cast<double*>(w);
+  double* c3 = reinterpret_cast<double*>(w + o_c3);
+  double* o2 = reinterpret_cast<double*>(w + o_o2);
+  double* c4 = reinterpret_cast<double*>(w + o_c4);
+  double* v2 = reinterpret_cast<double*>(w + o_v2);
+  double* b4 = reinterpret_cast<double*>(w + o_b4);
+  hipStream_t s = c->stream;
+  ScopedTimer tm(c, RIMP2_K_EOS);
+  auto gemm = [&](bool ta, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sA1,
+                  const double* B, int64_t ldb, double* C, int64_t ldc, int64_t sC1, int64_t Z1) {
+    Gemm g;
+    g.ta = ta;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.A = A;
+    g.lda = lda;
+    g.sA1 = sA1;
+    g.B = B;
+    g.ldb = ldb;
+    g.C = C;
+    g.ldc = ldc;
+    g.sC1 = sC1;
+    g.Z1 = Z1;
+    run_gemm(s, g);
+  };
+  // djik: C3(alpha,Bl,Jl) = sum_B UvirtT(B,Bl) sum_J Calpha(alpha,B,J) UoccEOST(J,Jl)   (RIMP2.F90:181-187)
+  gemm(false, nba * nv, nE, no, Cal, nba * nv, 0, Uo5, no, c2, nba * nv, 0, 1);
+  gemm(false, nba, nv, nv, c2, nba, nba * nv, Uv5, nv, c3, nba, nba * nv, nE);
+  // C4(alpha,Il,K) = sum_i UoccEOST(i,Il) sum_k CalphaOcc(alpha,i,k) UoccT(k,K)   (RIMP2.F90:193-205)
+  gemm(false, nba * no, nt, nt, Cocc, nba * no, 0, Uo6, nt, o2, nba * no, 0, 1);
+  gemm(false, nba, nE, no, o2, nba, nba * no, Uo5, no, c4, nba, nba * nE, nt);
+  gemm(true, nv * nE, nE * nt, nba, c3, nba, 0, c4, nba, djik, nv * nE, 0, 1);
+  // blad: B3(alpha,Al,J) = sum_B UvirtEOST(B,Al) sum_j Calpha(alpha,B,j) UoccT(j,J)   (RIMP2.F90:220-228)
+  gemm(false, nba * nv, no, no, Cal, nba * nv, 0, Uo6, no, c2, nba * nv, 0, 1);
+  gemm(false, nba, vE, nv, c2, nba, nba * nv, Uv6, nv, c3, nba, nba * vE, no);
+  // B4(alpha,Al,b) = sum_a UvirtEOST(a,Al) sum_{k < nocc} CalphaVV(alpha,a,k) UvirtT[k + nocc b]   (RIMP2.F90:231-244)
+  gemm(false, nba * nv, nv, no, Cvv, nba * nv, 0, Uv5, no, v2, nba * nv, 0, 1);
+  gemm(false, nba, vE, nv, v2, nba, nba * nv, Uv6, nv, b4, nba, nba * vE, nv);
+  gemm(true, vE * no, vE * nv, nba, c3, nba, 0, b4, nba, blad, vE * no, 0, 1);
+}
+
 void dgemm_device(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
                   int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
   Gemm g;
@@ -730,6 +791,18 @@ void eos_validate(int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS
   if (noccEOS > nocc || nvirtEOS > nvirt)
     fail(RIMP2_ERR_ARG, "noccEOS=%lld > nocc=%lld or nvirtEOS=%lld > nvirt=%lld", (long long)noccEOS,
          (long long)nocc, (long long)nvirtEOS, (long long)nvirt);
+}
+
+void first_order_validate(int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                          int64_t nocctot) {
+  eos_validate(nvirt, nocc, noccEOS, nvirtEOS, nba);
+  if (nocctot != nocc)
+    fail(RIMP2_ERR_ARG, "nocctot=%lld != nocc=%lld: the reference multiplies CalphaOcc by UoccT(nocc,nocc) as an "
+         "nocctot x nocctot matrix (RIMP2.F90:195-198), in bounds only for nocctot = nocc",
+         (long long)nocctot, (long long)nocc);
+  if (nocc > nvirt)
+    fail(RIMP2_ERR_ARG, "nocc=%lld > nvirt=%lld: the reference's CalphaVV product (K = ldb = nocc, "
+         "RIMP2.F90:231-235) reads past UvirtT", (long long)nocc, (long long)nvirt);
 }
 
 }  // namespace
@@ -808,6 +881,62 @@ int rimp2_eos_arrays(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, in
   for (int i = 0; i < 4; i++) o[i] = reinterpret_cast<double*>(dout + offo[i]);
   eos_arrays_device(c, nv, no, nE, vE, nba, d[0], d[1], d[2], d[3], d[4], d[5], d[6], o[0], o[1], o[2], o[3], c->eos_chunk);
   for (int i = 0; i < 4; i++)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(hout[i], o[i], (size_t)nout[i] * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS,
+                              int64_t nba, int64_t nocctot, const double* d_Calpha, const double* d_CalphaOcc,
+                              const double* d_CalphaVV, const double* d_UoccEOST, const double* d_UvirtT,
+                              const double* d_UoccT, const double* d_UvirtEOST, double* d_djik, double* d_blad) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  first_order_validate(nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot);
+  if (!d_Calpha || !d_CalphaOcc || !d_CalphaVV || !d_UoccEOST || !d_UvirtT || !d_UoccT || !d_UvirtEOST || !d_djik ||
+      !d_blad)
+    fail(RIMP2_ERR_ARG, "null device pointer");
+  first_order_device(c, nvirt, nocc, noccEOS, nvirtEOS, nba, d_Calpha, d_CalphaOcc, d_CalphaVV, d_UoccEOST, d_UvirtT,
+                     d_UoccT, d_UvirtEOST, d_djik, d_blad);
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_first_order(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, int64_t nvirtEOS, int64_t nba,
+                      int64_t nocctot, const double* Calpha, const double* CalphaOcc, const double* CalphaVV,
+                      const double* UoccEOST, const double* UvirtT, const double* UoccT, const double* UvirtEOST,
+                      double* djik, double* blad) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  first_order_validate(nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot);
+  if (!Calpha || !CalphaOcc || !CalphaVV || !UoccEOST || !UvirtT || !UoccT || !UvirtEOST || !djik || !blad)
+    fail(RIMP2_ERR_ARG, "null pointer");
+  if (!c->eos) c->eos = new EosState();
+  const int64_t nv = nvirt, no = nocc, nE = noccEOS, vE = nvirtEOS;
+  const int64_t nin[7] = {nba * nv * no, nba * no * nocctot, nba * nv * nv, no * nE, nv * nv, no * no, nv * vE};
+  const double* hin[7] = {Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST};
+  const int64_t nout[2] = {nv * nE * nE * nocctot, vE * no * vE * nv};
+  double* hout[2] = {djik, blad};
+  size_t offi[7], offo[2], bi = 0, bo = 0;
+  for (int i = 0; i < 7; i++) {
+    offi[i] = bi;
+    bi += align256((size_t)nin[i] * sizeof(double));
+  }
+  for (int i = 0; i < 2; i++) {
+    offo[i] = bo;
+    bo += align256((size_t)nout[i] * sizeof(double));
+  }
+  unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
+  unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
+  for (int i = 0; i < 7; i++)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(din + offi[i], hin[i], (size_t)nin[i] * sizeof(double), hipMemcpyHostToDevice,
+                                   c->stream));
+  const double* d[7];
+  for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
+  double* o[2];
+  for (int i = 0; i < 2; i++) o[i] = reinterpret_cast<double*>(dout + offo[i]);
+  first_order_device(c, nv, no, nE, vE, nba, d[0], d[1], d[2], d[3], d[4], d[5], d[6], o[0], o[1]);
+  for (int i = 0; i < 2; i++)
     RIMP2_HIP_CHECK(hipMemcpyAsync(hout[i], o[i], (size_t)nout[i] * sizeof(double), hipMemcpyDeviceToHost,
                                    c->stream));
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -11,7 +11,7 @@ largest |A||B| product sum.
 import numpy as np
 import pytest
 
-from test_oracle import FIXTURES, ref_fixture
+from test_oracle import FIXTURES, first_order_inputs, ref_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -114,6 +114,39 @@ def test_eos_rejects_bad_dimensions(ctx, rimp2):
     assert e.value.code == rimp2.ERR_ARG
     with pytest.raises(rimp2.Rimp2Error) as e:
         ctx.eos_set_chunk(-1)
+    assert e.value.code == rimp2.ERR_ARG
+
+
+# ---------------------------------------------------------------------------
+# first-order djik / blad (RIMP2.F90:172-251) against the oracle, which keeps
+# the reference's K = ldb = nocc CalphaVV product (RIMP2.F90:231-235)
+# ---------------------------------------------------------------------------
+FO_SHAPES = [(74, 10, 4, 15, 273), (104, 20, 4, 15, 373), (1, 1, 1, 1, 1), (65, 33, 2, 64, 17)]
+
+
+@pytest.mark.parametrize("dims", FO_SHAPES)
+def test_first_order_vs_oracle(ctx, oracle, dims):
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    say(f"first order {dims}")
+    rng = np.random.default_rng(41 + sum(dims))
+    args = [0.1 * a if a.ndim == 3 else a for a in first_order_inputs(rng, nba, nvirt, nocc, noccEOS, nvirtEOS)]
+    got = ctx.first_order(*args)
+    exp = oracle.first_order(*args)
+    for name, g, e in zip(("djik", "blad"), got, exp):
+        err, scale = float(np.max(np.abs(g - e))), float(np.max(np.abs(e)))
+        say(f"  {name}: max|d| = {err:.3e} (max|ref| {scale:.3e})")
+        assert err <= 1e-12 * scale + 1e-14
+
+
+def test_first_order_rejects_out_of_bounds_shapes(ctx, rimp2):
+    rng = np.random.default_rng(2)
+    C, Co, Cvv, UoE, Uv, Uo, UvE = first_order_inputs(rng, 5, 6, 3, 2, 4)
+    with pytest.raises(rimp2.Rimp2Error) as e:  # nocctot != nocc
+        ctx.first_order(C, np.asfortranarray(np.ones((5, 3, 4))), Cvv, UoE, Uv, Uo, UvE)
+    assert e.value.code == rimp2.ERR_ARG
+    C, Co, Cvv, UoE, Uv, Uo, UvE = first_order_inputs(rng, 5, 3, 4, 2, 2)  # nocc 4 > nvirt 3
+    with pytest.raises(rimp2.Rimp2Error) as e:
+        ctx.first_order(C, Co, Cvv, UoE, Uv, Uo, UvE)
     assert e.value.code == rimp2.ERR_ARG
 
 

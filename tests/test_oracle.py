@@ -132,6 +132,70 @@ def test_gen4dimfo_einsum(oracle):
 
 
 # ---------------------------------------------------------------------------
+# first-order intermediates djik / blad (RIMP2.F90:172-251)
+# ---------------------------------------------------------------------------
+def numpy_first_order(C, Co, Cvv, UoE, Uv, Uo, UvE):
+    """Independent restatement.  RIMP2.F90:231-235 calls dgemm with K = ldb =
+    nocc on CalphaVV(NBA,nvirt,nvirt) and UvirtT(nvirt,nvirt): only the first
+    nocc virtual columns of CalphaVV are summed, against UvirtT's storage read
+    as an nocc x nvirt matrix."""
+    nba, nvirt, nocc = C.shape
+    C3 = np.einsum("qbj,jJ,bB->qBJ", C, UoE, Uv)
+    C4 = np.einsum("qik,kK,iI->qIK", Co, Uo, UoE)
+    djik = np.einsum("qab,qcd->abcd", C3, C4)
+    B3 = np.einsum("qbj,jJ,bB->qBJ", C, Uo, UvE)
+    Uq = Uv.ravel(order="F")[: nocc * nvirt].reshape((nocc, nvirt), order="F")
+    B4 = np.einsum("qak,kb,aA->qAb", Cvv[:, :, :nocc], Uq, UvE)
+    blad = np.einsum("qab,qcd->abcd", B3, B4)
+    return djik, blad
+
+
+def first_order_inputs(rng, nba, nvirt, nocc, noccEOS, nvirtEOS):
+    C = np.asfortranarray(rng.standard_normal((nba, nvirt, nocc)))
+    Co = np.asfortranarray(rng.standard_normal((nba, nocc, nocc)))
+    Cvv = np.asfortranarray(rng.standard_normal((nba, nvirt, nvirt)))
+    U = [np.asfortranarray(rng.standard_normal(s)) for s in
+         ((nocc, noccEOS), (nvirt, nvirt), (nocc, nocc), (nvirt, nvirtEOS))]
+    return C, Co, Cvv, *U
+
+
+def test_first_order_random_inputs(oracle):
+    args = first_order_inputs(np.random.default_rng(23), 7, 9, 4, 3, 5)
+    got = oracle.first_order(*args)
+    exp = numpy_first_order(*args)
+    for g, e in zip(got, exp):
+        assert g.shape == e.shape
+        np.testing.assert_allclose(g, e, rtol=1e-12, atol=1e-12)
+
+
+def test_first_order_closed_form(oracle):
+    nvirt, nocc, noccEOS, nvirtEOS, nba = 11, 5, 2, 3, 13
+    Cal, _, _, UoE, Uv, Uo, UvE = ref_fixture(nvirt, nocc, noccEOS, nvirtEOS, nba)[:7]
+    cO, cV = 0.0173, 0.0291
+    Co = np.full((nba, nocc, nocc), cO, order="F")
+    Cvv = np.full((nba, nvirt, nvirt), cV, order="F")
+    djik, blad = oracle.first_order(Cal, Co, Cvv, UoE, Uv, Uo, UvE)
+    c = 0.00124235636 + 0.01010101 + 0.014321
+    uoE, uv, uo, uvE = 0.030321 + 0.101121, 0.021321 + 0.01321, 0.032321 + 0.01521, 0.023321 + 0.01421
+    d_val = nba * (c * nocc * uoE * nvirt * uv) * (cO * nocc * uo * nocc * uoE)
+    # K = nocc in the CalphaVV product: nocc terms, not nvirt
+    b_val = nba * (c * nocc * uo * nvirt * uvE) * (nvirt * uvE * cV * nocc * uv)
+    assert np.max(np.abs(djik - d_val)) <= 1e-13 * d_val
+    assert np.max(np.abs(blad - b_val)) <= 1e-13 * b_val
+
+
+def test_first_order_quirk_is_kept(oracle):
+    """A dgemm with K = nvirt (the evident intent) gives a different blad."""
+    args = first_order_inputs(np.random.default_rng(29), 6, 8, 3, 2, 4)
+    C, Co, Cvv, UoE, Uv, Uo, UvE = args
+    _, blad = oracle.first_order(*args)
+    B3 = np.einsum("qbj,jJ,bB->qBJ", C, Uo, UvE)
+    B4 = np.einsum("qak,kb,aA->qAb", Cvv, Uv, UvE)
+    intended = np.einsum("qab,qcd->abcd", B3, B4)
+    assert np.max(np.abs(blad - intended)) > 1e-3 * np.max(np.abs(intended))
+
+
+# ---------------------------------------------------------------------------
 # pair energies
 # ---------------------------------------------------------------------------
 def brute_energy(B, eo, ev):
