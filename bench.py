@@ -331,7 +331,7 @@ def workload_eos(r, steps, warmup, cpu=True):
     ctx.timing(False)
     outs = [ctx.from_device(p, s) for p, s in zip(d_out, shapes)]
     rel = [float(np.max(np.abs(o - e)) / abs(e)) for o, e in zip(outs, expect)]
-    for p in d_in + d_out:
+    for p in d_out:
         ctx.free(p)
     per_launch_s = ampl_ms / max(ampl_n, 1) / 1e3
     launches = ampl_n / steps
@@ -345,6 +345,27 @@ def workload_eos(r, steps, warmup, cpu=True):
                max_rel_err_vs_closed_form=max(rel))
     log(f"[bench] eos: {t * 1e3:.2f} ms/call, {F / t / 1e12:.2f} TFLOP/s; amplitude GEMM "
         f"{per_launch_s * 1e3:.3f} ms x {launches:.0f}, {res['amplitude']['tflops']:.2f} TFLOP/s; rel err {rel}")
+    # first-order intermediates at the same size (RIMP2.F90:172-251); the
+    # extra 3-index inputs CalphaOcc(NBA,nocc,nocc) and CalphaVV(NBA,nvirt,nvirt)
+    # come from the device generator (timing only: parity is in the tests)
+    d_co, d_cvv = ctx.alloc(nba * nocc * nocc * 8), ctx.alloc(nba * nvirt * nvirt * 8)
+    ctx.gen_device(11, nba, nocc, nocc, d_co, None, None, None)
+    ctx.gen_device(12, nba, nvirt, nvirt, d_cvv, None, None, None)
+    d_dj, d_bl = ctx.alloc(nvirt * noccEOS * noccEOS * nocc * 8), ctx.alloc(nvirtEOS * nocc * nvirtEOS * nvirt * 8)
+    fo_args = (nvirt, nocc, noccEOS, nvirtEOS, nba, nocc, d_in[0], d_co, d_cvv, d_in[3], d_in[4], d_in[5], d_in[6],
+               d_dj, d_bl)
+    ctx.first_order_device(*fo_args)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.first_order_device(*fo_args)
+    ctx.sync()
+    t_fo = (time.perf_counter() - t0) / steps
+    for p in (d_co, d_cvv, d_dj, d_bl):
+        ctx.free(p)
+    F_fo = r.first_order_flops(nvirt, nocc, noccEOS, nvirtEOS, nba)
+    res["first_order"] = dict(ms_per_call=t_fo * 1e3, flops=F_fo, tflops=F_fo / t_fo / 1e12)
+    log(f"[bench] first order: {t_fo * 1e3:.2f} ms/call, {F_fo / t_fo / 1e12:.2f} TFLOP/s")
     # the reference's own fixture size: GPU and the oracle on the host cores
     small = (104, 20, 4, 15, 373)
     sargs, sexp = ref_fixture(*small)
@@ -362,6 +383,8 @@ def workload_eos(r, steps, warmup, cpu=True):
         t_cpu = time.perf_counter() - t0
         res["fixture_104_20_4_15_373"].update(oracle_s=t_cpu, oracle_threads=oracle.num_threads(),
                                               gpu_speedup=t_cpu / t_small)
+    for p in d_in:
+        ctx.free(p)
     ctx.close()
     return res
 

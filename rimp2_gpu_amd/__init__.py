@@ -424,6 +424,11 @@ class Context:
                                           _ptr(blad)))
         return djik, blad
 
+    def first_order_device(self, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, *ptrs):
+        """ptrs: Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT,
+        UvirtEOST, djik, blad as device pointers (asynchronous)."""
+        self._chk(lib().rimp2_first_order_device(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, *ptrs))
+
     def eos_set_chunk(self, nocc_per_chunk):
         self._chk(lib().rimp2_eos_set_chunk(self.h, int(nocc_per_chunk)))
 
@@ -519,6 +524,17 @@ def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba, chunk):
     gvirt = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * (vE * no) ** 2 * nba
     launched = sum(2.0 * nba * nv * nv * (nocc - i0) * min(chunk, nocc - i0) for i0 in range(0, nocc, chunk))
     return ampl + tocc + tvirt + gocc + gvirt, launched
+
+
+def first_order_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
+    """FLOPs of rimp2_first_order (nocctot = nocc): the 3-index transforms and
+    the two gen4DimFO contractions, with the CalphaVV product at K = nocc."""
+    nv, no, nE, vE = float(nvirt), float(nocc), float(noccEOS), float(nvirtEOS)
+    djik = 2 * nba * nv * no * nE + 2 * nba * nv * nv * nE + 2 * nba * no * no * no + 2 * nba * no * nE * no \
+        + 2 * (nv * nE) * (nE * no) * nba
+    blad = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * nba * nv * nv * no + 2 * nba * nv * vE * nv \
+        + 2 * (vE * no) * (vE * nv) * nba
+    return djik + blad
 
 
 def metric_flops(nocc, nvirt, naux):

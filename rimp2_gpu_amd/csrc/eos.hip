@@ -70,7 +70,7 @@ struct DgemmArgs {
 
 // 256 threads = 4 waves in 2x2, wave tile (BM/2)x(BN/2) of 16x16 MFMA blocks
 // (v_mfma_f64_16x16x4f64; lane l holds A[l%16][l/16], B[l/16][l%16], result
-// d[r] = D[4r + l/16][l%16]).  K tiles of 16 go through LDS with the next
+// d[r] = D[4r + l/16][l%16]).  K tiles go through LDS with the next
 // tile fetched into registers while the MFMAs run.  LDS layouts keep every
 // fragment read free of bank conflicts: a stored-untransposed A as As[k][m]
 // (pitch BM+16), a transposed A as As[m][k] (pitch 17); B likewise.
@@ -82,7 +82,10 @@ struct DgemmArgs {
 // that share a column panel of op(B) share that XCD's L2.
 template <int BM, int BN, bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_kernel(const DgemmArgs p) {
-  constexpr int BK = 16;
+  // K tile: 32 on the 96x128 tile (twice the loads in flight per barrier:
+  // -2.9 % on the amplitude slab against 16), 16 elsewhere (the 128x128
+  // tile spills at 32)
+  constexpr int BK = (BM == 96) ? 32 : 16;
   constexpr int PA = TA ? BK + 1 : BM + 16;
   constexpr int PB = TB ? BN + 16 : BK + 1;
   constexpr int WM = BM / 2, WN = BN / 2;
