@@ -73,7 +73,7 @@ struct DgemmArgs {
 // d[r] = D[4r + l/16][l%16]).  K tiles go through LDS with the next
 // tile fetched into registers while the MFMAs run.  LDS layouts keep every
 // fragment read free of bank conflicts: a stored-untransposed A as As[k][m]
-// (pitch BM+16), a transposed A as As[m][k] (pitch 17); B likewise.
+// (pitch BM+16), a transposed A as As[m][k] (pitch KP); B likewise.
 // At least two waves per SIMD: the compiler then keeps the accumulators of
 // the larger tiles in VGPRs with no spills (measured 70 ms against 99 ms for
 // the one-wave build, 76 ms for 64x64, on the 1041/99/2173 amplitude slab).
@@ -86,8 +86,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   // -2.9 % on the amplitude slab against 16), 16 elsewhere (the 128x128
   // tile spills at 32)
   constexpr int BK = (BM == 96) ? 32 : 16;
-  constexpr int PA = TA ? BK + 1 : BM + 16;
-  constexpr int PB = TB ? BN + 16 : BK + 1;
+  // pitch of the k-contiguous LDS layouts: a half-wave reads 16 rows at k and
+  // 16 at k + 1; 17 (BK 16) and 34 (BK 32) doubles put them on disjoint banks
+  constexpr int KP = BK == 32 ? BK + 2 : BK + 1;
+  constexpr int PA = TA ? KP : BM + 16;
+  constexpr int PB = TB ? BN + 16 : KP;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int NA = BM * BK / 512, NB = BN * BK / 512;
