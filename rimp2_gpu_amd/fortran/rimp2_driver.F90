@@ -18,6 +18,12 @@
 !>       RIMP2.F90:253-323 does (|test - got| > 1e-10 stops with ERROR3).
 !>       The energy runs only when every denominator e_i + e_j - e_a - e_b is
 !>       negative (the reference's fixtures have them positive).
+!>   rimp2_driver.x file <Calpha.data> <mode> fo
+!>       the same with the first-order record layout the reference reads when
+!>       first_order is on (RIMP2.F90:65, 189, 230; generate.F90:262, 308,
+!>       338-339): CalphaOcc(NBA,nocc,nocctot) and CalphaVV(NBA,nvirt,nvirt)
+!>       after the U matrices, djik and blad after the four arrays; djik and
+!>       blad are formed on the GPU and checked as RIMP2.F90:327-364 does.
 !> mode: 0 fp64, 1 fp32, 2 bf16x3, 3 i8s4, 4 i8s5.
 !> Prints "E_MP2 <mode> <E>" and "dE_uEh <mode-vs-fp64>"; stops with
 !> 'ERROR3' when the energy is not finite (the reference's failure word,
@@ -34,7 +40,10 @@ program rimp2_driver
   real(C_DOUBLE), allocatable :: Pia(:,:,:), V(:,:), eo(:), ev(:), eps(:,:), eps64(:,:)
   real(C_DOUBLE) :: E, E64
   integer(C_SIZE_T) :: total, free
-  logical :: fitted, have_u, have_ref, energy_ok
+  logical :: fitted, have_u, have_ref, energy_ok, first_order
+  integer(C_INT64_T) :: nocctot
+  real(C_DOUBLE), allocatable :: CalphaOcc(:,:,:), CalphaVV(:,:,:), djik(:,:,:,:), blad(:,:,:,:)
+  real(C_DOUBLE), allocatable :: dref(:,:,:,:), bref(:,:,:,:)
   integer(C_INT64_T) :: noccEOS, nvirtEOS
   real(C_DOUBLE), allocatable :: UoccEOST(:,:), UvirtT(:,:), UoccT(:,:), UvirtEOST(:,:)
   real(C_DOUBLE), allocatable :: toccEOS(:,:,:,:), tvirtEOS(:,:,:,:), goccEOS(:,:,:,:), gvirtEOS(:,:,:,:)
@@ -43,6 +52,7 @@ program rimp2_driver
   fitted = .false.
   have_u = .false.
   have_ref = .false.
+  first_order = .false.
   call get_command_argument(1, arg)
   if (trim(arg) == 'gen') then
      call get_command_argument(2, arg); read(arg, *) seed
@@ -56,6 +66,8 @@ program rimp2_driver
   else if (trim(arg) == 'file') then
      call get_command_argument(2, fname)
      call get_command_argument(3, arg); read(arg, *) imode
+     call get_command_argument(4, arg)
+     first_order = (trim(arg) == 'fo')
      lu = 1345
      open(unit=lu, file=trim(fname), status='OLD', form='UNFORMATTED', iostat=ios)
      if (ios /= 0) then
@@ -81,6 +93,15 @@ program rimp2_driver
      if (ios == 0) read(lu, iostat=ios) UoccT
      if (ios == 0) read(lu, iostat=ios) UvirtEOST
      have_u = (ios == 0)
+     nocctot = hdr(6)
+     if (first_order) then
+        if (.not. have_u) error stop 'ERROR3'
+        allocate(CalphaOcc(naux, nocc, nocctot), CalphaVV(naux, nvirt, nvirt))
+        read(lu, iostat=ios) CalphaOcc
+        if (ios /= 0) error stop 'ERROR3'
+        read(lu, iostat=ios) CalphaVV
+        if (ios /= 0) error stop 'ERROR3'
+     endif
      if (have_u) then
         allocate(tref(nvirt, noccEOS, nvirt, noccEOS), vref(nvirtEOS, nocc, nvirtEOS, nocc))
         allocate(gvref(nvirtEOS, nocc, nvirtEOS, nocc), goref(nvirt, noccEOS, nvirt, noccEOS))
@@ -90,9 +111,15 @@ program rimp2_driver
         if (ios == 0) read(lu, iostat=ios) goref
         have_ref = (ios == 0)
      endif
+     if (first_order .and. have_ref) then
+        allocate(dref(nvirt, noccEOS, noccEOS, nocctot), bref(nvirtEOS, nocc, nvirtEOS, nvirt))
+        read(lu, iostat=ios) dref
+        if (ios == 0) read(lu, iostat=ios) bref
+        if (ios /= 0) error stop 'ERROR3'
+     endif
      close(lu)
   else
-     print *, 'usage: rimp2_driver.x gen <seed> <naux> <nvirt> <nocc> <mode> | file <path> <mode>'
+     print *, 'usage: rimp2_driver.x gen <seed> <naux> <nvirt> <nocc> <mode> | file <path> <mode> [fo]'
      error stop 'ERROR3'
   endif
   mode = int(imode, C_INT)
@@ -116,6 +143,18 @@ program rimp2_driver
         call verify('gvirtEOS', gvref, gvirtEOS)
         call verify('goccEOS', goref, goccEOS)
         print '(A)', 'EOS arrays match the file within 1e-10'
+     endif
+  endif
+  if (first_order) then
+     allocate(djik(nvirt, noccEOS, noccEOS, nocctot), blad(nvirtEOS, nocc, nvirtEOS, nvirt))
+     stat = rimp2_first_order(ctx, nvirt, nocc, noccEOS, nvirtEOS, naux, nocctot, Pia, CalphaOcc, CalphaVV, &
+          & UoccEOST, UvirtT, UoccT, UvirtEOST, djik, blad)
+     call rimp2_check(stat, ctx, 'rimp2_first_order')
+     print '(A,2ES25.16)', 'FO_sums ', sum(djik), sum(blad)
+     if (have_ref) then
+        call verify('djik', dref, djik)
+        call verify('blad', bref, blad)
+        print '(A)', 'first-order arrays match the file within 1e-10'
      endif
   endif
 

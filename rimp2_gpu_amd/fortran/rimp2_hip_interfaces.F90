@@ -126,6 +126,16 @@ module rimp2_hip_interfaces
        real(C_DOUBLE) :: toccEOS(*), tvirtEOS(*), goccEOS(*), gvirtEOS(*)
      end function rimp2_eos_arrays
 
+     !> First-order block (RIMP2.F90:172-251) on the GPU, host arrays, blocking.
+     integer(C_INT) function rimp2_first_order(ctx, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, Calpha, &
+          & CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST, djik, blad) bind(C, name="rimp2_first_order")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot
+       real(C_DOUBLE) :: Calpha(*), CalphaOcc(*), CalphaVV(*), UoccEOST(*), UvirtT(*), UoccT(*), UvirtEOST(*)
+       real(C_DOUBLE) :: djik(*), blad(*)
+     end function rimp2_first_order
+
      !> cublasDgemm_v2 as bound in gpu_interfaces.F90:24-32: device pointers,
      !> asynchronous on the context stream, alpha/beta by reference.
      integer(C_INT) function rimp2_dgemm(ctx, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc) &

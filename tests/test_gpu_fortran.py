@@ -180,3 +180,33 @@ def test_record_file_eos_sums(tmp_path, oracle):
     exp = oracle.steps5to8(C, eo, ev, *U)
     for g, e in zip(got, exp):
         assert abs(g - float(np.sum(e))) <= 1e-12 * float(np.sum(np.abs(e))), (g, float(np.sum(e)))
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_first_order_record_file(tmp_path, oracle):
+    """The first-order record layout (RIMP2.F90:189, 230; generate.F90:262,
+    308, 333-339): the driver forms toccEOS..goccEOS and djik/blad on the GPU
+    and checks every element against the file at 1e-10."""
+    from test_gpu_eos import random_inputs
+    nvirt, nocc, noccEOS, nvirtEOS, nba = 40, 9, 3, 5, 160
+    C, eo, ev, UoE, Uv, Uo, UvE = random_inputs(77, nvirt, nocc, noccEOS, nvirtEOS, nba)
+    rng = np.random.default_rng(78)
+    Co = np.asfortranarray(0.1 * rng.standard_normal((nba, nocc, nocc)))
+    Cvv = np.asfortranarray(0.1 * rng.standard_normal((nba, nvirt, nvirt)))
+    to, tv, go, gv = oracle.steps5to8(C, eo, ev, UoE, Uv, Uo, UvE)
+    dj, bl = oracle.first_order(C, Co, Cvv, UoE, Uv, Uo, UvE)
+    hdr = np.array([nvirt, nocc, noccEOS, nvirtEOS, nba, nocc], dtype=np.int32)
+    path = tmp_path / "Calpha_fo.data"
+    write_records(path, [hdr, C, eo, ev, UoE, Uv, Uo, UvE, Co, Cvv, to, tv, gv, go, dj, bl])
+    say("fortran first-order file")
+    rc, out = run("file", path, 4, "fo")
+    assert rc == 0, out
+    assert "EOS arrays match the file within 1e-10" in out
+    assert "first-order arrays match the file within 1e-10" in out
+    # a planted difference in blad stops the run as the reference's check does
+    bl2 = bl.copy(order="F")
+    bl2[1, 2, 3, 4] += 3e-10
+    write_records(path, [hdr, C, eo, ev, UoE, Uv, Uo, UvE, Co, Cvv, to, tv, gv, go, dj, bl2])
+    rc, out = run("file", path, 4, "fo")
+    assert rc != 0 and "ERROR3" in out and "blad" in out
