@@ -192,6 +192,8 @@ contains
 
   !> The reference's element check (RIMP2.F90:257-271): the first element
   !> with |test - got| > 1e-10 is printed and the run stops with ERROR3.
+  !> Written as .not. (|test - got| <= 1e-10) so that a NaN on either side
+  !> fails too (the reference's ">" test lets a NaN pass).
   subroutine verify(what, test, got)
     character(len=*), intent(in) :: what
     real(C_DOUBLE), intent(in) :: test(:,:,:,:), got(:,:,:,:)
@@ -200,7 +202,7 @@ contains
        do c = 1, size(test, 3)
           do b = 1, size(test, 2)
              do a = 1, size(test, 1)
-                if (abs(test(a, b, c, d) - got(a, b, c, d)) > 1.0e-10_C_DOUBLE) then
+                if (.not. (abs(test(a, b, c, d) - got(a, b, c, d)) <= 1.0e-10_C_DOUBLE)) then
                    print *, 'Test(A,B,C,D)', test(a, b, c, d)
                    print *, what, '(A,B,C,D)', got(a, b, c, d)
                    print *, 'Error ', a, b, c, d, 'DIFF', abs(test(a, b, c, d) - got(a, b, c, d))

@@ -161,6 +161,21 @@ def test_reference_fixture_file_mismatch_stops_with_error3(tmp_path, oracle):
 
 @needs_exe
 @pytest.mark.gpu
+def test_reference_fixture_file_nan_stops_with_error3(tmp_path, oracle):
+    """A NaN in the expected arrays fails the element check (the reference's
+    `ABS(test - got) > 1e-10` would let it pass)."""
+    recs, _ = reference_fixture_records(oracle, (74, 10, 4, 15, 273))
+    go = recs[11].copy(order="F")
+    go[0, 1, 2, 3] = np.nan
+    recs[11] = go
+    path = tmp_path / "Calpha_nan.data"
+    write_records(path, recs)
+    rc, out = run("file", path, 0)
+    assert rc != 0 and "ERROR3" in out and "goccEOS" in out
+
+
+@needs_exe
+@pytest.mark.gpu
 def test_record_file_eos_sums(tmp_path, oracle):
     """Random inputs with U records but no expected arrays: the driver forms
     the four arrays and prints their sums, checked against the oracle."""
