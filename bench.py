@@ -309,7 +309,7 @@ def workload_eos(r, steps, warmup, cpu=True):
     dims = (1041, 99, 14, 45, 2173)
     nvirt, nocc, noccEOS, nvirtEOS, nba = dims
     chunk = r.eos_chunk(nvirt, nocc, noccEOS, nvirtEOS)
-    F, Fa = r.eos_flops(*dims, chunk)
+    F, Fa = r.eos_flops(*dims)
     args, expect = ref_fixture(*dims)
     ctx = r.Context(0)
     d_in = [ctx.to_device(x) for x in args]
@@ -338,12 +338,13 @@ def workload_eos(r, steps, warmup, cpu=True):
     assert launches == -(-nocc // chunk), (launches, chunk)
     res = dict(workload="eos", dims=dict(nvirt=nvirt, nocc=nocc, noccEOS=noccEOS, nvirtEOS=nvirtEOS, nba=nba),
                ms_per_call=t * 1e3, ms_per_call_events=eos_ms / steps, flops=F, tflops=F / t / 1e12,
-               amplitude=dict(launches_per_call=launches, chunk=chunk, ms_per_launch=per_launch_s * 1e3,
+               amplitude=dict(kernel="pair_ampl_kernel (FP64 pair kernel, amplitude form)",
+                              launches_per_call=launches, chunk=chunk, ms_per_launch=per_launch_s * 1e3,
                               tflops=Fa / launches / per_launch_s / 1e12,
                               frac_fp64_peak=Fa / launches / per_launch_s / 1e12 / PEAK["fp64_tflops"],
                               share_of_call=ampl_ms / max(eos_ms, 1e-9)),
                max_rel_err_vs_closed_form=max(rel))
-    log(f"[bench] eos: {t * 1e3:.2f} ms/call, {F / t / 1e12:.2f} TFLOP/s; amplitude GEMM "
+    log(f"[bench] eos: {t * 1e3:.2f} ms/call, {F / t / 1e12:.2f} TFLOP/s; amplitude slab "
         f"{per_launch_s * 1e3:.3f} ms x {launches:.0f}, {res['amplitude']['tflops']:.2f} TFLOP/s; rel err {rel}")
     # first-order intermediates at the same size (RIMP2.F90:172-251); the
     # extra 3-index inputs CalphaOcc(NBA,nocc,nocc) and CalphaVV(NBA,nvirt,nvirt)

@@ -510,11 +510,12 @@ def eos_chunk(nvirt, nocc, noccEOS, nvirtEOS):
     return max(1, min(nocc, int((8 << 30) // per)))
 
 
-def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba, chunk):
-    """FLOPs of rimp2_eos_arrays.  Returns (algorithmic, amplitude GEMMs as
-    launched): the algorithmic count builds each unordered occupied pair once
-    (nocc (nocc + 1) / 2 pairs) and transforms it; the launched amplitude
-    GEMMs of a chunk of c orbitals starting at i0 cover c (nocc - i0) pairs."""
+def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
+    """Algorithmic FLOPs of rimp2_eos_arrays in its order of work.  Returns
+    (total, amplitude build): the amplitude build forms each unordered
+    occupied pair once (nocc (nocc + 1) / 2 pairs, 2 NBA nvirt^2 each), then
+    each array's transforms (the index that shrinks most first) and Steps 7-8
+    as C3^T C3."""
     nv, no, nE, vE = float(nvirt), float(nocc), float(noccEOS), float(nvirtEOS)
     npair = no * (no + 1) / 2
     ampl = 2 * nba * nv * nv * npair
@@ -522,8 +523,7 @@ def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba, chunk):
     tvirt = 2 * vE * nv * nv * npair + 2 * vE * vE * nv * npair + 4 * vE * vE * no ** 3
     gocc = 2 * nba * nv * no * nE + 2 * nba * nv * nv * nE + 2 * (nv * nE) ** 2 * nba
     gvirt = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * (vE * no) ** 2 * nba
-    launched = sum(2.0 * nba * nv * nv * (nocc - i0) * min(chunk, nocc - i0) for i0 in range(0, nocc, chunk))
-    return ampl + tocc + tvirt + gocc + gvirt, launched
+    return ampl + tocc + tvirt + gocc + gvirt, ampl
 
 
 def first_order_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):

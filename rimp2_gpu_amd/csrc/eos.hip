@@ -11,15 +11,16 @@
 //   transformed 3-index tensor (RIMP2.F90:136-170).
 //
 // Order of work here (not the reference's): both amplitude arrays share one
-// t, built chunk by chunk over I as a batched GEMM with the 1/Delta applied in
-// its epilogue (t never exists whole) and for each unordered occupied pair
-// once (t(A,B,J,I) = t(B,A,I,J)); from each chunk the occupied pair of
+// t, built chunk by chunk over I (t never exists whole) for each unordered
+// occupied pair once (t(A,B,J,I) = t(B,A,I,J)) by the FP64 pair kernel in its
+// amplitude form (pair.hip: the energy kernel's operand pipeline, with an
+// epilogue that writes w V / Delta); from each chunk the occupied pair of
 // toccEOS and the virtual pair of tvirtEOS are contracted at once (the index
 // of each array that shrinks most goes first), the remaining two indices of
 // each are transformed after the last chunk, and a symmetrization restores
-// the pairs J < I.  Every step is one launch of
-// the FP64 MFMA GEMM below with two-level batch strides, so the reorders of
-// toccB / tvirtB (RIMP2.F90:456-508) are just output strides.
+// the pairs J < I.  Every other step is one launch of the FP64 MFMA GEMM
+// below with two-level batch strides, so the reorders of toccB / tvirtB
+// (RIMP2.F90:456-508) are just output strides.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,7 +30,7 @@
 namespace rimp2 {
 
 struct EosState {
-  DevBuf in, work, out, fo;
+  DevBuf in, work, out, fo, pk;
 };
 void eos_destroy(EosState* s) {
   if (!s) return;
@@ -37,6 +38,7 @@ void eos_destroy(EosState* s) {
   s->work.release();
   s->out.release();
   s->fo.release();
+  s->pk.release();
   delete s;
 }
 
@@ -44,13 +46,8 @@ namespace {
 
 using v4d = double __attribute__((ext_vector_type(4)));
 
-enum { EPI_PLAIN = 0, EPI_AMPL = 1 };
-
 // C[z] = alpha op(A[z]) op(B[z]) + beta C[z] for batch members z = z1 + Z1 z2,
-// member offsets z1 * s?1 + z2 * s?2 (elements).  EPI_AMPL: C(m, n) of member
-// z is the weighted amplitude of occupied pair (I = i0 + z, J = j0 + n / nv),
-// virtuals (A = m, B = n % nv): w_IJ acc / (e_I + e_J - e_B - e_A) with
-// w_IJ = 1 (J > I), 1/2 (J = I), 0 (J < I): each unordered pair once.
+// member offsets z1 * s?1 + z2 * s?2 (elements).
 struct DgemmArgs {
   int64_t M, N, K;
   double alpha, beta;
@@ -63,9 +60,6 @@ struct DgemmArgs {
   int64_t Z1, zbase;
   int64_t tm, tiles;  // row tiles, tiles per member
   int a_vec, b_vec;   // 16-byte loads allowed (even strides, aligned bases)
-  const double* eo;
-  const double* ev;
-  int64_t nv, i0, j0;
 };
 
 // 256 threads = 4 waves in 2x2, wave tile (BM/2)x(BN/2) of 16x16 MFMA blocks
@@ -76,14 +70,15 @@ struct DgemmArgs {
 // (pitch BM+16), a transposed A as As[m][k] (pitch KP); B likewise.
 // At least two waves per SIMD: the compiler then keeps the accumulators of
 // the larger tiles in VGPRs with no spills (measured 70 ms against 99 ms for
-// the one-wave build, 76 ms for 64x64, on the 1041/99/2173 amplitude slab).
+// the one-wave build, 76 ms for 64x64, on a batch of eight 1041 x 103059 x
+// 2173 products).
 // Tiles are numbered row tile fastest and laid out so that each XCD (blocks
 // b with the same b % 8) runs one contiguous range of them: the row tiles
 // that share a column panel of op(B) share that XCD's L2.
-template <int BM, int BN, bool TA, bool TB, int EPI>
+template <int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dgemm_kernel(const DgemmArgs p) {
   // K tile: 32 on the 96x128 tile (twice the loads in flight per barrier:
-  // -2.9 % on the amplitude slab against 16), 16 elsewhere (the 128x128
+  // -2.9 % against 16 on the products above), 16 elsewhere (the 128x128
   // tile spills at 32)
   constexpr int BK = (BM == 96) ? 32 : 16;
   // pitch of the k-contiguous LDS layouts: a half-wave reads 16 rows at k and
@@ -252,24 +247,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     for (int j = 0; j < FN; j++) {
       const int64_t gn = n0 + wn + 16 * j + (lane & 15);
       if (gn >= N) continue;
-      double dn = 0.0, w = 1.0;
-      if (EPI == EPI_AMPL) {
-        const int64_t Jr = gn / p.nv, Bv = gn - Jr * p.nv, J = p.j0 + Jr, I = p.i0 + z;
-        dn = (p.eo[I] + p.eo[J]) - p.ev[Bv];
-        w = J > I ? 1.0 : (J == I ? 0.5 : 0.0);
-      }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int64_t gm = m0 + wm + 16 * i + 4 * r + (lane >> 4);
         if (gm >= M) continue;
         double* q = C + gm + p.ldc * gn;
-        if (EPI == EPI_AMPL) {
-          *q = w == 0.0 ? 0.0 : w * (acc[i][j][r] / (dn - p.ev[gm]));
-        } else {
-          double v = p.alpha * acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * *q;
-          *q = v;
-        }
+        double v = p.alpha * acc[i][j][r];
+        if (p.beta != 0.0) v += p.beta * *q;
+        *q = v;
       }
     }
 }
@@ -286,11 +271,6 @@ struct Gemm {
   double* C = nullptr;
   int64_t ldc = 1, sC1 = 0, sC2 = 0;
   int64_t Z1 = 1, Z2 = 1;
-  // amplitude epilogue
-  bool ampl = false;
-  const double* eo = nullptr;
-  const double* ev = nullptr;
-  int64_t nv = 1, i0 = 0, j0 = 0;
 };
 
 bool even_aligned(const double* base, int64_t ld, int64_t s1, int64_t s2) {
@@ -310,11 +290,10 @@ void launch_tiles(hipStream_t s, const Gemm& g, DgemmArgs a) {
   for (int64_t z0 = 0; z0 < Z; z0 += zmax) {
     a.zbase = z0;
     const dim3 grid((unsigned)gx, (unsigned)std::min<int64_t>(zmax, Z - z0));
-    if (g.ampl) hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, false, EPI_AMPL>), grid, dim3(256), 0, s, a);
-    else if (!g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, false, EPI_PLAIN>), grid, dim3(256), 0, s, a);
-    else if (g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, false, EPI_PLAIN>), grid, dim3(256), 0, s, a);
-    else if (!g.ta && g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, true, EPI_PLAIN>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, true, EPI_PLAIN>), grid, dim3(256), 0, s, a);
+    if (!g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, false>), grid, dim3(256), 0, s, a);
+    else if (g.ta && !g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, false>), grid, dim3(256), 0, s, a);
+    else if (!g.ta && g.tb) hipLaunchKernelGGL((dgemm_kernel<BM, BN, false, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((dgemm_kernel<BM, BN, true, true>), grid, dim3(256), 0, s, a);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
 }
@@ -354,11 +333,6 @@ void run_gemm(hipStream_t s, const Gemm& g) {
   a.zbase = 0;
   a.a_vec = even_aligned(g.A, g.lda, g.sA1, g.sA2);
   a.b_vec = even_aligned(g.B, g.ldb, g.sB1, g.sB2);
-  a.eo = g.eo;
-  a.ev = g.ev;
-  a.nv = g.nv;
-  a.i0 = g.i0;
-  a.j0 = g.j0;
   switch (pick_tile(g)) {
     case TILE_64x64: launch_tiles<64, 64>(s, g, a); break;
     case TILE_96x128: launch_tiles<96, 128>(s, g, a); break;
@@ -469,6 +443,11 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
   ScopedTimer tm(c, RIMP2_K_EOS);
   const int64_t nv2 = nv * nv;
 
+  // The 3-index tensor in the FP64 pair kernel's operand order (64-wide
+  // virtual blocks), once per call.
+  const PackGeom pg = make_geom(RIMP2_MODE_FP64, nba, nv, 64);
+  void* pk = c->eos->pk.ensure(pg.bytes_per_occ() * (size_t)no);
+  pack_B(c, Cal, nba, nv, no, pg, pk, nullptr, s);
   // Each unordered occupied pair once: t(A,B,J,I) = t(B,A,I,J), so a chunk
   // of I builds J >= i0 only, weighted by w_IJ (1 above the diagonal, 1/2 on
   // it, 0 below).  Any 4-index transform X of the weighted half satisfies
@@ -480,29 +459,14 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
   }
   for (int64_t i0 = 0; i0 < no; i0 += ci) {
     const int64_t cz = std::min(ci, no - i0), nj = no - i0;
-    {  // t(A, B, J - i0, I - i0), J >= i0:  w C_I^T C_J / Delta   (RIMP2.F90:399-404)
-      Gemm g;
-      g.ampl = true;
-      g.ta = true;
-      g.M = nv;
-      g.N = nv * nj;
-      g.K = nba;
-      g.A = Cal + nba * nv * i0;
-      g.lda = nba;
-      g.sA1 = nba * nv;
-      g.B = Cal + nba * nv * i0;
-      g.ldb = nba;
-      g.C = T;
-      g.ldc = nv;
-      g.sC1 = nv2 * nj;
-      g.Z1 = cz;
-      g.eo = eo;
-      g.ev = ev;
-      g.nv = nv;
-      g.i0 = i0;
-      g.j0 = i0;
+    {  // t(A, B, J - i0, I - i0), J >= I:  w V^{IJ}_AB / Delta on the FP64 pair kernel   (RIMP2.F90:399-404)
       ScopedTimer ta(c, RIMP2_K_EOS_AMPL);
-      run_gemm(s, g);
+      for (int64_t z = 1; z < cz; z++) {  // J in [i0, I): weight 0
+        const int64_t n = nv2 * z;
+        hipLaunchKernelGGL(zero_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, s, T + nv2 * nj * z, n);
+        RIMP2_HIP_CHECK(hipGetLastError());
+      }
+      pair_amplitudes_device(s, pg, pk, no, nv, eo, ev, i0, cz, T);
     }
     {  // toccEOS, occupied J: P(A,B,Jl,I) = sum_J t(A,B,J,I) UoccEOST(J,Jl)
       Gemm g;

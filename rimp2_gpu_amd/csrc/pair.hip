@@ -60,6 +60,30 @@ __global__ __launch_bounds__(PAIR_THREADS) void pair_kernel(PairParams p) {
   }
 }
 
+// Amplitude form over one chunk of occupied orbitals: the pairs i <= j with
+// i in [i0, i0 + nchunk), ordered by i, then j; units of a pair are
+// consecutive (same XCD mapping as pair_kernel).
+template <class Pol, int TM>
+__global__ __launch_bounds__(PAIR_THREADS) void pair_ampl_kernel(PairParams p, AmplOut o, int64_t nchunk) {
+  RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
+  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
+  for (int64_t k = slot; k < p.chunk; k += nslot) {
+    const int64_t wu = xcd * p.chunk + k;
+    if (wu >= p.nwu) break;
+    const int64_t pidx = wu / p.ntp, tp = wu % p.ntp;
+    // pairs of orbital i0 + z: nj - z  (nj = nocc - i0)
+    int64_t z = 0, rem = pidx;
+    while (z + 1 < nchunk && rem >= o.nj - z) {
+      rem -= o.nj - z;
+      z++;
+    }
+    const int64_t i = o.i0 + z, j = i + rem;
+    int64_t A, Bk;
+    const int kind = unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk);
+    pair_tile<Pol, TM, true>(p.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3, &o);
+  }
+}
+
 // Per pair: Neumaier sum of its ntp partials in order -> eps_pair (both
 // triangles) and the pair's contribution to E (x2 off the diagonal).
 __global__ void pair_reduce_kernel(const double* partials, int64_t ntp, int64_t npairs, const int64_t* pair_list,
@@ -230,6 +254,43 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
     RIMP2_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(total_kernel, dim3(1), dim3(256), 0, s, contrib, npairs, d_etot);
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+// Amplitude slab of one chunk (FP64 pair kernel, 64-wide virtual blocks) on
+// stream s: T[a + nv b + nv^2 (j - i0) + nv^2 (nocc - i0) (i - i0)] =
+// w_ij V^{ij}_ab / Delta for i in [i0, i0 + nchunk), j >= i (entries j < i
+// are not written).  packed: FP64 packing of the 3-index tensor, geometry g
+// (tile 64).
+void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed, int64_t nocc, int64_t nvirt,
+                            const double* d_eo, const double* d_ev, int64_t i0, int64_t nchunk, double* T) {
+  constexpr int TM = 64;
+  if (g.mode != RIMP2_MODE_FP64 || g.nvp % TM) fail(RIMP2_ERR_ARG, "amplitude slab needs the FP64 64-wide packing");
+  if (i0 < 0 || nchunk < 1 || i0 + nchunk > nocc) fail(RIMP2_ERR_ARG, "bad occupied chunk");
+  const int64_t nb = g.nvp / TM;
+  const int64_t ntp = pair_units(nb, packs_diagonal<PolF64, TM>());
+  const int64_t nj = nocc - i0;
+  const int64_t npairs = nchunk * nj - nchunk * (nchunk - 1) / 2;
+  PairParams p;
+  p.f.packed = static_cast<const unsigned char*>(packed);
+  p.f.sigma = nullptr;
+  p.f.eps_o = d_eo;
+  p.f.eps_v = d_ev;
+  p.f.nvp = g.nvp;
+  p.f.nks = g.nks;
+  p.f.nrb = g.nrb;
+  p.partials = nullptr;
+  p.pair_list = nullptr;
+  p.nocc = nocc;
+  p.nb = nb;
+  p.ntp = ntp;
+  p.npairs = npairs;
+  p.nwu = npairs * ntp;
+  p.chunk = ceil_div(p.nwu, 8);
+  AmplOut o{T, d_eo, d_ev, nvirt, nj, i0};
+  const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
+  hipLaunchKernelGGL((pair_ampl_kernel<PolF64, TM>), dim3((unsigned)(8 * nslot)), dim3(PAIR_THREADS), 0, s, p, o,
+                     nchunk);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

@@ -160,6 +160,17 @@ struct FragView {
   int64_t nvp, nks, nrb;
 };
 
+// Output of the amplitude form of the pair kernel (FP64): for the unordered
+// occupied pair i <= j of a chunk i in [i0, i0 + nchunk), every element
+//   T[a + nv b + nv^2 (j - i0) + nv^2 nj (i - i0)] = w_ij V^{ij}_ab / (e_i + e_j - e_b - e_a)
+// with w_ij = 1 (i < j), 1/2 (i = j), a, b < nv (padding rows are not written).
+struct AmplOut {
+  double* T;
+  const double* eo;  // [nocc]
+  const double* ev;  // [nv]
+  int64_t nv, nj, i0;
+};
+
 // Bytes of LDS one stage of the operand pipeline occupies (4 operand slabs).
 template <class Pol, int TM>
 constexpr int pair_stage_bytes() {
@@ -313,10 +324,12 @@ constexpr int PAIR_THREADS = 512;
 // DMA addressing: the slab resource and the stage offset of each transfer are
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
 // offset, so each transfer is one buffer load plus the M0 write.
-template <class Pol, int TM>
+// AMPL: the amplitude form (AmplOut above) instead of the energy epilogue.
+template <class Pol, int TM, bool AMPL = false>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
-                            unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3) {
+                            unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3,
+                            const AmplOut* ao = nullptr) {
   constexpr int NW = PAIR_THREADS / 64;       // waves
   constexpr int FPB = TM / Pol::RB;           // fragments per block side
   constexpr int WT = TM / 2;                  // wave tile side
@@ -458,6 +471,39 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     step(st + 3, rb, ra, buf3, buf0);
   }
   RIMP2_WAIT_VMCNT_LGKM0(0);  // trailing re-fetches and the unused last read are done before LDS is reused
+
+  if constexpr (AMPL) {
+    static_assert(!is_i8<Pol>::value, "the amplitude form applies no exact-slice row scales");
+    // ---- epilogue: amplitudes to global memory (registers only) ----
+    // tile pair: tile 1 holds V^{ij}[A,B] (a, b); tile 2 holds V^{ji}[A,B]
+    // (a, b) = V^{ij}(b, a), written transposed.  Diagonal units: each wave
+    // group holds V^{ij}[X,X] of its block X (group 1 idle for a lone block).
+    const double w = (i == j) ? 0.5 : 1.0;
+    const double eij = ao->eo[i] + ao->eo[j];
+    double* __restrict__ T = ao->T + ao->nv * ao->nv * ((j - ao->i0) + ao->nj * (i - ao->i0));
+    auto put = [&](int64_t a, int64_t b, double v) {
+      if (a < ao->nv && b < ao->nv) T[a + ao->nv * b] = w * (v / ((eij - ao->ev[b]) - ao->ev[a]));
+    };
+    const bool active = tile_pair || !(grp == 1 && kind == UNIT_ONE_DIAG);
+    const int64_t X = grp ? Bk : A;
+    if (active) {
+#pragma unroll
+      for (int x = 0; x < FW; x++)
+#pragma unroll
+        for (int y = 0; y < FW; y++)
+#pragma unroll
+          for (int r = 0; r < Pol::NE; r++) {
+            const int64_t al = wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int64_t bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
+            const double v = Pol::value(acc[x][y], r);
+            if (!tile_pair) put(X * TM + al, X * TM + bl, v);
+            else if (grp == 0) put(A * TM + al, Bk * TM + bl, v);
+            else put(Bk * TM + bl, A * TM + al, v);
+          }
+    }
+    __syncthreads();  // every wave's LDS reads are done before the next unit's DMA
+    return 0.0;
+  }
 
   // ---- epilogue: energy terms in FP64 ----
   const double ei = f.eps_o[i], ej = f.eps_o[j];
