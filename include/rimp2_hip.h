@@ -276,8 +276,10 @@ int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noc
                               const double* d_UoccT, const double* d_UvirtEOST, double* d_djik, double* d_blad);
 /* Occupied orbitals per amplitude chunk of rimp2_eos_arrays (0 = automatic:
  * the chunk's amplitude slab, nvirt^2 nocc doubles per orbital, and its
- * partial transforms stay within 8 GiB).  Bounds the work memory; has no
- * counterpart in the reference, which holds every amplitude at once. */
+ * tvirtEOS partial transform stay within 8 GiB; the toccEOS partial of every
+ * orbital, nvirt^2 noccEOS nocc doubles, is kept whole).  Bounds the work
+ * memory; has no counterpart in the reference, which holds every amplitude at
+ * once. */
 int rimp2_eos_set_chunk(void* ctx, int64_t nocc_per_chunk);
 
 #ifdef __cplusplus

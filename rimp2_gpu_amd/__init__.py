@@ -505,8 +505,8 @@ def pair_flops(nocc, nvirt, naux):
 
 def eos_chunk(nvirt, nocc, noccEOS, nvirtEOS):
     """Occupied orbitals per amplitude slab chosen by rimp2_eos_arrays when no
-    chunk is set: the slab and its partial transforms within 8 GiB."""
-    per = (nvirt * nvirt * nocc + nvirt * nvirt * noccEOS + nvirtEOS * nvirt * nocc) * 8
+    chunk is set: the slab and its tvirtEOS partial transform within 8 GiB."""
+    per = (nvirt * nvirt * nocc + nvirtEOS * nvirt * nocc) * 8
     return max(1, min(nocc, int((8 << 30) // per)))
 
 

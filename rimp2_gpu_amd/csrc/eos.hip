@@ -25,7 +25,7 @@
 
 #include <algorithm>
 
-#include "rimp2_common.h"
+#include "pair_core.h"
 
 namespace rimp2 {
 
@@ -60,6 +60,7 @@ struct DgemmArgs {
   int64_t Z1, zbase;
   int64_t tm, tiles;  // row tiles, tiles per member
   int a_vec, b_vec;   // 16-byte loads allowed (even strides, aligned bases)
+  int lower;          // symmetric product: tiles strictly above the diagonal are skipped
 };
 
 // 256 threads = 4 waves in 2x2, wave tile (BM/2)x(BN/2) of 16x16 MFMA blocks
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   double* __restrict__ C = p.C + z1 * p.sC1 + z2 * p.sC2;
   const int64_t M = p.M, N = p.N, K = p.K, lda = p.lda, ldb = p.ldb;
   const int64_t m0 = (t % p.tm) * BM, n0 = (t / p.tm) * BN;
+  if (p.lower && n0 > m0 + BM - 1) return;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
@@ -271,6 +273,7 @@ struct Gemm {
   double* C = nullptr;
   int64_t ldc = 1, sC1 = 0, sC2 = 0;
   int64_t Z1 = 1, Z2 = 1;
+  bool lower = false;  // C symmetric: only tiles touching the lower triangle
 };
 
 bool even_aligned(const double* base, int64_t ld, int64_t s1, int64_t s2) {
@@ -331,12 +334,34 @@ void run_gemm(hipStream_t s, const Gemm& g) {
   a.sC2 = g.sC2;
   a.Z1 = g.Z1;
   a.zbase = 0;
+  a.lower = g.lower ? 1 : 0;
   a.a_vec = even_aligned(g.A, g.lda, g.sA1, g.sA2);
   a.b_vec = even_aligned(g.B, g.ldb, g.sB1, g.sB2);
   switch (pick_tile(g)) {
     case TILE_64x64: launch_tiles<64, 64>(s, g, a); break;
     case TILE_96x128: launch_tiles<96, 128>(s, g, a); break;
     case TILE_128x128: launch_tiles<128, 128>(s, g, a); break;
+  }
+}
+
+// C[m + ldc n] = C[n + ldc m] for n > m (n x n), through 32 x 32 LDS tiles:
+// block (bi, bj), bi < bj, reads tile (bj, bi) of the lower triangle row-
+// contiguously and writes its transpose to tile (bi, bj).
+__global__ __launch_bounds__(256) void mirror_lower_kernel(double* C, int64_t n, int64_t ldc) {
+  __shared__ double tile[32][33];
+  const int64_t nt = (n + 31) / 32;
+  int64_t bi, bj;
+  pairk::tri_decode((int64_t)blockIdx.x, bi, bj);  // bi <= bj
+  if (bi == bj || bj >= nt) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {  // source: rows bj*32 + tx, cols bi*32 + r  (lower: row > col)
+    const int64_t row = bj * 32 + tx, col = bi * 32 + r;
+    if (row < n && col < n) tile[r][tx] = C[row + ldc * col];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {  // destination: rows bi*32 + tx, cols bj*32 + r
+    const int64_t row = bi * 32 + tx, col = bj * 32 + r;
+    if (row < n && col < n) C[row + ldc * col] = tile[tx][r];
   }
 }
 
@@ -390,7 +415,7 @@ EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, in
   p.nba = nba;
   p.ci = ci;
   p.T = nv * nv * no * ci;
-  p.P = nv * nv * nE * ci;
+  p.P = nv * nv * nE * no;  // every occupied orbital: the I contraction runs once
   p.S = nv * nv * nE * nE;
   p.Q = vE * nv * no * ci;
   p.R = vE * vE * no * no;
@@ -407,9 +432,9 @@ EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, in
 }
 
 // Chunk of occupied orbitals per amplitude slab: the t slab (nv^2 no doubles
-// per orbital) and its two partial transforms stay within 8 GiB.
+// per orbital) and its tvirtEOS partial transform stay within 8 GiB.
 int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE) {
-  const double per = (double)(nv * nv * no + nv * nv * nE + vE * nv * no) * sizeof(double);
+  const double per = (double)(nv * nv * no + vE * nv * no) * sizeof(double);
   const double budget = 8.0 * (1ull << 30);
   int64_t ci = (int64_t)(budget / per);
   return std::max<int64_t>(1, std::min<int64_t>(no, ci));
@@ -478,24 +503,10 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.sA1 = nv2 * nj;
       g.B = Uo5 + i0;
       g.ldb = no;
-      g.C = P;
+      g.C = P + nv2 * nE * i0;
       g.ldc = nv2;
       g.sC1 = nv2 * nE;
       g.Z1 = cz;
-      run_gemm(s, g);
-    }
-    {  // toccEOS, occupied I: S(A,B,Jl,Il) += sum_I P(A,B,Jl,I) UoccEOST(I,Il)
-      Gemm g;
-      g.M = nv2 * nE;
-      g.N = nE;
-      g.K = cz;
-      g.A = P;
-      g.lda = nv2 * nE;
-      g.B = Uo5 + i0;
-      g.ldb = no;
-      g.beta = i0 > 0 ? 1.0 : 0.0;
-      g.C = S;
-      g.ldc = nv2 * nE;
       run_gemm(s, g);
     }
     {  // tvirtEOS, virtual A: Q(Al, B, J, I) = sum_A UvirtEOST(A,Al) t(A,B,J,I)
@@ -531,6 +542,19 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.Z2 = cz;
       run_gemm(s, g);
     }
+  }
+  {  // toccEOS, occupied I: S(A,B,Jl,Il) = sum_I P(A,B,Jl,I) UoccEOST(I,Il)
+    Gemm g;
+    g.M = nv2 * nE;
+    g.N = nE;
+    g.K = no;
+    g.A = P;
+    g.lda = nv2 * nE;
+    g.B = Uo5;
+    g.ldb = no;
+    g.C = S;
+    g.ldc = nv2 * nE;
+    run_gemm(s, g);
   }
   {  // toccEOS, virtual A: S2(Al,B,Jl,Il) = sum_A UvirtT(A,Al) S(A,B,Jl,Il)
     Gemm g;
@@ -644,9 +668,10 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.Z1 = nO;
       run_gemm(s, g);
     }
-    {
+    {  // C3^T C3 is symmetric: the lower tiles, then the mirror
       Gemm g;
       g.ta = true;
+      g.lower = true;
       g.M = nV * nO;
       g.N = nV * nO;
       g.K = nba;
@@ -657,6 +682,9 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.C = out;
       g.ldc = nV * nO;
       run_gemm(s, g);
+      const int64_t n = nV * nO, nt = ceil_div(n, 32);
+      hipLaunchKernelGGL(mirror_lower_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, out, n, n);
+      RIMP2_HIP_CHECK(hipGetLastError());
     }
   }
 }
