@@ -528,11 +528,12 @@ def eos_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
 
 def first_order_flops(nvirt, nocc, noccEOS, nvirtEOS, nba):
     """FLOPs of rimp2_first_order (nocctot = nocc): the 3-index transforms and
-    the two gen4DimFO contractions, with the CalphaVV product at K = nocc."""
+    the two gen4DimFO contractions; the CalphaVV chain (K = nocc) is summed
+    with its virtual contraction first, on the nocc columns it uses."""
     nv, no, nE, vE = float(nvirt), float(nocc), float(noccEOS), float(nvirtEOS)
     djik = 2 * nba * nv * no * nE + 2 * nba * nv * nv * nE + 2 * nba * no * no * no + 2 * nba * no * nE * no \
         + 2 * (nv * nE) * (nE * no) * nba
-    blad = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * nba * nv * nv * no + 2 * nba * nv * vE * nv \
+    blad = 2 * nba * nv * no * no + 2 * nba * nv * vE * no + 2 * nba * nv * vE * no + 2 * nba * vE * no * nv \
         + 2 * (vE * no) * (vE * nv) * nba
     return djik + blad
 

@@ -702,7 +702,7 @@ void first_order_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, 
   if (!c->eos) c->eos = new EosState();
   const int64_t nt = no;
   const int64_t n_c2 = nba * nv * std::max(nE, no), n_c3 = nba * std::max(nv * nE, vE * no);
-  const int64_t n_o2 = nba * no * nt, n_c4 = nba * nE * nt, n_v2 = nba * nv * nv, n_b4 = nba * vE * nv;
+  const int64_t n_o2 = nba * no * nt, n_c4 = nba * nE * nt, n_v2 = nba * vE * no, n_b4 = nba * vE * nv;
   const size_t o_c3 = align256((size_t)n_c2 * 8), o_o2 = o_c3 + align256((size_t)n_c3 * 8),
                o_c4 = o_o2 + align256((size_t)n_o2 * 8), o_v2 = o_c4 + align256((size_t)n_c4 * 8),
                o_b4 = o_v2 + align256((size_t)n_v2 * 8), total = o_b4 + align256((size_t)n_b4 * 8);
@@ -743,9 +743,11 @@ void first_order_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, 
   // blad: B3(alpha,Al,J) = sum_B UvirtEOST(B,Al) sum_j Calpha(alpha,B,j) UoccT(j,J)   (RIMP2.F90:220-228)
   gemm(false, nba * nv, no, no, Cal, nba * nv, 0, Uo6, no, c2, nba * nv, 0, 1);
   gemm(false, nba, vE, nv, c2, nba, nba * nv, Uv6, nv, c3, nba, nba * vE, no);
-  // B4(alpha,Al,b) = sum_a UvirtEOST(a,Al) sum_{k < nocc} CalphaVV(alpha,a,k) UvirtT[k + nocc b]   (RIMP2.F90:231-244)
-  gemm(false, nba * nv, nv, no, Cvv, nba * nv, 0, Uv5, no, v2, nba * nv, 0, 1);
-  gemm(false, nba, vE, nv, v2, nba, nba * nv, Uv6, nv, b4, nba, nba * vE, nv);
+  // B4(alpha,Al,b) = sum_a UvirtEOST(a,Al) sum_{k < nocc} CalphaVV(alpha,a,k) UvirtT[k + nocc b]   (RIMP2.F90:231-244),
+  // summed as sum_{k < nocc} [sum_a UvirtEOST(a,Al) CalphaVV(alpha,a,k)] UvirtT[k + nocc b]: the virtual
+  // contraction on the nocc columns the product uses instead of on all nvirt of them
+  gemm(false, nba, vE, nv, Cvv, nba, nba * nv, Uv6, nv, v2, nba, nba * vE, no);
+  gemm(false, nba * vE, nv, no, v2, nba * vE, 0, Uv5, no, b4, nba * vE, 0, 1);
   gemm(true, vE * no, vE * nv, nba, c3, nba, 0, b4, nba, blad, vE * no, 0, 1);
 }
 
