@@ -274,11 +274,12 @@ int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noc
                               int64_t nba, int64_t nocctot, const double* d_Calpha, const double* d_CalphaOcc,
                               const double* d_CalphaVV, const double* d_UoccEOST, const double* d_UvirtT,
                               const double* d_UoccT, const double* d_UvirtEOST, double* d_djik, double* d_blad);
-/* Occupied orbitals per amplitude chunk of rimp2_eos_arrays (0 = automatic:
- * the chunk's amplitude slab, nvirt^2 nocc doubles per orbital, and its
- * tvirtEOS partial transform stay within 8 GiB; the toccEOS partial of every
- * orbital, nvirt^2 noccEOS nocc doubles, is kept whole).  Bounds the work
- * memory; has no counterpart in the reference, which holds every amplitude at
+/* Occupied orbitals per amplitude chunk of rimp2_eos_arrays.  0 (default):
+ * automatic, the chunk's amplitude slab (nvirt^2 nocc doubles per orbital)
+ * and its tvirtEOS partial within 8 GiB, and the toccEOS partial of every
+ * orbital (nvirt^2 noccEOS nocc doubles) kept whole when it fits 32 GiB.  An
+ * explicit chunk bounds the work memory: the toccEOS partial then holds one
+ * chunk.  Has no counterpart in the reference, which holds every amplitude at
  * once. */
 int rimp2_eos_set_chunk(void* ctx, int64_t nocc_per_chunk);
 

@@ -505,8 +505,10 @@ def pair_flops(nocc, nvirt, naux):
 
 def eos_chunk(nvirt, nocc, noccEOS, nvirtEOS):
     """Occupied orbitals per amplitude slab chosen by rimp2_eos_arrays when no
-    chunk is set: the slab and its tvirtEOS partial transform within 8 GiB."""
-    per = (nvirt * nvirt * nocc + nvirtEOS * nvirt * nocc) * 8
+    chunk is set: the slab and its tvirtEOS partial transform (plus the
+    toccEOS partial when that is not kept whole, above 32 GiB) within 8 GiB."""
+    p_all = nvirt * nvirt * noccEOS * nocc * 8 <= (32 << 30)
+    per = (nvirt * nvirt * nocc + (0 if p_all else nvirt * nvirt * noccEOS) + nvirtEOS * nvirt * nocc) * 8
     return max(1, min(nocc, int((8 << 30) // per)))
 
 

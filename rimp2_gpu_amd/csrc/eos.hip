@@ -406,7 +406,9 @@ struct EosPlan {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, int64_t ci) {
+// p_all: the toccEOS partial P of every occupied orbital is kept (one I
+// contraction at the end); else P holds one chunk and S accumulates per chunk.
+EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, int64_t ci, bool p_all) {
   EosPlan p;
   p.nv = nv;
   p.no = no;
@@ -415,7 +417,7 @@ EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, in
   p.nba = nba;
   p.ci = ci;
   p.T = nv * nv * no * ci;
-  p.P = nv * nv * nE * no;  // every occupied orbital: the I contraction runs once
+  p.P = nv * nv * nE * (p_all ? no : ci);
   p.S = nv * nv * nE * nE;
   p.Q = vE * nv * no * ci;
   p.R = vE * vE * no * no;
@@ -433,8 +435,8 @@ EosPlan eos_plan(int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, in
 
 // Chunk of occupied orbitals per amplitude slab: the t slab (nv^2 no doubles
 // per orbital) and its tvirtEOS partial transform stay within 8 GiB.
-int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE) {
-  const double per = (double)(nv * nv * no + vE * nv * no) * sizeof(double);
+int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE, bool p_all) {
+  const double per = (double)(nv * nv * no + (p_all ? 0 : nv * nv * nE) + vE * nv * no) * sizeof(double);
   const double budget = 8.0 * (1ull << 30);
   int64_t ci = (int64_t)(budget / per);
   return std::max<int64_t>(1, std::min<int64_t>(no, ci));
@@ -442,7 +444,6 @@ int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE) {
 
 }  // namespace
 
-int64_t eos_chunk_size(int64_t nv, int64_t no, int64_t nE, int64_t vE) { return eos_chunk(nv, no, nE, vE); }
 
 // Queues Steps 5-8 on c->stream (device arrays; outputs in the reference's
 // layouts, RIMP2.F90:92,123,146,169).
@@ -451,9 +452,12 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
                        const double* Uv6, double* toccEOS, double* tvirtEOS, double* goccEOS, double* gvirtEOS,
                        int64_t ci) {
   if (!c->eos) c->eos = new EosState();
-  if (ci <= 0) ci = eos_chunk(nv, no, nE, vE);
+  // Automatic chunking keeps P whole when it fits 32 GiB (nv^2 nE no doubles);
+  // an explicit chunk bounds the work memory, so P then holds one chunk.
+  const bool p_all = ci <= 0 && (double)(nv * nv * nE * no) * sizeof(double) <= 32.0 * (1ull << 30);
+  if (ci <= 0) ci = eos_chunk(nv, no, nE, vE, p_all);
   ci = std::min(ci, no);
-  const EosPlan pl = eos_plan(nv, no, nE, vE, nba, ci);
+  const EosPlan pl = eos_plan(nv, no, nE, vE, nba, ci, p_all);
   unsigned char* w = static_cast<unsigned char*>(c->eos->work.ensure(pl.bytes));
   double* T = reinterpret_cast<double*>(w + pl.off[0]);
   double* P = reinterpret_cast<double*>(w + pl.off[1]);
@@ -503,10 +507,24 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       g.sA1 = nv2 * nj;
       g.B = Uo5 + i0;
       g.ldb = no;
-      g.C = P + nv2 * nE * i0;
+      g.C = p_all ? P + nv2 * nE * i0 : P;
       g.ldc = nv2;
       g.sC1 = nv2 * nE;
       g.Z1 = cz;
+      run_gemm(s, g);
+    }
+    if (!p_all) {  // toccEOS, occupied I: S(A,B,Jl,Il) += sum_I P(A,B,Jl,I) UoccEOST(I,Il)
+      Gemm g;
+      g.M = nv2 * nE;
+      g.N = nE;
+      g.K = cz;
+      g.A = P;
+      g.lda = nv2 * nE;
+      g.B = Uo5 + i0;
+      g.ldb = no;
+      g.beta = i0 > 0 ? 1.0 : 0.0;
+      g.C = S;
+      g.ldc = nv2 * nE;
       run_gemm(s, g);
     }
     {  // tvirtEOS, virtual A: Q(Al, B, J, I) = sum_A UvirtEOST(A,Al) t(A,B,J,I)
@@ -543,7 +561,7 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
       run_gemm(s, g);
     }
   }
-  {  // toccEOS, occupied I: S(A,B,Jl,Il) = sum_I P(A,B,Jl,I) UoccEOST(I,Il)
+  if (p_all) {  // toccEOS, occupied I: S(A,B,Jl,Il) = sum_I P(A,B,Jl,I) UoccEOST(I,Il)
     Gemm g;
     g.M = nv2 * nE;
     g.N = nE;
