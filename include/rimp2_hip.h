@@ -233,7 +233,8 @@ int rimp2_gen_batch_shapes(uint64_t seed, int64_t nfrag, int64_t* nocc, int64_t*
  * binding's handle + cublasSetStream pair becomes the context).  transa /
  * transb: 0 = N, 1 = T (2 = C, the same for real data; 'N'/'T'/'C' also
  * accepted).  alpha and beta are host pointers (cuBLAS host pointer mode);
- * beta = 0 does not read C.  Leading dimensions are checked as BLAS does. */
+ * beta = 0 does not read C, alpha = 0 does not read A or B.  Leading
+ * dimensions are checked as BLAS does. */
 int rimp2_dgemm(void* ctx, int transa, int transb, int64_t m, int64_t n, int64_t k, const double* alpha,
                 const double* d_A, int64_t lda, const double* d_B, int64_t ldb, const double* beta, double* d_C,
                 int64_t ldc);

@@ -444,7 +444,6 @@ int64_t eos_chunk(int64_t nv, int64_t no, int64_t nE, int64_t vE, bool p_all) {
 
 }  // namespace
 
-
 // Queues Steps 5-8 on c->stream (device arrays; outputs in the reference's
 // layouts, RIMP2.F90:92,123,146,169).
 void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, int64_t nba, const double* Cal,
@@ -842,7 +841,8 @@ int rimp2_dgemm(void* ctx, int transa, int transb, int64_t m, int64_t n, int64_t
          (long long)ldc);
   if (m == 0 || n == 0) return RIMP2_OK;
   if (!d_C || (k > 0 && (!d_A || !d_B))) fail(RIMP2_ERR_ARG, "null device pointer");
-  dgemm_device(c->stream, ta, tb, m, n, k, *alpha, d_A, lda, d_B, ldb, *beta, d_C, ldc);
+  // alpha = 0: A and B are not read (BLAS), so a NaN there cannot reach C
+  dgemm_device(c->stream, ta, tb, m, n, *alpha == 0.0 ? 0 : k, *alpha, d_A, lda, d_B, ldb, *beta, d_C, ldc);
   RIMP2_API_END(ctx)
 }
 

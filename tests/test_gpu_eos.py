@@ -197,6 +197,23 @@ def test_dgemm_matches_numpy(ctx, ta, tb, m, n, k, pad):
         ctx.free(dB)
 
 
+def test_dgemm_alpha_zero_reads_neither_operand(ctx):
+    """BLAS semantics: alpha = 0 gives beta C without touching A or B (NaN there)."""
+    m, n, k = 33, 17, 9
+    A = np.full((m, k), np.nan, order="F")
+    B = np.full((k, n), np.nan, order="F")
+    C0 = np.asfortranarray(np.random.default_rng(3).standard_normal((m, n)))
+    dA, dB, dC = ctx.to_device(A), ctx.to_device(B), ctx.to_device(C0)
+    try:
+        ctx.dgemm(0, 0, m, n, k, 0.0, dA, m, dB, k, 0.5, dC, m)
+        ctx.sync()
+        got = ctx.from_device(dC, (m, n))
+    finally:
+        for p in (dA, dB, dC):
+            ctx.free(p)
+    np.testing.assert_array_equal(got, 0.5 * C0)
+
+
 def test_dgemm_rejects_bad_arguments(ctx, rimp2):
     p = ctx.alloc(64 * 8)
     try:
