@@ -62,7 +62,8 @@ def test_eos_reference_fixture(ctx, dims, expect):
 
 # shapes: the reference's two fixture sizes, a one-orbital case, ragged
 # tiles (65 = 64 + 1 virtuals), full EOS spaces (noccEOS = nocc,
-# nvirtEOS = nvirt); chunk = occupied orbitals per amplitude slab (0: auto)
+# nvirtEOS = nvirt), more than 8 virtual blocks; chunk = occupied orbitals
+# per amplitude slab (0: automatic, the whole-P form; > 0: per-chunk P)
 SHAPES = [
     ((74, 10, 4, 15, 273), 0),
     ((74, 10, 4, 15, 273), 1),
@@ -71,6 +72,9 @@ SHAPES = [
     ((1, 1, 1, 1, 1), 0),
     ((65, 3, 2, 64, 17), 2),
     ((33, 7, 7, 33, 130), 0),
+    # 520 virtuals = 9 blocks of 64: the pair kernel's super-tile unit order
+    ((520, 6, 3, 10, 64), 0),
+    ((520, 6, 3, 10, 64), 4),
 ]
 
 
