@@ -331,8 +331,6 @@ def workload_eos(r, steps, warmup, cpu=True):
     ctx.timing(False)
     outs = [ctx.from_device(p, s) for p, s in zip(d_out, shapes)]
     rel = [float(np.max(np.abs(o - e)) / abs(e)) for o, e in zip(outs, expect)]
-    for p in d_out:
-        ctx.free(p)
     per_launch_s = ampl_ms / max(ampl_n, 1) / 1e3
     launches = ampl_n / steps
     assert launches == -(-nocc // chunk), (launches, chunk)
@@ -346,6 +344,20 @@ def workload_eos(r, steps, warmup, cpu=True):
                max_rel_err_vs_closed_form=max(rel))
     log(f"[bench] eos: {t * 1e3:.2f} ms/call, {F / t / 1e12:.2f} TFLOP/s; amplitude slab "
         f"{per_launch_s * 1e3:.3f} ms x {launches:.0f}, {res['amplitude']['tflops']:.2f} TFLOP/s; rel err {rel}")
+    # the same call with the exact-slice int8 amplitude slab (opt-in mode)
+    ctx.eos_set_mode("i8s5")
+    ctx.eos_arrays_device(*dims, d_in, d_out)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.eos_arrays_device(*dims, d_in, d_out)
+    ctx.sync()
+    t_i8 = (time.perf_counter() - t0) / steps
+    ctx.eos_set_mode("fp64")
+    outs8 = [ctx.from_device(p, s) for p, s in zip(d_out, shapes)]
+    rel8 = [float(np.max(np.abs(o - e)) / abs(e)) for o, e in zip(outs8, expect)]
+    res["i8s5_slab"] = dict(ms_per_call=t_i8 * 1e3, speedup_vs_fp64=t / t_i8, max_rel_err_vs_closed_form=max(rel8))
+    log(f"[bench] eos i8s5 slab: {t_i8 * 1e3:.2f} ms/call ({t / t_i8:.2f}x), rel err {max(rel8):.2e}")
     # first-order intermediates at the same size (RIMP2.F90:172-251); the
     # extra 3-index inputs CalphaOcc(NBA,nocc,nocc) and CalphaVV(NBA,nvirt,nvirt)
     # come from the device generator (timing only: parity is in the tests)
@@ -363,6 +375,8 @@ def workload_eos(r, steps, warmup, cpu=True):
     ctx.sync()
     t_fo = (time.perf_counter() - t0) / steps
     for p in (d_co, d_cvv, d_dj, d_bl):
+        ctx.free(p)
+    for p in d_out:
         ctx.free(p)
     F_fo = r.first_order_flops(nvirt, nocc, noccEOS, nvirtEOS, nba)
     res["first_order"] = dict(ms_per_call=t_fo * 1e3, flops=F_fo, tflops=F_fo / t_fo / 1e12)

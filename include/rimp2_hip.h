@@ -283,6 +283,13 @@ int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noc
  * chunk.  Has no counterpart in the reference, which holds every amplitude at
  * once. */
 int rimp2_eos_set_chunk(void* ctx, int64_t nocc_per_chunk);
+/* Precision of rimp2_eos_arrays' amplitude slab V^{IJ}_AB (the rest is FP64
+ * either way): RIMP2_MODE_FP64 (default; the reference's arithmetic) or
+ * RIMP2_MODE_I8S5 (the exact-slice int8 pair kernel, ~2x faster, V to ~1e-12
+ * relative: within the reference's 1e-10 absolute check for arrays of O(1)
+ * values, not for its constant fixtures of values ~10^2).  Other modes return
+ * RIMP2_ERR_ARG.  No counterpart in the reference, which is FP64 only. */
+int rimp2_eos_set_mode(void* ctx, int mode);
 
 #ifdef __cplusplus
 }

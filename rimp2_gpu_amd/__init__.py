@@ -99,6 +99,7 @@ SIGNATURES = {
     "rimp2_eos_arrays": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_dp] * 11),
     "rimp2_eos_arrays_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64] + [_vp] * 11),
     "rimp2_eos_set_chunk": (ctypes.c_int, [_vp, _i64]),
+    "rimp2_eos_set_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rimp2_first_order": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64, _i64] + [_dp] * 9),
     "rimp2_first_order_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _i64, _i64, _i64] + [_vp] * 9),
 }
@@ -428,6 +429,10 @@ class Context:
         """ptrs: Calpha, CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT,
         UvirtEOST, djik, blad as device pointers (asynchronous)."""
         self._chk(lib().rimp2_first_order_device(self.h, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, *ptrs))
+
+    def eos_set_mode(self, mode):
+        """Amplitude slab precision of eos_arrays: "fp64" (default) or "i8s5"."""
+        self._chk(lib().rimp2_eos_set_mode(self.h, _mode(mode)))
 
     def eos_set_chunk(self, nocc_per_chunk):
         self._chk(lib().rimp2_eos_set_chunk(self.h, int(nocc_per_chunk)))

@@ -30,7 +30,7 @@
 namespace rimp2 {
 
 struct EosState {
-  DevBuf in, work, out, fo, pk;
+  DevBuf in, work, out, fo, pk, sg;
 };
 void eos_destroy(EosState* s) {
   if (!s) return;
@@ -39,6 +39,7 @@ void eos_destroy(EosState* s) {
   s->out.release();
   s->fo.release();
   s->pk.release();
+  s->sg.release();
   delete s;
 }
 
@@ -471,11 +472,17 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
   ScopedTimer tm(c, RIMP2_K_EOS);
   const int64_t nv2 = nv * nv;
 
-  // The 3-index tensor in the FP64 pair kernel's operand order (64-wide
-  // virtual blocks), once per call.
-  const PackGeom pg = make_geom(RIMP2_MODE_FP64, nba, nv, 64);
+  // The 3-index tensor in the pair kernel's operand order (64-wide virtual
+  // blocks), once per call: FP64, or exact-slice int8 with row scales (int32
+  // slice sums exact for NBA <= 16384: 5 * 127^2 * 16511 < 2^31).
+  if (c->eos_mode == RIMP2_MODE_I8S5 && nba > 16384)
+    fail(RIMP2_ERR_ARG, "exact-slice amplitude slab supports NBA <= 16384 (int32 accumulation bound)");
+  const PackGeom pg = make_geom(c->eos_mode, nba, nv, 64);
   void* pk = c->eos->pk.ensure(pg.bytes_per_occ() * (size_t)no);
-  pack_B(c, Cal, nba, nv, no, pg, pk, nullptr, s);
+  double* sigma = c->eos_mode == RIMP2_MODE_I8S5
+                      ? static_cast<double*>(c->eos->sg.ensure((size_t)(no * pg.nvp) * sizeof(double)))
+                      : nullptr;
+  pack_B(c, Cal, nba, nv, no, pg, pk, sigma, s);
   // Each unordered occupied pair once: t(A,B,J,I) = t(B,A,I,J), so a chunk
   // of I builds J >= i0 only, weighted by w_IJ (1 above the diagonal, 1/2 on
   // it, 0 below).  Any 4-index transform X of the weighted half satisfies
@@ -494,7 +501,7 @@ void eos_arrays_device(Ctx* c, int64_t nv, int64_t no, int64_t nE, int64_t vE, i
         hipLaunchKernelGGL(zero_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, s, T + nv2 * nj * z, n);
         RIMP2_HIP_CHECK(hipGetLastError());
       }
-      pair_amplitudes_device(s, pg, pk, no, nv, eo, ev, i0, cz, T);
+      pair_amplitudes_device(s, pg, pk, sigma, no, nv, eo, ev, i0, cz, T);
     }
     {  // toccEOS, occupied J: P(A,B,Jl,I) = sum_J t(A,B,J,I) UoccEOST(J,Jl)
       Gemm g;
@@ -958,6 +965,15 @@ int rimp2_first_order(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, i
     RIMP2_HIP_CHECK(hipMemcpyAsync(hout[i], o[i], (size_t)nout[i] * sizeof(double), hipMemcpyDeviceToHost,
                                    c->stream));
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_eos_set_mode(void* ctx, int mode) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (mode != RIMP2_MODE_FP64 && mode != RIMP2_MODE_I8S5)
+    fail(RIMP2_ERR_ARG, "4-index amplitude mode %d: only RIMP2_MODE_FP64 and RIMP2_MODE_I8S5", mode);
+  c->eos_mode = mode;
   RIMP2_API_END(ctx)
 }
 

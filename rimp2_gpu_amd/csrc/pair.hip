@@ -257,23 +257,27 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
-// Amplitude slab of one chunk (FP64 pair kernel, 64-wide virtual blocks) on
+// Amplitude slab of one chunk (pair kernel, 64-wide virtual blocks) on
 // stream s: T[a + nv b + nv^2 (j - i0) + nv^2 (nocc - i0) (i - i0)] =
 // w_ij V^{ij}_ab / Delta for i in [i0, i0 + nchunk), j >= i (entries j < i
-// are not written).  packed: FP64 packing of the 3-index tensor, geometry g
-// (tile 64).
-void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed, int64_t nocc, int64_t nvirt,
-                            const double* d_eo, const double* d_ev, int64_t i0, int64_t nchunk, double* T) {
+// are not written).  packed: the FP64 or exact-slice i8s5 packing of the
+// 3-index tensor, geometry g (tile 64); sigma: the i8s5 row scales.
+void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed, const double* sigma, int64_t nocc,
+                            int64_t nvirt, const double* d_eo, const double* d_ev, int64_t i0, int64_t nchunk,
+                            double* T) {
   constexpr int TM = 64;
-  if (g.mode != RIMP2_MODE_FP64 || g.nvp % TM) fail(RIMP2_ERR_ARG, "amplitude slab needs the FP64 64-wide packing");
+  if ((g.mode != RIMP2_MODE_FP64 && g.mode != RIMP2_MODE_I8S5) || g.nvp % TM)
+    fail(RIMP2_ERR_ARG, "amplitude slab needs the FP64 or i8s5 64-wide packing");
+  if (g.mode == RIMP2_MODE_I8S5 && !sigma) fail(RIMP2_ERR_ARG, "i8s5 amplitude slab without row scales");
   if (i0 < 0 || nchunk < 1 || i0 + nchunk > nocc) fail(RIMP2_ERR_ARG, "bad occupied chunk");
   const int64_t nb = g.nvp / TM;
-  const int64_t ntp = pair_units(nb, packs_diagonal<PolF64, TM>());
+  const bool i8 = g.mode == RIMP2_MODE_I8S5;
+  const int64_t ntp = pair_units(nb, i8 ? packs_diagonal<PolI8<5, 4>, TM>() : packs_diagonal<PolF64, TM>());
   const int64_t nj = nocc - i0;
   const int64_t npairs = nchunk * nj - nchunk * (nchunk - 1) / 2;
   PairParams p;
   p.f.packed = static_cast<const unsigned char*>(packed);
-  p.f.sigma = nullptr;
+  p.f.sigma = sigma;
   p.f.eps_o = d_eo;
   p.f.eps_v = d_ev;
   p.f.nvp = g.nvp;
@@ -289,8 +293,12 @@ void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed
   p.chunk = ceil_div(p.nwu, 8);
   AmplOut o{T, d_eo, d_ev, nvirt, nj, i0};
   const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
-  hipLaunchKernelGGL((pair_ampl_kernel<PolF64, TM>), dim3((unsigned)(8 * nslot)), dim3(PAIR_THREADS), 0, s, p, o,
-                     nchunk);
+  if (i8)
+    hipLaunchKernelGGL((pair_ampl_kernel<PolI8<5, 4>, TM>), dim3((unsigned)(8 * nslot)), dim3(PAIR_THREADS), 0, s, p,
+                       o, nchunk);
+  else
+    hipLaunchKernelGGL((pair_ampl_kernel<PolF64, TM>), dim3((unsigned)(8 * nslot)), dim3(PAIR_THREADS), 0, s, p, o,
+                       nchunk);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

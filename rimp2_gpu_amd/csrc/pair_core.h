@@ -160,7 +160,7 @@ struct FragView {
   int64_t nvp, nks, nrb;
 };
 
-// Output of the amplitude form of the pair kernel (FP64): for the unordered
+// Output of the amplitude form of the pair kernel: for the unordered
 // occupied pair i <= j of a chunk i in [i0, i0 + nchunk), every element
 //   T[a + nv b + nv^2 (j - i0) + nv^2 nj (i - i0)] = w_ij V^{ij}_ab / (e_i + e_j - e_b - e_a)
 // with w_ij = 1 (i < j), 1/2 (i = j), a, b < nv (padding rows are not written).
@@ -473,7 +473,6 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   RIMP2_WAIT_VMCNT_LGKM0(0);  // trailing re-fetches and the unused last read are done before LDS is reused
 
   if constexpr (AMPL) {
-    static_assert(!is_i8<Pol>::value, "the amplitude form applies no exact-slice row scales");
     // ---- epilogue: amplitudes to global memory (registers only) ----
     // tile pair: tile 1 holds V^{ij}[A,B] (a, b); tile 2 holds V^{ji}[A,B]
     // (a, b) = V^{ij}(b, a), written transposed.  Diagonal units: each wave
@@ -495,10 +494,17 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int64_t al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int64_t bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            const double v = Pol::value(acc[x][y], r);
-            if (!tile_pair) put(X * TM + al, X * TM + bl, v);
-            else if (grp == 0) put(A * TM + al, Bk * TM + bl, v);
-            else put(Bk * TM + bl, A * TM + al, v);
+            double v = Pol::value(acc[x][y], r);
+            if (!tile_pair) {
+              if (is_i8<Pol>::value) v *= f.sigma[i * f.nvp + X * TM + al] * f.sigma[j * f.nvp + X * TM + bl];
+              put(X * TM + al, X * TM + bl, v);
+            } else if (grp == 0) {
+              if (is_i8<Pol>::value) v *= f.sigma[i * f.nvp + A * TM + al] * f.sigma[j * f.nvp + Bk * TM + bl];
+              put(A * TM + al, Bk * TM + bl, v);
+            } else {
+              if (is_i8<Pol>::value) v *= f.sigma[j * f.nvp + A * TM + al] * f.sigma[i * f.nvp + Bk * TM + bl];
+              put(Bk * TM + bl, A * TM + al, v);
+            }
           }
     }
     __syncthreads();  // every wave's LDS reads are done before the next unit's DMA

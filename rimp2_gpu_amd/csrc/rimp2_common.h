@@ -105,6 +105,7 @@ struct Ctx {
   // chunk (0: sized automatically)
   EosState* eos = nullptr;
   int64_t eos_chunk = 0;
+  int eos_mode = RIMP2_MODE_FP64;  // amplitude slab precision: FP64 or I8S5
 };
 
 // Records hipEvents around a launch on ctx->stream when timing is on (no-op
@@ -186,11 +187,13 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
                           int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
                           double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
-// Amplitude slab of occupied orbitals [i0, i0 + nchunk) from the FP64 packing
-// (tile 64): T[a + nv b + nv^2 (j - i0) + nv^2 (nocc - i0) (i - i0)] =
+// Amplitude slab of occupied orbitals [i0, i0 + nchunk) from the FP64 or
+// i8s5 packing (tile 64; sigma: i8s5 row scales, else null):
+// T[a + nv b + nv^2 (j - i0) + nv^2 (nocc - i0) (i - i0)] =
 // w_ij V^{ij}_ab / (e_i + e_j - e_b - e_a), j >= i, w = 1 (i < j), 1/2 (i = j).
-void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed, int64_t nocc, int64_t nvirt,
-                            const double* d_eo, const double* d_ev, int64_t i0, int64_t nchunk, double* T);
+void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed, const double* sigma, int64_t nocc,
+                            int64_t nvirt, const double* d_eo, const double* d_ev, int64_t i0, int64_t nchunk,
+                            double* T);
 // Status bits on a given stream: 4 = a non-finite value among p[0..n);
 // 8 = max(eo) >= min(ev) (some pair denominator >= 0).
 void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n, unsigned max_blocks = 8192);

@@ -90,6 +90,31 @@ def test_eos_random_vs_oracle(ctx, oracle, dims, chunk):
     check_arrays(got, oracle.steps5to8(*args))
 
 
+# Exact-slice amplitude slab (rimp2_eos_set_mode i8s5): V to ~1e-12 relative;
+# on arrays of O(1) values the reference's 1e-10 absolute check still holds.
+@pytest.mark.parametrize("dims", [(74, 10, 4, 15, 273), (104, 20, 4, 15, 373), (520, 6, 3, 10, 64)])
+def test_eos_i8s5_slab_vs_oracle(ctx, oracle, dims):
+    say(f"eos i8s5 {dims}")
+    args = random_inputs(23 + sum(dims), *dims)
+    ctx.eos_set_mode("i8s5")
+    try:
+        got = ctx.eos_arrays(*args)
+    finally:
+        ctx.eos_set_mode("fp64")
+    for name, g, e in zip(("toccEOS", "tvirtEOS", "goccEOS", "gvirtEOS"), got, oracle.steps5to8(*args)):
+        err, scale = float(np.max(np.abs(g - e))), float(np.max(np.abs(e)))
+        say(f"  {name}: max|d| = {err:.3e} (max|ref| {scale:.3e})")
+        assert err <= 1e-10, (name, err)
+        assert err <= 1e-9 * scale, (name, err, scale)
+
+
+def test_eos_mode_rejects_others(ctx, rimp2):
+    for m in ("fp32", "bf16x3", "i8s4"):
+        with pytest.raises(rimp2.Rimp2Error) as e:
+            ctx.eos_set_mode(m)
+        assert e.value.code == rimp2.ERR_ARG
+
+
 def test_eos_device_entry_matches_host_entry(ctx):
     dims = (74, 10, 4, 15, 273)
     nvirt, nocc, noccEOS, nvirtEOS, nba = dims
