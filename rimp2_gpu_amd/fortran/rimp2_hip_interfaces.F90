@@ -126,6 +126,21 @@ module rimp2_hip_interfaces
        real(C_DOUBLE) :: toccEOS(*), tvirtEOS(*), goccEOS(*), gvirtEOS(*)
      end function rimp2_eos_arrays
 
+     !> Occupied orbitals per amplitude chunk of rimp2_eos_arrays (0: automatic).
+     integer(C_INT) function rimp2_eos_set_chunk(ctx, nocc_per_chunk) bind(C, name="rimp2_eos_set_chunk")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nocc_per_chunk
+     end function rimp2_eos_set_chunk
+
+     !> Amplitude slab precision of rimp2_eos_arrays: RIMP2_MODE_FP64 (default)
+     !> or RIMP2_MODE_I8S5.
+     integer(C_INT) function rimp2_eos_set_mode(ctx, mode) bind(C, name="rimp2_eos_set_mode")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx
+       integer(C_INT), value :: mode
+     end function rimp2_eos_set_mode
+
      !> First-order block (RIMP2.F90:172-251) on the GPU, host arrays, blocking.
      integer(C_INT) function rimp2_first_order(ctx, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, Calpha, &
           & CalphaOcc, CalphaVV, UoccEOST, UvirtT, UoccT, UvirtEOST, djik, blad) bind(C, name="rimp2_first_order")
