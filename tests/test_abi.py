@@ -33,7 +33,8 @@ def test_fortran_interfaces_cover_core_api():
     f = open(os.path.join(ROOT, "rimp2_gpu_amd", "fortran", "rimp2_hip_interfaces.F90")).read()
     for s in ["rimp2_ctx_create", "rimp2_ctx_destroy", "rimp2_get_dev_mem", "rimp2_metric", "rimp2_transform",
               "rimp2_pair_energies", "rimp2_energy", "rimp2_last_error", "rimp2_batch", "rimp2_load_fitted",
-              "rimp2_energy_fitted"]:
+              "rimp2_energy_fitted", "rimp2_dgemm", "rimp2_eos_arrays", "rimp2_eos_set_chunk", "rimp2_eos_set_mode",
+              "rimp2_first_order"]:
         assert f'name="{s}"' in f, s
 
 
