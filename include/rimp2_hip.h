@@ -56,7 +56,7 @@ extern "C" {
 #define RIMP2_K_BATCH 6
 #define RIMP2_K_MLP 7
 #define RIMP2_K_FEATURES 8
-#define RIMP2_K_EOS 9        /* a whole rimp2_eos_arrays call */
+#define RIMP2_K_EOS 9        /* a whole rimp2_eos_arrays or rimp2_first_order call */
 #define RIMP2_K_EOS_AMPL 10  /* its amplitude GEMMs (1/Delta epilogue) */
 #define RIMP2_NUM_KERNELS 11
 
