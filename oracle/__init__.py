@@ -1,8 +1,8 @@
 """CPU FP64 oracle for the RI-MP2 hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
-this package, and only as the checker (or the timed CPU baseline); the
-product (rimp2_gpu_amd) never imports it.
+this package, and only to compare the GPU results against (or as the timed
+CPU baseline); the product (rimp2_gpu_amd) never imports it.
 
 Functions follow the reference (januseriksen/rimp2_gpu) line by line as cited
 in rimp2_oracle.c; the MLP oracle follows mlp_oracle.c.

@@ -1,9 +1,10 @@
 /*
  * rimp2_oracle.c -- CPU FP64 restatement of the reference RI-MP2 algorithm.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
- * check and the CPU-baseline leg of bench.py may call it; the product library
- * (rimp2_gpu_amd) never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the GPU results are
+ * compared with: tests/, the smoke check and the CPU-baseline leg of bench.py
+ * may call it; the product library (rimp2_gpu_amd) never links, loads or
+ * calls it.
  *
  * Reference files restated here (januseriksen/rimp2_gpu):
  *   RIMP2.F90:381-417  RIMP2_calc_toccA   -> oracle_toccA

@@ -2,8 +2,9 @@
 // (GELU, erf form) with hand-written forward / backward / Adam kernels, and the
 // screened energy (pairs predicted below a threshold skip the exact kernel).
 //
-// Nothing here exists in the reference; the exact pair kernel stays the judge:
-// screened energies are always the exact kernel's energies on the kept pairs.
+// Nothing here exists in the reference; the exact pair kernel has the last
+// word: screened energies are always the exact kernel's energies on the kept
+// pairs.
 //
 // Features of a unique pair (i <= j), from the FP64 B of the last transform
 // (r_i[Q] = ||B[Q,:,i]||, the row norm of occupied i at auxiliary index Q;

@@ -104,7 +104,7 @@ def test_features_match_numpy(ctx, rimp2):
             k += 1
 
 
-def test_screened_energy_is_judged_by_the_exact_kernel(ctx, rimp2):
+def test_screened_energy_comes_from_the_exact_kernel(ctx, rimp2):
     """Locality-decay fragments (labelled variant of the generator): train on
     12 fragments, screen a 13th; kept pairs carry the exact kernel's energies."""
     from rimp2_gpu_amd import surrogate
