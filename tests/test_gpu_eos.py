@@ -150,7 +150,8 @@ def test_eos_rejects_bad_dimensions(ctx, rimp2):
 # first-order djik / blad (RIMP2.F90:172-251) against the oracle, which keeps
 # the reference's K = ldb = nocc CalphaVV product (RIMP2.F90:231-235)
 # ---------------------------------------------------------------------------
-FO_SHAPES = [(74, 10, 4, 15, 273), (104, 20, 4, 15, 373), (1, 1, 1, 1, 1), (65, 33, 2, 64, 17)]
+# (6, 6, ...): nocc = nvirt, where the K = ldb = nocc product reads UvirtT up to its last element
+FO_SHAPES = [(74, 10, 4, 15, 273), (104, 20, 4, 15, 373), (1, 1, 1, 1, 1), (65, 33, 2, 64, 17), (6, 6, 2, 3, 7)]
 
 
 @pytest.mark.parametrize("dims", FO_SHAPES)
