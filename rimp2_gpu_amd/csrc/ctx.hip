@@ -115,8 +115,8 @@ static void check_status(Ctx* c) {
   RIMP2_HIP_CHECK(hipMemcpyAsync(st, d, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (st[0]) RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+  if (st[0] & 4) fail(RIMP2_ERR_NUMERIC, "non-finite value in an input array");
   if (st[0] & 2) fail(RIMP2_ERR_NOT_SPD, "metric is not positive definite (pivot %d <= 0)", st[1]);
-  if (st[0] & 4) fail(RIMP2_ERR_NUMERIC, "non-finite value in a device input");
   if (st[0] & 8) fail(RIMP2_ERR_NUMERIC, "orbital-energy denominator e_i+e_j-e_a-e_b >= 0");
 }
 
@@ -171,8 +171,10 @@ static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* 
 
 // Already-fitted coefficients (B = Calpha, no metric) from a device array.
 // A factored metric of another naux no longer matches the context.
+// With check_now, the copied coefficients are scanned on the device and a
+// non-finite value fails the call before anything is packed.
 static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc, const double* dB, int mode,
-                               hipMemcpyKind kind) {
+                               hipMemcpyKind kind, bool check_now = false) {
   check_slice_bound(naux, mode);
   c->have_B = false;
   if (naux != c->naux) c->have_metric = false;
@@ -180,6 +182,10 @@ static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc
   const size_t bytes = (size_t)(naux * nvirt * nocc) * sizeof(double);
   double* B = static_cast<double*>(c->B64.ensure(bytes));
   RIMP2_HIP_CHECK(hipMemcpyAsync(B, dB, bytes, kind, c->stream));
+  if (check_now) {
+    check_inputs_device(c, B, naux * nvirt * nocc, "Calpha");
+    check_status(c);
+  }
   pack_current_B(c, nvirt, nocc, mode);
 }
 
@@ -277,12 +283,15 @@ int rimp2_metric(void* ctx, int64_t naux, const double* V) {
   Ctx* c = as_ctx(ctx);
   if (naux <= 0 || naux > 65536) fail(RIMP2_ERR_ARG, "naux=%lld out of range", (long long)naux);
   if (!V) fail(RIMP2_ERR_ARG, "null V");
-  check_host_finite(V, naux * naux, "V");
   c->have_metric = false;
   c->have_B = false;
   reset_status(c);
   double* d = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
   RIMP2_HIP_CHECK(hipMemcpyAsync(d, V, (size_t)(naux * naux) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // host arrays are scanned for NaN/Inf on the device once copied: a NaN
+  // never reaches the factorization
+  check_inputs_device(c, d, naux * naux, "V");
+  check_status(c);
   metric_factor(c, naux, d);
   check_status(c);
   c->have_metric = true;
@@ -297,9 +306,11 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
   if (!Pia) fail(RIMP2_ERR_ARG, "null Pia");
   if (!c->have_metric || c->naux != naux) fail(RIMP2_ERR_STATE, "rimp2_metric with naux=%lld must run first", (long long)naux);
   const int64_t n = naux * nvirt * nocc;
-  check_host_finite(Pia, n, "Pia");
+  reset_status(c);
   double* dX = static_cast<double*>(c->X.ensure((size_t)n * sizeof(double)));
   RIMP2_HIP_CHECK(hipMemcpyAsync(dX, Pia, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  check_inputs_device(c, dX, n, "Pia");
+  check_status(c);  // a rejected (P|ia) leaves the coefficients of the last call in place
   transform_device(c, nvirt, nocc, dX, mode);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   RIMP2_API_END(ctx)
@@ -311,8 +322,8 @@ int rimp2_load_fitted(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, cons
   validate_dims(naux, nvirt, nocc);
   validate_mode(mode);
   if (!Calpha) fail(RIMP2_ERR_ARG, "null Calpha");
-  check_host_finite(Calpha, naux * nvirt * nocc, "Calpha");
-  load_fitted_device(c, naux, nvirt, nocc, Calpha, mode, hipMemcpyHostToDevice);
+  reset_status(c);
+  load_fitted_device(c, naux, nvirt, nocc, Calpha, mode, hipMemcpyHostToDevice, true);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   RIMP2_API_END(ctx)
 }
