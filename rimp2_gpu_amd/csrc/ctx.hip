@@ -1,5 +1,8 @@
 // C-ABI entry points: context lifetime, device-memory query, errors, timers,
 // and the blocking / device-resident RI-MP2 energy path.
+#include <omp.h>
+
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -100,6 +103,41 @@ static void check_host_finite(const double* x, int64_t n, const char* what) {
     if (!std::isfinite(x[k])) fail(RIMP2_ERR_NUMERIC, "%s[%lld] is not finite", what, (long long)k);
 }
 
+constexpr size_t kPinChunk = (size_t)16 << 20;
+
+static void parallel_memcpy(char* d, const char* s, size_t n) {
+  const int T = std::max(1, std::min(omp_get_max_threads(), (int)(n >> 20)));  // >= 1 MiB per thread
+  const size_t piece = (n + T - 1) / T;
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (int t = 0; t < T; t++) {
+    const size_t o = (size_t)t * piece;
+    if (o < n) memcpy(d + o, s + o, std::min(piece, n - o));
+  }
+}
+
+void h2d_copy(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes < kPinChunk / 2) {
+    RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return;
+  }
+  for (int b = 0; b < 2; b++) {
+    if (!c->pin[b]) RIMP2_HIP_CHECK(hipHostMalloc((void**)&c->pin[b], kPinChunk, hipHostMallocDefault));
+    if (!c->pin_ev[b]) RIMP2_HIP_CHECK(hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
+  }
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, k++) {
+    const int b = (int)(k & 1);
+    const size_t n = std::min(kPinChunk, bytes - off);
+    if (c->pin_busy[b]) RIMP2_HIP_CHECK(hipEventSynchronize(c->pin_ev[b]));
+    c->pin_busy[b] = false;
+    parallel_memcpy(c->pin[b], s + off, n);
+    RIMP2_HIP_CHECK(hipMemcpyAsync(d + off, c->pin[b], n, hipMemcpyHostToDevice, c->stream));
+    RIMP2_HIP_CHECK(hipEventRecord(c->pin_ev[b], c->stream));
+    c->pin_busy[b] = true;
+  }
+}
+
 static void check_host_denominators(const double* eo, int64_t nocc, const double* ev, int64_t nvirt) {
   double mo = -INFINITY, mv = INFINITY;
   for (int64_t i = 0; i < nocc; i++) mo = std::fmax(mo, eo[i]);
@@ -181,7 +219,8 @@ static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc
   c->naux = naux;
   const size_t bytes = (size_t)(naux * nvirt * nocc) * sizeof(double);
   double* B = static_cast<double*>(c->B64.ensure(bytes));
-  RIMP2_HIP_CHECK(hipMemcpyAsync(B, dB, bytes, kind, c->stream));
+  if (kind == hipMemcpyHostToDevice) h2d_copy(c, B, dB, bytes);
+  else RIMP2_HIP_CHECK(hipMemcpyAsync(B, dB, bytes, kind, c->stream));
   if (check_now) {
     check_inputs_device(c, B, naux * nvirt * nocc, "Calpha");
     check_status(c);
@@ -242,6 +281,10 @@ int rimp2_ctx_destroy(void* ctx) {
                     &c->partials, &c->eps_pair, &c->etot, &c->status})
     b->release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (int b = 0; b < 2; b++) {
+    if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+    if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
+  }
   destroy_streams(c);
   delete c;
   RIMP2_API_END(nullptr)
@@ -287,7 +330,7 @@ int rimp2_metric(void* ctx, int64_t naux, const double* V) {
   c->have_B = false;
   reset_status(c);
   double* d = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
-  RIMP2_HIP_CHECK(hipMemcpyAsync(d, V, (size_t)(naux * naux) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  h2d_copy(c, d, V, (size_t)(naux * naux) * sizeof(double));
   // host arrays are scanned for NaN/Inf on the device once copied: a NaN
   // never reaches the factorization
   check_inputs_device(c, d, naux * naux, "V");
@@ -308,7 +351,7 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
   const int64_t n = naux * nvirt * nocc;
   reset_status(c);
   double* dX = static_cast<double*>(c->X.ensure((size_t)n * sizeof(double)));
-  RIMP2_HIP_CHECK(hipMemcpyAsync(dX, Pia, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  h2d_copy(c, dX, Pia, (size_t)n * sizeof(double));
   check_inputs_device(c, dX, n, "Pia");
   check_status(c);  // a rejected (P|ia) leaves the coefficients of the last call in place
   transform_device(c, nvirt, nocc, dX, mode);
@@ -484,7 +527,7 @@ int rimp2_copy_h2d(void* ctx, void* dst, const void* src, size_t bytes) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
   if ((!dst || !src) && bytes) fail(RIMP2_ERR_ARG, "null pointer");
-  RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  h2d_copy(c, dst, src, bytes);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   RIMP2_API_END(ctx)
 }

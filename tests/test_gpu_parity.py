@@ -124,6 +124,37 @@ def test_device_resident_path_matches_host(ctx, rimp2):
         np.testing.assert_array_equal(eps_dev, eps_host)
 
 
+def test_staged_host_copies_are_exact(ctx, rimp2):
+    """Host arrays of 8 MiB and more reach the device through two pinned chunk
+    buffers: byte-exact at chunk-ragged sizes, twice in a row (buffer reuse),
+    and an energy from host arrays above the chunk size equals the one from
+    device arrays."""
+    say("staged host copies")
+    rng = np.random.default_rng(11)
+    for nbytes in (40 * 2**20 + 13, 9 * 2**20 + 1, 2**24, 4096):
+        for _ in range(2):
+            x = rng.integers(0, 256, nbytes, dtype=np.uint8)
+            d = ctx.to_device(x)
+            back = ctx.from_device(d, (nbytes,), np.uint8)
+            ctx.free(d)
+            np.testing.assert_array_equal(back, x)
+    naux, nvirt, nocc = 512, 128, 40  # (P|ia) 21 MB: two chunks, the second partial
+    P, V, eo, ev = rimp2.gen_fragment(21, naux, nvirt, nocc)
+    E_host, eps_host = ctx.energy(P, V, eo, ev, "i8s5")
+    d = [ctx.to_device(x) for x in (P, V, eo, ev)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    ctx.energy_device(naux, nvirt, nocc, *d, "i8s5", dpe, det)
+    ctx.sync()
+    E_dev = float(ctx.from_device(det, (1,))[0])
+    eps_dev = ctx.from_device(dpe, (nocc, nocc))
+    for p in d + [dpe, det]:
+        ctx.free(p)
+    assert E_dev == E_host
+    np.testing.assert_array_equal(eps_dev, eps_host)
+    E_fit, _ = ctx.energy_fitted(ctx.get_B(naux, nvirt, nocc), eo, ev, "i8s5")
+    assert E_fit == E_host
+
+
 def test_deterministic(ctx, rimp2):
     P, V, eo, ev = rimp2.gen_fragment(5, 192, 80, 7)
     say("determinism")
