@@ -425,11 +425,11 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
         double* sV = stage + so;
         double* sX = sV + d.naux * d.naux;
         so += (size_t)(d.naux * d.naux + nx);
-        h2d_copy(c, sV, d.V, (size_t)(d.naux * d.naux) * sizeof(double));
-        h2d_copy(c, sX, d.Pia, (size_t)nx * sizeof(double));
+        h2d_copy(c, c->stream, sV, d.V, (size_t)(d.naux * d.naux) * sizeof(double));
+        h2d_copy(c, c->stream, sX, d.Pia, (size_t)nx * sizeof(double));
         items.push_back(mg::Item{d.naux, d.nvirt, d.nocc, sV, sX, dst, st});
       } else {
-        h2d_copy(c, dst, d.Pia, (size_t)nx * sizeof(double));
+        h2d_copy(c, c->stream, dst, d.Pia, (size_t)nx * sizeof(double));
         copies.push_back(mg::Copy{dst, dst, nx, st});
       }
     }

@@ -115,25 +115,25 @@ static void parallel_memcpy(char* d, const char* s, size_t n) {
   }
 }
 
-void h2d_copy(Ctx* c, void* dst, const void* src, size_t bytes) {
+void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes) {
   if (bytes < kPinChunk / 2) {
-    RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return;
   }
   for (int b = 0; b < 2; b++) {
     if (!c->pin[b]) RIMP2_HIP_CHECK(hipHostMalloc((void**)&c->pin[b], kPinChunk, hipHostMallocDefault));
     if (!c->pin_ev[b]) RIMP2_HIP_CHECK(hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming));
   }
-  const char* s = static_cast<const char*>(src);
-  char* d = static_cast<char*>(dst);
+  const char* from = static_cast<const char*>(src);
+  char* to = static_cast<char*>(dst);
   for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, k++) {
     const int b = (int)(k & 1);
     const size_t n = std::min(kPinChunk, bytes - off);
     if (c->pin_busy[b]) RIMP2_HIP_CHECK(hipEventSynchronize(c->pin_ev[b]));
     c->pin_busy[b] = false;
-    parallel_memcpy(c->pin[b], s + off, n);
-    RIMP2_HIP_CHECK(hipMemcpyAsync(d + off, c->pin[b], n, hipMemcpyHostToDevice, c->stream));
-    RIMP2_HIP_CHECK(hipEventRecord(c->pin_ev[b], c->stream));
+    parallel_memcpy(c->pin[b], from + off, n);
+    RIMP2_HIP_CHECK(hipMemcpyAsync(to + off, c->pin[b], n, hipMemcpyHostToDevice, s));
+    RIMP2_HIP_CHECK(hipEventRecord(c->pin_ev[b], s));
     c->pin_busy[b] = true;
   }
 }
@@ -199,12 +199,38 @@ static void pack_current_B(Ctx* c, int64_t nvirt, int64_t nocc, int mode) {
   c->have_B = true;
 }
 
-static void transform_device(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX, int mode) {
-  validate_mode(mode);
-  if (!c->have_metric) fail(RIMP2_ERR_STATE, "rimp2_metric must run before the transform");
-  check_slice_bound(c->naux, mode);
-  metric_transform(c, nvirt, nocc, dX);
-  pack_current_B(c, nvirt, nocc, mode);
+// Starts the side stream behind everything queued so far on the context
+// stream (earlier readers of X and of the packed X, the status reset).
+static void side_after_stream(Ctx* c) {
+  RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_go, c->stream));
+  RIMP2_HIP_CHECK(hipStreamWaitEvent(c->side_pack, c->ev_pack_go, 0));
+}
+
+// (P|ia) from host memory into c->B64 = L^{-1} X, by chunks of occupied
+// orbitals (about 32 MiB each). Each chunk is copied, checked for NaN/Inf and
+// packed on the side stream (which side_after_stream has started), then
+// transformed on the context stream, behind whatever that stream holds by then
+// (the metric factorisation in rimp2_energy), while the host copies the next
+// chunk. A non-finite value is reported by the next check_status.
+static void transform_from_host(Ctx* c, int64_t nvirt, int64_t nocc, const double* Pia) {
+  const int64_t n = c->naux;
+  const int64_t per = n * nvirt;  // elements per occupied orbital
+  const int64_t step = std::max<int64_t>(1, ((int64_t)32 << 20) / (per * (int64_t)sizeof(double)));
+  double* dX = static_cast<double*>(c->X.ensure((size_t)(per * nocc) * sizeof(double)));
+  double* B = static_cast<double*>(c->B64.ensure((size_t)(per * nocc) * sizeof(double)));
+  const MetricView m = ctx_metric_view(c);
+  metric_reserve_x(m, n, nvirt, nocc);
+  int* status = static_cast<int*>(c->status.ensure(64));
+  for (int64_t i0 = 0; i0 < nocc; i0 += step) {
+    const int64_t ni = std::min(step, nocc - i0);
+    double* dXi = dX + i0 * per;
+    h2d_copy(c, c->side_pack, dXi, Pia + i0 * per, (size_t)(ni * per) * sizeof(double));
+    check_finite_on(c->side_pack, status, dXi, ni * per, 512);
+    metric_pack_x_range(m, c->side_pack, n, nvirt, i0, ni, dXi);
+    RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_done, c->side_pack));
+    RIMP2_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_pack_done, 0));
+    metric_transform_range(m, n, nvirt, i0, ni, B);
+  }
 }
 
 // Already-fitted coefficients (B = Calpha, no metric) from a device array.
@@ -219,7 +245,7 @@ static void load_fitted_device(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc
   c->naux = naux;
   const size_t bytes = (size_t)(naux * nvirt * nocc) * sizeof(double);
   double* B = static_cast<double*>(c->B64.ensure(bytes));
-  if (kind == hipMemcpyHostToDevice) h2d_copy(c, B, dB, bytes);
+  if (kind == hipMemcpyHostToDevice) h2d_copy(c, c->stream, B, dB, bytes);
   else RIMP2_HIP_CHECK(hipMemcpyAsync(B, dB, bytes, kind, c->stream));
   if (check_now) {
     check_inputs_device(c, B, naux * nvirt * nocc, "Calpha");
@@ -330,7 +356,7 @@ int rimp2_metric(void* ctx, int64_t naux, const double* V) {
   c->have_B = false;
   reset_status(c);
   double* d = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
-  h2d_copy(c, d, V, (size_t)(naux * naux) * sizeof(double));
+  h2d_copy(c, c->stream, d, V, (size_t)(naux * naux) * sizeof(double));
   // host arrays are scanned for NaN/Inf on the device once copied: a NaN
   // never reaches the factorization
   check_inputs_device(c, d, naux * naux, "V");
@@ -348,13 +374,13 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
   validate_mode(mode);
   if (!Pia) fail(RIMP2_ERR_ARG, "null Pia");
   if (!c->have_metric || c->naux != naux) fail(RIMP2_ERR_STATE, "rimp2_metric with naux=%lld must run first", (long long)naux);
-  const int64_t n = naux * nvirt * nocc;
+  check_slice_bound(naux, mode);
   reset_status(c);
-  double* dX = static_cast<double*>(c->X.ensure((size_t)n * sizeof(double)));
-  h2d_copy(c, dX, Pia, (size_t)n * sizeof(double));
-  check_inputs_device(c, dX, n, "Pia");
-  check_status(c);  // a rejected (P|ia) leaves the coefficients of the last call in place
-  transform_device(c, nvirt, nocc, dX, mode);
+  c->have_B = false;
+  side_after_stream(c);
+  transform_from_host(c, nvirt, nocc, Pia);
+  check_status(c);
+  pack_current_B(c, nvirt, nocc, mode);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   RIMP2_API_END(ctx)
 }
@@ -425,13 +451,49 @@ int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* ep
   RIMP2_API_END(ctx)
 }
 
+// rimp2_metric + rimp2_transform + rimp2_pair_energies in one queue: the
+// (P|ia) copy runs on the side stream under the Cholesky, each chunk's
+// transform under the next chunk's copy, and every input check is read once,
+// at the end, before any result is copied out.
 int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, const double* V,
                  const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot) {
-  int st = rimp2_metric(ctx, naux, V);
-  if (st) return st;
-  st = rimp2_transform(ctx, naux, nvirt, nocc, Pia, mode);
-  if (st) return st;
-  return rimp2_pair_energies(ctx, nocc, nvirt, eps_o, eps_v, mode, eps_pair, e_tot);
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  validate_dims(naux, nvirt, nocc);
+  validate_mode(mode);
+  if (!Pia || !V || !eps_o || !eps_v || !e_tot) fail(RIMP2_ERR_ARG, "null Pia / V / eps_o / eps_v / e_tot");
+  check_slice_bound(naux, mode);
+  check_host_finite(eps_o, nocc, "eps_o");
+  check_host_finite(eps_v, nvirt, "eps_v");
+  check_host_denominators(eps_o, nocc, eps_v, nvirt);
+  c->have_metric = false;
+  c->have_B = false;
+  reset_status(c);
+  // the (P|ia) chunks wait for the status reset only, not for the metric
+  side_after_stream(c);
+  double* dV = static_cast<double*>(c->V.ensure((size_t)(naux * naux) * sizeof(double)));
+  h2d_copy(c, c->stream, dV, V, (size_t)(naux * naux) * sizeof(double));
+  check_inputs_device(c, dV, naux * naux, "V");
+  metric_factor(c, naux, dV);
+  transform_from_host(c, nvirt, nocc, Pia);
+  double* deo = static_cast<double*>(c->eps_o.ensure((size_t)(nocc + nvirt) * sizeof(double)));
+  double* dev = deo + nocc;
+  RIMP2_HIP_CHECK(hipMemcpyAsync(deo, eps_o, (size_t)nocc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(dev, eps_v, (size_t)nvirt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  pack_current_B(c, nvirt, nocc, mode);
+  c->have_B = false;  // until the input checks are known to have passed
+  double* dpe = eps_pair ? static_cast<double*>(c->eps_pair.ensure((size_t)(nocc * nocc) * sizeof(double))) : nullptr;
+  double* det = static_cast<double*>(c->etot.ensure(sizeof(double)));
+  pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, deo, dev, dpe, det, nullptr, 0);
+  check_status(c);
+  c->have_metric = true;
+  c->have_B = true;
+  if (eps_pair)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(eps_pair, dpe, (size_t)(nocc * nocc) * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(e_tot, det, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
+  RIMP2_API_END(ctx)
 }
 
 int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Pia, const double* d_V,
@@ -527,7 +589,7 @@ int rimp2_copy_h2d(void* ctx, void* dst, const void* src, size_t bytes) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
   if ((!dst || !src) && bytes) fail(RIMP2_ERR_ARG, "null pointer");
-  h2d_copy(c, dst, src, bytes);
+  h2d_copy(c, c->stream, dst, src, bytes);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   RIMP2_API_END(ctx)
 }

@@ -898,7 +898,7 @@ int rimp2_eos_arrays(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, in
   unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
   unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
   for (int i = 0; i < 7; i++)
-    h2d_copy(c, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
+    h2d_copy(c, c->stream, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
   const double* d[7];
   for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
   double* o[4];
@@ -953,7 +953,7 @@ int rimp2_first_order(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, i
   unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
   unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
   for (int i = 0; i < 7; i++)
-    h2d_copy(c, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
+    h2d_copy(c, c->stream, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
   const double* d[7];
   for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
   double* o[2];

@@ -776,7 +776,10 @@ void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, c
 
 // dB = Linv * X  (X: n x (nvirt*nocc) device, column-major): X is packed
 // into B-fragment order, then one DMA-pipelined FP64 MFMA GEMM.
-void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t nocc, const double* dX) {
+// Packs ni occupied orbitals of X (dXi: n x nvirt each) into Xp, one slab per
+// orbital in the transform's operand order.
+static void pack_x_into(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t ni, const double* dXi,
+                        void* Xp) {
   PackGeom gx;
   gx.mode = RIMP2_MODE_FP64;
   gx.rb = 16;
@@ -787,11 +790,25 @@ void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt,
   gx.nks = round_up(n, TR_KQ) / 4;
   gx.nvp = gx.nrb * 16;
   gx.nqp = gx.nks * 4;
-  void* Xp = m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
   // on a side stream, a capped grid: it runs beside the Cholesky's chain of
   // small launches instead of competing with it for the whole GPU (512
   // workgroups measured best: a full grid slows the Cholesky, 128 outlasts it)
-  pack_B(s == m.stream ? m.timer : nullptr, dX, n, nvirt, nocc, gx, Xp, nullptr, s, s == m.stream ? 0u : 512u);
+  pack_B(s == m.stream ? m.timer : nullptr, dXi, n, nvirt, ni, gx, Xp, nullptr, s, s == m.stream ? 0u : 512u);
+}
+
+void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t nocc, const double* dX) {
+  pack_x_into(m, s, n, nvirt, nocc, dX, m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc)));
+}
+
+void metric_reserve_x(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc) {
+  m.x_packed.ensure(x_packed_bytes(n, nvirt, nocc));
+}
+
+void metric_pack_x_range(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t i0, int64_t ni,
+                         const double* dXi) {
+  if (m.x_packed.bytes < x_packed_bytes(n, nvirt, i0 + ni))
+    fail(RIMP2_ERR_STATE, "packed X holds fewer than %lld orbitals", (long long)(i0 + ni));
+  pack_x_into(m, s, n, nvirt, ni, dXi, m.x_packed.as<unsigned char>() + x_packed_bytes(n, nvirt, i0));
 }
 
 void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, const double* dX, double* B) {
@@ -799,7 +816,9 @@ void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t noc
   metric_transform_packed(m, n, nvirt, nocc, B);
 }
 
-void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, double* B) {
+// B (n x nvirt x ni) = L^{-1} X for the ni orbitals whose packed X starts at Xp.
+static void transform_packed_at(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc,
+                                const unsigned char* Xp, double* B) {
   TransformArgs p;
   p.naux = n;
   p.nvirt = nvirt;
@@ -817,19 +836,22 @@ void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int6
   p.chunk = ceil_div(p.nwu, 8);
   ScopedTimer t(m.timer, RIMP2_K_TRANSFORM);
   p.Lp = m.linv_packed.as<unsigned char>();
-  p.Xp = m.x_packed.as<unsigned char>();
+  p.Xp = Xp;
   p.B = B;
   const int64_t nslot = p.chunk < kMaxGridBlocks / 8 ? p.chunk : kMaxGridBlocks / 8;
   hipLaunchKernelGGL(transform_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, m.stream, p);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 
-// c->B64 = Linv * X for the context's metric (metric_factor(c, ...) first).
-void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX) {
-  const int64_t n = c->naux;
-  double* B = static_cast<double*>(c->B64.ensure((size_t)(n * nvirt * nocc) * sizeof(double)));
-  metric_transform(ctx_metric_view(c), n, nvirt, nocc, dX, B);
+void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, double* B) {
+  transform_packed_at(m, n, nvirt, nocc, m.x_packed.as<unsigned char>(), B);
 }
+
+void metric_transform_range(const MetricView& m, int64_t n, int64_t nvirt, int64_t i0, int64_t ni, double* B) {
+  transform_packed_at(m, n, nvirt, ni, m.x_packed.as<unsigned char>() + x_packed_bytes(n, nvirt, i0),
+                      B + i0 * n * nvirt);
+}
+
 
 
 // ---------------------------------------------------------------------------

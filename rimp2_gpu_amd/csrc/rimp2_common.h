@@ -113,11 +113,11 @@ struct Ctx {
   bool pin_busy[2] = {false, false};
 };
 
-// Copies a pageable host array to the device, in order on c->stream. Copies
+// Copies a pageable host array to the device, in order on stream s. Copies
 // of 8 MiB and more go through the pinned chunk buffers: all host threads
 // fill one buffer while the DMA engine drains the other. The host array may
 // be reused as soon as the call returns.
-void h2d_copy(Ctx* c, void* dst, const void* src, size_t bytes);
+void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes);
 
 // Records hipEvents around a launch on ctx->stream when timing is on (no-op
 // for a null ctx).
@@ -147,8 +147,13 @@ void metric_transform(const MetricView& m, int64_t n, int64_t nvirt, int64_t noc
 // stream s, then the product from the packed X on m.stream.
 void metric_pack_x(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t nocc, const double* dX);
 void metric_transform_packed(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc, double* dB);
+// The same for orbitals [i0, i0 + ni) only (dXi points at orbital i0 of X,
+// dB is the whole B); metric_reserve_x sizes m.x_packed for all orbitals first.
+void metric_reserve_x(const MetricView& m, int64_t n, int64_t nvirt, int64_t nocc);
+void metric_pack_x_range(const MetricView& m, hipStream_t s, int64_t n, int64_t nvirt, int64_t i0, int64_t ni,
+                         const double* dXi);
+void metric_transform_range(const MetricView& m, int64_t n, int64_t nvirt, int64_t i0, int64_t ni, double* dB);
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
-void metric_transform(Ctx* c, int64_t nvirt, int64_t nocc, const double* dX);  // B64 = Linv * X
 // Grouped metric pipeline: the Cholesky, inverse and transform of many
 // fragments with one launch per step for the whole group (metric_group.h).
 // Arenas and job tables of a grouped pipeline (grow-only, reused call after

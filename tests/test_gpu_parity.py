@@ -155,6 +155,58 @@ def test_staged_host_copies_are_exact(ctx, rimp2):
     assert E_fit == E_host
 
 
+def test_host_energy_pipeline_by_orbital_chunks(ctx, rimp2):
+    """rimp2_energy from host arrays copies (P|ia) in chunks of occupied
+    orbitals under the Cholesky and transforms each chunk as it lands: the
+    energy equals the device-resident path's bit for bit, a NaN/Inf in any
+    chunk or in V is rejected before results are copied out, and the context
+    keeps working afterwards."""
+    naux, nvirt, nocc = 1024, 512, 20  # 4 MiB per orbital: chunks of 8, 8 and 4 orbitals
+    say("host energy pipeline", naux, nvirt, nocc)
+    P, V, eo, ev = rimp2.gen_fragment(31, naux, nvirt, nocc)
+    d = [ctx.to_device(x) for x in (P, V, eo, ev)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    for mode in ["i8s5", "fp64"]:
+        E_host, eps_host = ctx.energy(P, V, eo, ev, mode)
+        ctx.energy_device(naux, nvirt, nocc, *d, mode, dpe, det)
+        ctx.sync()
+        assert float(ctx.from_device(det, (1,))[0]) == E_host
+        np.testing.assert_array_equal(ctx.from_device(dpe, (nocc, nocc)), eps_host)
+        E_split, _ = ctx.energy_fitted(ctx.get_B(naux, nvirt, nocc), eo, ev, mode)
+        assert E_split == E_host
+    for p in d + [dpe, det]:
+        ctx.free(p)
+    for occ, val in [(0, np.nan), (8, np.inf), (19, -np.inf)]:
+        Pn = P.copy(order="F")
+        Pn[naux - 1, nvirt - 1, occ] = val
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.energy(Pn, V, eo, ev, "i8s5")
+        assert ei.value.code == rimp2.ERR_NUMERIC
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.get_B(naux, nvirt, nocc)  # a rejected call leaves no coefficients
+        assert ei.value.code == rimp2.ERR_STATE
+    Vn = V.copy(order="F")
+    Vn[3, 5] = np.nan
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.energy(P, Vn, eo, ev, "i8s5")
+    assert ei.value.code == rimp2.ERR_NUMERIC
+    Vb = V.copy(order="F")
+    Vb[7, 7] = -10.0
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.energy(P, Vb, eo, ev, "i8s5")
+    assert ei.value.code == rimp2.ERR_NOT_SPD
+    with pytest.raises(rimp2.Rimp2Error) as ei:
+        ctx.transform(P, "i8s5")  # no metric factored after the failures
+    assert ei.value.code == rimp2.ERR_STATE
+    E_again, _ = ctx.energy(P, V, eo, ev, "i8s5")
+    ctx.metric(V)
+    ctx.transform(P, "i8s5")
+    E_steps, _ = ctx.pair_energies(eo, ev, "i8s5")
+    assert E_again == E_steps
+    E_ref, _ = ctx.energy(P, V, eo, ev, "fp64")
+    assert abs(E_again - E_ref) < 1e-6
+
+
 def test_deterministic(ctx, rimp2):
     P, V, eo, ev = rimp2.gen_fragment(5, 192, 80, 7)
     say("determinism")
