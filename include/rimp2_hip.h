@@ -97,7 +97,10 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
  * energies.  eps_pair is [nocc,nocc] (may be NULL); e_tot receives E. */
 int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* eps_o, const double* eps_v, int mode,
                         double* eps_pair, double* e_tot);
-/* Whole path from host inputs to host energies (metric, transform, pairs). */
+/* Whole path from host inputs to host energies: the three calls above as one
+ * queue ((P|ia) is copied by chunks of occupied orbitals under the Cholesky).
+ * A NaN/Inf input (RIMP2_ERR_NUMERIC) or a non-positive pivot
+ * (RIMP2_ERR_NOT_SPD) is reported before eps_pair / e_tot are written. */
 int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* Pia, const double* V,
                  const double* eps_o, const double* eps_v, int mode, double* eps_pair, double* e_tot);
 /* Same path with every pointer in device memory; asynchronous on the context
