@@ -207,6 +207,27 @@ def test_host_energy_pipeline_by_orbital_chunks(ctx, rimp2):
     assert abs(E_again - E_ref) < 1e-6
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "bf16x3", "i8s4", "i8s5"])
+def test_host_energy_chunks_at_ragged_sizes(ctx, rimp2, mode):
+    """naux and nvirt off every tile multiple, 5.6 MB per occupied orbital:
+    (P|ia) arrives in chunks of 5, 5 and 2 orbitals, each packed into padded
+    slabs; the host path equals the device path bit for bit in every mode."""
+    naux, nvirt, nocc = 1001, 700, 12
+    say("ragged host chunks", mode)
+    P, V, eo, ev = rimp2.gen_fragment(41, naux, nvirt, nocc)
+    E_host, eps_host = ctx.energy(P, V, eo, ev, mode)
+    d = [ctx.to_device(x) for x in (P, V, eo, ev)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    ctx.energy_device(naux, nvirt, nocc, *d, mode, dpe, det)
+    ctx.sync()
+    E_dev = float(ctx.from_device(det, (1,))[0])
+    eps_dev = ctx.from_device(dpe, (nocc, nocc))
+    for p in d + [dpe, det]:
+        ctx.free(p)
+    assert E_dev == E_host
+    np.testing.assert_array_equal(eps_dev, eps_host)
+
+
 def test_deterministic(ctx, rimp2):
     P, V, eo, ev = rimp2.gen_fragment(5, 192, 80, 7)
     say("determinism")
