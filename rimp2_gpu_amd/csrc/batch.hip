@@ -23,6 +23,7 @@
 // on the batch itself; the winner is cached in the context.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <numeric>
 
@@ -382,11 +383,20 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
     if (mode_slices(mode) && d.naux > 16384)
       fail(RIMP2_ERR_ARG, "fragment %lld: exact-slice modes support naux <= 16384", (long long)f);
     if (!on_device) {
+      // fmax / fmin skip a NaN, so every orbital energy is tested on its own
       double mo = -INFINITY, mv = INFINITY;
-      for (int64_t i = 0; i < d.nocc; i++) mo = fmax(mo, d.eps_o[i]);
-      for (int64_t a = 0; a < d.nvirt; a++) mv = fmin(mv, d.eps_v[a]);
-      if (!(2.0 * mo - 2.0 * mv < 0.0) || !std::isfinite(mo) || !std::isfinite(mv))
-        fail(RIMP2_ERR_NUMERIC, "fragment %lld: orbital-energy denominator >= 0 or non-finite", (long long)f);
+      bool finite = true;
+      for (int64_t i = 0; i < d.nocc; i++) {
+        finite = finite && std::isfinite(d.eps_o[i]);
+        mo = fmax(mo, d.eps_o[i]);
+      }
+      for (int64_t a = 0; a < d.nvirt; a++) {
+        finite = finite && std::isfinite(d.eps_v[a]);
+        mv = fmin(mv, d.eps_v[a]);
+      }
+      if (!finite) fail(RIMP2_ERR_NUMERIC, "fragment %lld: non-finite orbital energy", (long long)f);
+      if (!(2.0 * mo - 2.0 * mv < 0.0))
+        fail(RIMP2_ERR_NUMERIC, "fragment %lld: orbital-energy denominator >= 0", (long long)f);
       if (d.V) stage_n += (size_t)(d.naux * d.naux + d.naux * d.nvirt * d.nocc);
     }
     b->nocc[f] = d.nocc;
@@ -407,6 +417,8 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
   std::vector<mg::Item> items;
   std::vector<mg::Copy> copies;
   std::vector<mg::Denom> denoms;
+  // host inputs: the orbital energies of all fragments go over in one copy
+  std::vector<double> heps(on_device ? 0 : occ_total + virt_total);
   size_t o = 0, v = 0, so = 0;
   for (int64_t f = 0; f < nfrag; f++) {
     const rimp2_frag_desc& d = frags[f];
@@ -419,8 +431,8 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
       if (d.V) items.push_back(mg::Item{d.naux, d.nvirt, d.nocc, d.V, d.Pia, dst, st});
       else copies.push_back(mg::Copy{d.Pia, dst, nx, st});  // already fitted: Pia is B
     } else {
-      RIMP2_HIP_CHECK(hipMemcpyAsync(eo + o, d.eps_o, (size_t)d.nocc * sizeof(double), hipMemcpyHostToDevice, s));
-      RIMP2_HIP_CHECK(hipMemcpyAsync(ev + v, d.eps_v, (size_t)d.nvirt * sizeof(double), hipMemcpyHostToDevice, s));
+      memcpy(heps.data() + o, d.eps_o, (size_t)d.nocc * sizeof(double));
+      memcpy(heps.data() + occ_total + v, d.eps_v, (size_t)d.nvirt * sizeof(double));
       if (d.V) {
         double* sV = stage + so;
         double* sX = sV + d.naux * d.naux;
@@ -437,6 +449,7 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
     o += (size_t)d.nocc;
     v += (size_t)d.nvirt;
   }
+  if (!on_device) h2d_copy(c, s, eo, heps.data(), heps.size() * sizeof(double));
   metric_group_run(b->group, s, items, copies, denoms);
   std::vector<int> hst((size_t)nfrag * 2);
   RIMP2_HIP_CHECK(hipMemcpyAsync(hst.data(), fst, hst.size() * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -106,7 +106,7 @@ static void check_host_finite(const double* x, int64_t n, const char* what) {
 constexpr size_t kPinChunk = (size_t)16 << 20;
 
 static void parallel_memcpy(char* d, const char* s, size_t n) {
-  const int T = std::max(1, std::min(omp_get_max_threads(), (int)(n >> 20)));  // >= 1 MiB per thread
+  const int T = std::max(1, std::min(omp_get_max_threads(), (int)(n >> 18)));  // >= 256 KiB per thread
   const size_t piece = (n + T - 1) / T;
 #pragma omp parallel for num_threads(T) schedule(static)
   for (int t = 0; t < T; t++) {
@@ -116,7 +116,7 @@ static void parallel_memcpy(char* d, const char* s, size_t n) {
 }
 
 void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes) {
-  if (bytes < kPinChunk / 2) {
+  if (bytes < ((size_t)1 << 20)) {
     RIMP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return;
   }
@@ -126,8 +126,9 @@ void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes) {
   }
   const char* from = static_cast<const char*>(src);
   char* to = static_cast<char*>(dst);
-  for (size_t off = 0, k = 0; off < bytes; off += kPinChunk, k++) {
-    const int b = (int)(k & 1);
+  for (size_t off = 0; off < bytes; off += kPinChunk) {
+    const int b = c->pin_next;
+    c->pin_next ^= 1;
     const size_t n = std::min(kPinChunk, bytes - off);
     if (c->pin_busy[b]) RIMP2_HIP_CHECK(hipEventSynchronize(c->pin_ev[b]));
     c->pin_busy[b] = false;

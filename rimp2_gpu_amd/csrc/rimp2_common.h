@@ -107,16 +107,18 @@ struct Ctx {
   int64_t eos_chunk = 0;
   int eos_mode = RIMP2_MODE_FP64;  // amplitude slab precision: FP64 or I8S5
   // host-to-device staging: two pinned chunk buffers (allocated on first
-  // use) and, per buffer, an event recorded after its last DMA
+  // use), per buffer an event recorded after its last DMA, and the buffer the
+  // next chunk goes to (they alternate across calls too)
   char* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool pin_busy[2] = {false, false};
+  int pin_next = 0;
 };
 
 // Copies a pageable host array to the device, in order on stream s. Copies
-// of 8 MiB and more go through the pinned chunk buffers: all host threads
-// fill one buffer while the DMA engine drains the other. The host array may
-// be reused as soon as the call returns.
+// of 1 MiB and more go through the pinned chunk buffers: host threads fill
+// one buffer while the DMA engine drains the other. The host array may be
+// reused as soon as the call returns.
 void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes);
 
 // Records hipEvents around a launch on ctx->stream when timing is on (no-op

@@ -90,6 +90,13 @@ def test_batch_errors(ctx, rimp2):
     with pytest.raises(rimp2.Rimp2Error) as ei:
         ctx.batch([(P, V, eo, ev - 10.0)], "fp64")
     assert ei.value.code == rimp2.ERR_NUMERIC
+    # one NaN orbital energy among finite ones (max / min would skip it)
+    for k, which in [(1, "eo"), (4, "ev")]:
+        eon, evn = eo.copy(), ev.copy()
+        (eon if which == "eo" else evn)[k] = np.nan
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.batch([(P, V, eo, ev), (P, V, eon, evn)], "fp64")
+        assert ei.value.code == rimp2.ERR_NUMERIC
     with pytest.raises(rimp2.Rimp2Error) as ei:
         ctx.batch([], "fp64")
     assert ei.value.code == rimp2.ERR_ARG
@@ -187,8 +194,10 @@ def test_batch_prepare_device_errors(ctx, rimp2):
     Vbad[5, 5] = -1.0
     Pbad = P.copy(order="F")
     Pbad[1, 2, 0] = np.nan
+    eon = eo.copy()
+    eon[1] = np.nan
     cases = [((P, Vbad, eo, ev), rimp2.ERR_NOT_SPD), ((Pbad, V, eo, ev), rimp2.ERR_NUMERIC),
-             ((P, V, eo, ev - 10.0), rimp2.ERR_NUMERIC)]
+             ((P, V, eo, ev - 10.0), rimp2.ERR_NUMERIC), ((P, V, eon, ev), rimp2.ERR_NUMERIC)]
     for bad, code in cases:
         say("batch prepare device error", code)
         descs, ptrs = _to_device_descs(ctx, [good, bad])

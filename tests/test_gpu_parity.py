@@ -125,13 +125,13 @@ def test_device_resident_path_matches_host(ctx, rimp2):
 
 
 def test_staged_host_copies_are_exact(ctx, rimp2):
-    """Host arrays of 8 MiB and more reach the device through two pinned chunk
+    """Host arrays of 1 MiB and more reach the device through two pinned chunk
     buffers: byte-exact at chunk-ragged sizes, twice in a row (buffer reuse),
     and an energy from host arrays above the chunk size equals the one from
     device arrays."""
     say("staged host copies")
     rng = np.random.default_rng(11)
-    for nbytes in (40 * 2**20 + 13, 9 * 2**20 + 1, 2**24, 4096):
+    for nbytes in (40 * 2**20 + 13, 9 * 2**20 + 1, 2**24, 3 * 2**19 + 7, 2**20, 4096):
         for _ in range(2):
             x = rng.integers(0, 256, nbytes, dtype=np.uint8)
             d = ctx.to_device(x)
