@@ -1,18 +1,19 @@
 """RI-MP2 energy benchmark on one MI355X.
 
 One step = one full energy evaluation of a single fragment with every input
-already resident in HBM: FP64 Cholesky + inverse of the metric, metric
-transform B = L^-1 (P|ia) (FP64 MFMA), packing for the precision mode, the
-fused pair kernel and the compensated reductions (rimp2_energy_device).
+already resident in HBM: input checks, FP64 Cholesky + inverse of the metric,
+metric transform B = L^-1 (P|ia) (FP64 MFMA), packing for the precision mode,
+the fused pair kernel and the compensated reductions (rimp2_energy_device).
 
-Workload: nocc=64, nvirt=512, naux=2048 (seeded synthetic fragment).  The
-headline mode is the exact-slice int8 path (i8s5); the same step is also
-timed in the plain FP64 path to give the mixed-precision speed-up and the
-energy difference in microhartree.
+Headline workload: the largest single-GPU fragment, nocc=256, nvirt=2048,
+naux=6144 (B has 3.2e9 elements), generated on the device from a seed.  The
+headline mode is the exact-slice int8 path (i8s5, |dE| < 1 uEh); the plain
+FP64 path is timed beside it, both from HBM-resident inputs (`value`) and from
+host arrays to energies in host memory (`time_to_energy_s`, PCIe included).
+The 64/512/2048 fragment is timed as extra fields.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode i8s5] [--no-cpu]
-  python bench.py --workload c1|c3|c4|c5     (the other configurations, every mode)
-  python bench.py --workload eos             (DEC local-basis 4-index arrays, Steps 5-8)
+  python bench.py --workload c1|c2|c3|c4|c5|eos  (one configuration, every mode)
 
 Prints ONE JSON line (plus progress on stderr).
 """
@@ -54,58 +55,6 @@ def time_mode(r, ctx, dims, dptrs, mode, steps, warmup):
     E = float(ctx.from_device(det, (1,))[0])
     eps = ctx.from_device(dpe, (nocc, nocc))
     return (t1 - t0) / steps, kern, E, eps
-
-
-def cpu_baseline(r, P, V, eo, ev, budget_s=25.0):
-    """The oracle's fused FP64 pair loops (RIMP2_calc_toccA + RIMP2_calc_gocc
-    restated, U = I) on the host cores; the metric step uses LAPACK."""
-    import scipy.linalg as sl
-    import oracle
-    naux, nvirt, nocc = P.shape
-    L = sl.cholesky(V, lower=True)
-    B = np.asfortranarray(sl.solve_triangular(L, P.reshape(naux, -1, order="F"), lower=True).reshape(P.shape,
-                                                                                                     order="F"))
-    cores = oracle.num_threads()
-    # calibrate on one pair per thread, then run the full fragment if it fits
-    ncal = max(1, min(cores, nocc * (nocc + 1) // 2))
-    cj = np.floor((np.sqrt(8 * np.arange(ncal) + 1) - 1) / 2).astype(np.int64)
-    ci = np.arange(ncal) - cj * (cj + 1) // 2
-    t0 = time.perf_counter()
-    oracle.pair_energies_subset(B, eo, ev, ci, cj)
-    per_pair = (time.perf_counter() - t0) / ncal
-    npairs = nocc * (nocc + 1) // 2
-    if per_pair * npairs <= budget_s:
-        t0 = time.perf_counter()
-        E, eps = oracle.pair_energies(B, eo, ev)
-        dt = time.perf_counter() - t0
-        sample = f"all {npairs} unique pairs (full fragment)"
-        flops = r.pair_flops(nocc, nvirt, naux)
-        return dict(value=flops / dt / 1e12, unit="TFLOP/s", cores=cores, kind="port", sample=sample,
-                    seconds=dt), E
-    nsamp = max(1, int(budget_s / per_pair))
-    rng = np.random.default_rng(0)
-    idx = rng.choice(npairs, size=min(nsamp, npairs), replace=False)
-    jj = np.floor((np.sqrt(8 * idx + 1) - 1) / 2).astype(np.int64)
-    ii = idx - jj * (jj + 1) // 2
-    t0 = time.perf_counter()
-    oracle.pair_energies_subset(B, eo, ev, ii, jj)
-    dt = time.perf_counter() - t0
-    flops = 2.0 * naux * nvirt * nvirt * len(idx)
-    return dict(value=flops / dt / 1e12, unit="TFLOP/s", cores=cores, kind="port",
-                sample=f"{len(idx)} of {npairs} unique pairs (extrapolated)", seconds=dt), None
-
-
-def measured_traffic(mode, nocc, nvirt, naux):
-    """HBM bytes per pair-kernel launch from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE/WRITE_SIZE scaled by a copy probe),
-    or None when no summary exists for it."""
-    path = os.path.join(ROOT, "profiles", f"pmc_pair_{mode}_nocc{nocc}_nvirt{nvirt}_naux{naux}.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.strip().startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
 
 
 def copy_probe(r, mib):
@@ -404,22 +353,147 @@ def workload_eos(r, steps, warmup, cpu=True):
     return res
 
 
+def cpu_baseline_c5(r, V, eo, ev, x_of, nocc, nvirt, naux, eps64, budget_orbs=8):
+    """The oracle (the restated reference loops: RIMP2_calc_toccA +
+    RIMP2_calc_gocc pair contractions, RIMP2.F90:381-417 / 562-587, with the
+    oracle's Cholesky and forward substitution for the metric) on the host
+    cores, on a bounded sample: the Cholesky in full, the metric transform of
+    budget_orbs occupied orbitals and the pair energies of the pairs among
+    them (a whole number of rounds over the threads); the full fragment's time
+    is extrapolated from those rates.  The sampled pair energies are also the
+    oracle check of the GPU's FP64 pair energies."""
+    import oracle
+    cores = oracle.num_threads()
+    model, isa = oracle.cpu_info()
+    orbs = np.unique(np.linspace(0, nocc - 1, budget_orbs).round().astype(np.int64))
+    X = np.empty((naux, nvirt, len(orbs)), order="F")
+    for k, i in enumerate(orbs):
+        X[:, :, k] = x_of(int(i))
+    t0 = time.perf_counter()
+    L = oracle.cholesky(V)
+    t_chol = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    B = oracle.trsm_lower(L, X)
+    t_trsm = time.perf_counter() - t0
+    del X, L
+    li, lj = np.triu_indices(len(orbs))
+    npairs = max(cores, (len(li) // cores) * cores)
+    li, lj = li[:npairs], lj[:npairs]
+    t0 = time.perf_counter()
+    ref = oracle.pair_energies_subset(np.asfortranarray(B), eo[orbs], ev, li, lj)
+    t_pairs = time.perf_counter() - t0
+    got = eps64[orbs[li], orbs[lj]]
+    rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
+    n_unique = nocc * (nocc + 1) // 2
+    t_full = t_chol + t_trsm / len(orbs) * nocc + t_pairs / len(li) * n_unique
+    F = r.pair_flops(nocc, nvirt, naux) + r.metric_flops(nocc, nvirt, naux)
+    cpu = dict(value=F / t_full / 1e12, unit="TFLOP/s", cores=cores, kind="port",
+               sample=(f"extrapolated: Cholesky of the {naux}x{naux} metric in full, the metric transform of "
+                       f"{len(orbs)} of {nocc} occupied orbitals and {len(li)} of {n_unique} unique pair energies, "
+                       f"scaled by orbital and pair counts"),
+               seconds=t_chol + t_trsm + t_pairs, time_to_energy_s_extrapolated=t_full,
+               parts_s=dict(cholesky=t_chol, transform_per_orbital=t_trsm / len(orbs), pair=t_pairs / len(li)),
+               cpu_model=model, nproc=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+               oracle_march=oracle.build_flavour(), cpu_isa=isa)
+    check = dict(pairs_checked=int(len(li)), orbitals=[int(i) for i in orbs], max_rel_err_fp64_pairs=rel)
+    return cpu, check
+
+
+def timed_steps(ctx, dims, dptrs, mode, steps, warmup, label):
+    """W untimed warmup steps, then K steps between two synchronisations, with
+    the per-kernel event timers on; progress on stderr after each step."""
+    naux, nvirt, nocc = dims
+    dP, dV, deo, dev, dpe, det = dptrs
+    for w in range(warmup):
+        t0 = time.perf_counter()
+        ctx.energy_device(naux, nvirt, nocc, dP, dV, deo, dev, mode, dpe, det)
+        ctx.sync()
+        log(f"[bench] {label} {mode} warmup {w + 1}/{warmup}: {time.perf_counter() - t0:.3f} s")
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.energy_device(naux, nvirt, nocc, dP, dV, deo, dev, mode, dpe, det)
+        if steps > 1 and (time.perf_counter() - t0) > 5.0 * (k + 1) / steps:
+            log(f"[bench] {label} {mode} step {k + 1}/{steps} queued at {time.perf_counter() - t0:.1f} s")
+    ctx.sync()
+    t1 = time.perf_counter()
+    kern = {k: ctx.timer(k) for k in ("pair", "transform", "pack", "cholesky", "trtri", "reduce")}
+    ctx.timing(False)
+    E = float(ctx.from_device(det, (1,))[0])
+    eps = ctx.from_device(dpe, (nocc, nocc))
+    log(f"[bench] {label} {mode}: {(t1 - t0) / steps:.4f} s/step over {steps} steps, E={E!r}")
+    return (t1 - t0) / steps, kern, E, eps
+
+
+def mean_ms(k):
+    return k[0] / max(1, k[1])
+
+
+def c2_extra(r, steps, warmup):
+    """The 64/512/2048 fragment: i8s5 and fp64 steps from HBM, as extra fields."""
+    naux, nvirt, nocc = 2048, 512, 64
+    ctx = r.Context(0)
+    d = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8),
+         ctx.alloc(nocc * nocc * 8), ctx.alloc(8)]
+    ctx.gen_device(2, naux, nvirt, nocc, *d[:4])
+    out = {}
+    for mode, k, w in (("i8s5", steps, warmup), ("fp64", max(2, steps // 2), 1)):
+        t, kern, E, _ = timed_steps(ctx, (naux, nvirt, nocc), d, mode, k, w, "c2")
+        out[mode] = dict(ms_per_step=t * 1e3, pair_kernel_ms=mean_ms(kern["pair"]), E=E,
+                         kernels_ms={n: mean_ms(v) for n, v in kern.items()})
+    out["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["i8s5"]["ms_per_step"]
+    out["pair_speedup_vs_fp64"] = out["fp64"]["pair_kernel_ms"] / out["i8s5"]["pair_kernel_ms"]
+    out["dE_uEh_vs_fp64"] = (out["i8s5"]["E"] - out["fp64"]["E"]) * 1e6
+    Fp = r.pair_flops(nocc, nvirt, naux)
+    out["i8s5"]["pair_int8_tops"] = Fp * NPROD["i8s5"] / (out["i8s5"]["pair_kernel_ms"] * 1e-3) / 1e12
+    out["workload"] = f"single RI-MP2 fragment nocc={nocc} nvirt={nvirt} naux={naux}, seed 2, inputs in HBM"
+    for p in d:
+        ctx.free(p)
+    ctx.close()
+    return out
+
+
+def profile_figures(mode, nocc, nvirt, naux):
+    """HBM bytes per pair-kernel launch and the launch duration in the
+    committed rocprofv3 summaries of this workload (counter runs scaled by a
+    copy probe; kernel trace), or None where no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_pair_{mode}_nocc{nocc}_nvirt{nvirt}_naux{naux}.txt")
+    vals = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if ":" in line:
+                k, v = line.split(":", 1)
+                k = k.strip()
+                if k in ("traffic_bytes_per_launch", "kernel_trace_mean_ms", "fetch_bytes_per_launch",
+                         "write_bytes_per_launch", "mfma_busy_fraction"):
+                    try:
+                        vals[k] = float(v.split()[0])
+                    except ValueError:
+                        pass
+        vals["file"] = os.path.relpath(path, ROOT)
+    return vals
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="i8s5")
-    ap.add_argument("--nocc", type=int, default=64)
-    ap.add_argument("--nvirt", type=int, default=512)
-    ap.add_argument("--naux", type=int, default=2048)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--nocc", type=int, default=256)
+    ap.add_argument("--nvirt", type=int, default=2048)
+    ap.add_argument("--naux", type=int, default=6144)
+    ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-host", action="store_true", help="skip the host-memory (PCIe-inclusive) run")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "eos"],
-                    help="run one of the other configurations in every mode (one JSON line) instead of the "
-                         "headline step")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-memory (PCIe-inclusive) runs")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 64/512/2048 extra fields")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock after which the optional legs (64/512/2048 fields, CPU baseline) are skipped")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "eos"],
+                    help="run one configuration in every mode (one JSON line) instead of the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates the HBM byte counters)")
     a = ap.parse_args()
@@ -427,6 +501,7 @@ def main():
         print(f"bench.py measures one MI355X only; --gpus {a.gpus} is not supported (the pair and fragment "
               "sharding across GPUs is a design note, not a measured path).", file=sys.stderr)
         sys.exit(2)
+    T0 = time.perf_counter()
 
     import rimp2_gpu_amd as r
     if a.copy_probe:
@@ -435,6 +510,8 @@ def main():
     if a.workload:
         if a.workload == "c1":
             res = workload_single(r, "c1", 1, 512, 128, 16, a.steps, a.warmup, None)
+        elif a.workload == "c2":
+            res = workload_single(r, "c2", 2, 2048, 512, 64, a.steps, a.warmup, 12)
         elif a.workload == "c5":
             res = workload_single(r, "c5", 5, 6144, 2048, 256, 1, 0, 4)
         elif a.workload == "c3":
@@ -446,62 +523,98 @@ def main():
         print(json.dumps(res), flush=True)
         return
     naux, nvirt, nocc = a.naux, a.nvirt, a.nocc
-    log(f"[bench] generating fragment nocc={nocc} nvirt={nvirt} naux={naux} seed={a.seed}")
-    P, V, eo, ev = r.gen_fragment(a.seed, naux, nvirt, nocc)
+    dims = (naux, nvirt, nocc)
     Fp = r.pair_flops(nocc, nvirt, naux)
     Fm = r.metric_flops(nocc, nvirt, naux)
-
     ctx = r.Context(0)
-    dptrs = [ctx.to_device(x) for x in (P, V, eo, ev)] + [ctx.alloc(nocc * nocc * 8), ctx.alloc(8)]
-    dims = (naux, nvirt, nocc)
-    log(f"[bench] timing mode {a.mode}: warmup {a.warmup}, steps {a.steps}")
-    t_mode, k_mode, E_mode, eps_mode = time_mode(r, ctx, dims, dptrs, a.mode, a.steps, a.warmup)
-    log(f"[bench] {a.mode}: {t_mode * 1e3:.3f} ms/step E={E_mode:.10f}")
+    per = naux * nvirt * 8
+    dptrs = [ctx.alloc(per * nocc), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8),
+             ctx.alloc(nocc * nocc * 8), ctx.alloc(8)]
+    log(f"[bench] generating fragment nocc={nocc} nvirt={nvirt} naux={naux} seed={a.seed} on the device")
+    ctx.gen_device(a.seed, naux, nvirt, nocc, *dptrs[:4])
+    ctx.sync()
+
+    # ---- the timed steps: inputs resident in HBM ----
+    t_mode, k_mode, E_mode, eps_mode = timed_steps(ctx, dims, dptrs, a.mode, a.steps, a.warmup, "headline")
     res64 = None
+    eps64 = None
     if not a.no_fp64 and a.mode != "fp64":
-        log("[bench] timing fp64 plain path")
-        t64, k64, E64, eps64 = time_mode(r, ctx, dims, dptrs, "fp64", max(1, a.steps // 2), 1)
-        res64 = dict(ms_per_step=t64 * 1e3, tflops=(Fp + Fm) / t64 / 1e12,
-                     pair_kernel_ms=k64["pair"][0] / max(1, k64["pair"][1]),
-                     pair_tflops=Fp / (k64["pair"][0] / max(1, k64["pair"][1]) * 1e-3) / 1e12, E=E64)
-        log(f"[bench] fp64: {t64 * 1e3:.3f} ms/step E={E64:.10f}")
+        t64, k64, E64, eps64 = timed_steps(ctx, dims, dptrs, "fp64", 2, 1, "headline")
+        res64 = dict(ms_per_step=t64 * 1e3, tflops=(Fp + Fm) / t64 / 1e12, pair_kernel_ms=mean_ms(k64["pair"]),
+                     pair_tflops=Fp / (mean_ms(k64["pair"]) * 1e-3) / 1e12,
+                     pair_frac_fp64_peak=Fp / (mean_ms(k64["pair"]) * 1e-3) / 1e12 / PEAK["fp64_tflops"],
+                     kernels_ms={n: mean_ms(v) for n, v in k64.items()}, E=E64, steps=2, warmup=1)
+    elif a.mode == "fp64":
+        eps64 = eps_mode
 
-    # time-to-energy from host memory (PCIe-inclusive): the first call (which
-    # also allocates the host path's device buffers) and the best of three more
-    tte_host = tte_host_first = host_equals_device = None
+    # ---- time-to-energy: host arrays in, E and eps_ij in host memory ----
+    tte = {}
+    host = None
     if not a.no_host:
-        runs, energies = [], []
-        for _ in range(4):
-            t0 = time.perf_counter()
-            E_host, _ = ctx.energy(P, V, eo, ev, a.mode)
-            runs.append(time.perf_counter() - t0)
-            energies.append(E_host)
-        tte_host_first, tte_host = runs[0], min(runs[1:])
-        host_equals_device = all(e == E_mode for e in energies)
-        log(f"[bench] from host arrays: first {tte_host_first * 1e3:.2f} ms, then {tte_host * 1e3:.2f} ms, "
-            f"energy equal to the device path's: {host_equals_device}")
+        log("[bench] copying the inputs to host memory for the host-array runs")
+        host = [ctx.from_device(dptrs[0], (naux, nvirt, nocc)), ctx.from_device(dptrs[1], (naux, naux)),
+                ctx.from_device(dptrs[2], (nocc,)), ctx.from_device(dptrs[3], (nvirt,))]
+        for mode, runs in ((a.mode, 2), ("fp64", 1)):
+            if mode == "fp64" and (a.no_fp64 and a.mode != "fp64"):
+                continue
+            if mode in tte:
+                continue
+            times, same = [], True
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                E_h, eps_h = ctx.energy(*host, mode)
+                times.append(time.perf_counter() - t0)
+                ref_E = E_mode if mode == a.mode else res64["E"]
+                same = same and (E_h == ref_E)
+            tte[mode] = dict(seconds=min(times), runs_s=times, energy_equals_device_path=same)
+            log(f"[bench] host arrays -> host energies, {mode}: {times} s, equal to the HBM path: {same}")
 
-    pair_ms = k_mode["pair"][0] / max(1, k_mode["pair"][1])
-    # Roofline of the contraction: F_pair / t against the FP32-matrix peak
-    # (FP64 peak for the fp64 mode).  Beside it, the rate of the MFMA type the
-    # mode executes: F_pair times the slice / split products per flop (int8 ops
-    # for the exact-slice modes, bf16 flops for split-BF16) against that peak.
-    achieved = Fp / (pair_ms * 1e-3) / 1e12
-    peak = PEAK["fp64_tflops"] if a.mode == "fp64" else PEAK["fp32_mfma_tflops"]
-    hw_rate = achieved * NPROD[a.mode]
-    hw_type, hw_peak = {"fp64": ("f64", PEAK["fp64_tflops"]), "fp32": ("f32", PEAK["fp32_mfma_tflops"]),
-                        "bf16x3": ("bf16", PEAK["bf16_mfma_tflops"])}.get(a.mode, ("i8", PEAK["int8_mfma_tops"]))
-    roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak,
-                traffic=measured_traffic(a.mode, nocc, nvirt, naux),
-                executed_mfma=dict(type=hw_type, rate=hw_rate, peak=hw_peak, frac=hw_rate / hw_peak,
-                                   unit="TOP/s" if hw_type == "i8" else "TFLOP/s"))
+    pair_ms = mean_ms(k_mode["pair"])
+    pair_s = pair_ms * 1e-3
+    # Roofline of the pair kernel: the rate of the MFMA type it executes
+    # (int8 slice products for the exact-slice modes) against that type's
+    # dense peak; the FP32-equivalent rate F_pair / t is kept beside it.
+    hw_type, hw_peak, hw_unit = {"fp64": ("f64", PEAK["fp64_tflops"], "TFLOP/s"),
+                                 "fp32": ("f32", PEAK["fp32_mfma_tflops"], "TFLOP/s"),
+                                 "bf16x3": ("bf16", PEAK["bf16_mfma_tflops"], "TFLOP/s")}.get(
+        a.mode, ("i8", PEAK["int8_mfma_tops"], "TOP/s"))
+    hw_rate = Fp * NPROD[a.mode] / pair_s / 1e12
+    prof = profile_figures(a.mode, nocc, nvirt, naux)
+    roof = dict(bound="mfma", achieved=hw_rate, peak=hw_peak, unit=hw_unit, frac=hw_rate / hw_peak,
+                traffic=prof.get("traffic_bytes_per_launch"), mfma_type=hw_type,
+                ops_per_launch=Fp * NPROD[a.mode],
+                work=f"F_pair = 2 naux nvirt^2 nocc(nocc+1)/2 = {Fp:.4e} flops x {NPROD[a.mode]} "
+                     f"{hw_type} MFMA products per flop",
+                fp32_equivalent=dict(achieved=Fp / pair_s / 1e12, peak=PEAK["fp32_mfma_tflops"], unit="TFLOP/s",
+                                     frac=Fp / pair_s / 1e12 / PEAK["fp32_mfma_tflops"]),
+                profile=prof or None,
+                live_vs_profile_duration=(pair_ms / prof["kernel_trace_mean_ms"]
+                                          if prof.get("kernel_trace_mean_ms") else None))
 
-    cpu = None
-    E_ref = None
-    if not a.no_cpu:
-        log("[bench] cpu baseline (oracle FP64 pair loops on host cores)")
-        cpu, E_ref = cpu_baseline(r, P, V, eo, ev)
-        log(f"[bench] cpu: {cpu}")
+    # ---- optional legs, within the wall-clock budget ----
+    def left():
+        return a.budget_s - (time.perf_counter() - T0)
+
+    cpu = check = None
+    if not a.no_cpu and host is not None and eps64 is not None:
+        if left() > 60:
+            log("[bench] cpu baseline (the oracle on the host cores, bounded sample) and oracle check")
+            cpu, check = cpu_baseline_c5(r, host[1], host[2], host[3],
+                                         lambda i: host[0][:, :, i], nocc, nvirt, naux, eps64)
+            log(f"[bench] cpu: {cpu}")
+            log(f"[bench] oracle check: {check}")
+        else:
+            log(f"[bench] cpu baseline skipped: {left():.0f} s of the wall-clock budget left")
+    del host
+    for p in dptrs:
+        ctx.free(p)
+    ctx.close()
+    c2 = None
+    if not a.no_c2 and (naux, nvirt, nocc) != (2048, 512, 64):
+        if left() > 30:
+            c2 = c2_extra(r, 20, 5)
+        else:
+            log(f"[bench] 64/512/2048 fields skipped: {left():.0f} s of the wall-clock budget left")
 
     out = {
         "metric": "RI-MP2 time-to-energy & TFLOP/s: mixed-prec vs FP64, 1 MI355X, |dE|<1uEh",
@@ -515,33 +628,40 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"i8s5": "i8", "i8s4": "i8", "fp32": "f32", "bf16x3": "bf16", "fp64": "f64"}[a.mode],
-        "data": "synthetic (seeded generator: (P|ia) ~ N(0,1)*0.1/sqrt(1+a/16), V = AA^T/naux + I)",
-        "config": {"workload": f"single RI-MP2 fragment nocc={nocc} nvirt={nvirt} naux={naux}",
+        "data": "synthetic (seeded generator on the device: (P|ia) ~ N(0,1)*0.1/sqrt(1+a/16), V = AA^T/naux + I, "
+                "e_occ ~ U[-2,-0.3], e_virt ~ U[0.1,3])",
+        "config": {"workload": f"single RI-MP2 fragment nocc={nocc} nvirt={nvirt} naux={naux} "
+                               f"(largest single-GPU configuration), inputs resident in HBM",
                    "nocc": nocc, "nvirt": nvirt, "naux": naux, "mode": a.mode, "seed": a.seed,
                    "global_batch": 1, "parallelism": "single-gpu"},
-        "time_to_energy_s": t_mode,
-        "time_to_energy_host_s": tte_host,
-        "time_to_energy_host_first_call_s": tte_host_first,
-        "host_energy_equals_device": host_equals_device,
+        "value_definition": "(F_pair + F_metric) / step time, inputs in HBM; F_metric = naux^2 nvirt nocc + "
+                            "2 naux^3 / 3",
+        "time_to_energy_s": tte.get(a.mode, {}).get("seconds"),
+        "time_to_energy_definition": "host arrays in -> E and eps_ij in host memory (rimp2_energy), PCIe "
+                                     "included; best of the runs",
+        "time_to_energy_runs": tte or None,
+        "time_to_energy_fp64_s": tte.get("fp64", {}).get("seconds"),
         "pair_kernel_ms": pair_ms,
-        "kernels_ms": {k: v[0] / max(1, v[1]) for k, v in k_mode.items()},
+        "kernels_ms": {k: mean_ms(v) for k, v in k_mode.items()},
         "E_mode": E_mode,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     if res64:
         out["fp64_plain"] = res64
-        out["speedup_vs_fp64"] = res64["ms_per_step"] / (t_mode * 1e3)
+        if "fp64" in tte and a.mode in tte:
+            out["speedup_vs_fp64"] = tte["fp64"]["seconds"] / tte[a.mode]["seconds"]
+            out["speedup_vs_fp64_definition"] = "time-to-energy from host arrays, fp64 / " + a.mode
+        out["step_speedup_vs_fp64"] = res64["ms_per_step"] / (t_mode * 1e3)
         out["pair_speedup_vs_fp64"] = res64["pair_kernel_ms"] / pair_ms
         out["dE_uEh_vs_fp64"] = (E_mode - res64["E"]) * 1e6
-    if E_ref is not None:
-        out["E_oracle_fp64"] = E_ref
-        out["dE_uEh_vs_oracle"] = (E_mode - E_ref) * 1e6
-        if res64:
-            out["dE_uEh_fp64_vs_oracle"] = (res64["E"] - E_ref) * 1e6
-    for p in dptrs:
-        ctx.free(p)
-    ctx.close()
+    if check:
+        out["oracle_check"] = check
+    if cpu and cpu.get("time_to_energy_s_extrapolated") and out["time_to_energy_s"]:
+        out["gpu_vs_cpu_time_to_energy"] = cpu["time_to_energy_s_extrapolated"] / out["time_to_energy_s"]
+    if c2:
+        out["c2"] = c2
+    out["wall_s"] = time.perf_counter() - T0
     print(json.dumps(out), flush=True)
 
 

@@ -33,7 +33,24 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "librimp2_oracle.so")
+LIB_PATH_V4 = os.path.join(_HERE, "build", "librimp2_oracle_v4.so")
 _lib = None
+_lib_path = None
+
+
+def cpu_info():
+    """Model name and the x86-64 ISA level of the running CPU."""
+    model, flags = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    return model, ("x86-64-v4" if v4 else "x86-64-v3")
 
 _i64 = ctypes.c_int64
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -46,11 +63,14 @@ def build():
 
 
 def lib():
-    global _lib
+    """Loads the oracle built for the running CPU's ISA level (AVX-512 where
+    the CPU has it, AVX2 + FMA otherwise)."""
+    global _lib, _lib_path
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not (os.path.exists(LIB_PATH) and os.path.exists(LIB_PATH_V4)):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        _lib_path = LIB_PATH_V4 if cpu_info()[1] == "x86-64-v4" else LIB_PATH
+        L = ctypes.CDLL(_lib_path)
         ci = ctypes.c_int
         sig = {
             "oracle_toccA": [ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp],
@@ -265,3 +285,9 @@ def metric_transform(V, X):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def build_flavour():
+    """The -march level of the loaded oracle build."""
+    lib()
+    return "x86-64-v4" if _lib_path == LIB_PATH_V4 else "x86-64-v3"

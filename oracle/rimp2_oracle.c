@@ -422,24 +422,54 @@ void oracle_pair_V(int64_t nvirt, int64_t naux, const double* B, int64_t i, int6
 /* Metric: V = L L^T (lower Cholesky, column-major) and B = L^{-1} X.        */
 /* ------------------------------------------------------------------------- */
 
-/* Left-looking Cholesky; returns 0 or (k+1) for a non-positive pivot at k. */
-int oracle_cholesky(int64_t n, const double* V, double* L) {
-  for (int64_t j = 0; j < n; j++)
-    for (int64_t i = 0; i < n; i++) L[i + n * j] = (i >= j) ? V[i + n * j] : 0.0;
-  for (int64_t j = 0; j < n; j++) {
-    double d = L[j + n * j];
-    for (int64_t k = 0; k < j; k++) d -= L[j + n * k] * L[j + n * k];
-    if (!(d > 0.0)) return (int)(j + 1);
-    d = sqrt(d);
-    L[j + n * j] = d;
-#pragma omp parallel for schedule(static)
-    for (int64_t i = j + 1; i < n; i++) {
-      double s = L[i + n * j];
-      for (int64_t k = 0; k < j; k++) s -= L[i + n * k] * L[j + n * k];
-      L[i + n * j] = s / d;
-    }
+/* dot(x[0..n), y[0..n)) with four interleaved partial sums (fixed order). */
+static double dot4(const double* x, const double* y, int64_t n) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t k = 0;
+  for (; k + 4 <= n; k += 4) {
+    s0 += x[k] * y[k];
+    s1 += x[k + 1] * y[k + 1];
+    s2 += x[k + 2] * y[k + 2];
+    s3 += x[k + 3] * y[k + 3];
   }
-  return 0;
+  for (; k < n; k++) s0 += x[k] * y[k];
+  return (s0 + s1) + (s2 + s3);
+}
+
+/* Cholesky V = L L^T by the Cholesky-Crout (left-looking) recurrence
+ *   L_jj = sqrt(V_jj - sum_{k<j} L_jk^2),
+ *   L_ij = (V_ij - sum_{k<j} L_ik L_jk) / L_jj   (i > j),
+ * evaluated on a row-major copy (each sum is a contiguous dot product of two
+ * rows) in column blocks of 64: the block's diagonal rows sequentially, then
+ * every row below it in parallel, each row kept in cache across the block's
+ * 64 columns.  L is returned column-major with zeros above the diagonal;
+ * returns 0 or (k+1) for a non-positive pivot at k. */
+int oracle_cholesky(int64_t n, const double* V, double* L) {
+  const int64_t NB = 64;
+  double* R = malloc(sizeof(double) * (size_t)(n * n));
+  int bad = 0;
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j <= i; j++) R[i * n + j] = V[i + n * j];
+  for (int64_t j0 = 0; j0 < n && !bad; j0 += NB) {
+    const int64_t j1 = (j0 + NB < n) ? j0 + NB : n;
+    for (int64_t j = j0; j < j1 && !bad; j++) {
+      const double d = R[j * n + j] - dot4(R + j * n, R + j * n, j);
+      if (!(d > 0.0)) {
+        bad = (int)(j + 1);
+        break;
+      }
+      R[j * n + j] = sqrt(d);
+      for (int64_t i = j + 1; i < j1; i++) R[i * n + j] = (R[i * n + j] - dot4(R + i * n, R + j * n, j)) / R[j * n + j];
+    }
+    if (bad) break;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = j1; i < n; i++)
+      for (int64_t j = j0; j < j1; j++) R[i * n + j] = (R[i * n + j] - dot4(R + i * n, R + j * n, j)) / R[j * n + j];
+  }
+  for (int64_t j = 0; j < n; j++)
+    for (int64_t i = 0; i < n; i++) L[i + n * j] = (i >= j && !bad) ? R[i * n + j] : 0.0;
+  free(R);
+  return bad;
 }
 
 /* B(:,c) = L^{-1} X(:,c) for each of ncols columns (forward substitution,

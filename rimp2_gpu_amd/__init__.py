@@ -131,6 +131,26 @@ def _f64(a, shape=None):
     return np.asfortranarray(a) if a.ndim > 1 else np.ascontiguousarray(a)
 
 
+def _fragment(Pia, V, eps_o, eps_v):
+    """Checks one fragment's arrays against the shape of (P|ia) [naux, nvirt,
+    nocc] before any pointer reaches the library (which reads V as
+    naux x naux, eps_o as nocc and eps_v as nvirt values); V may be None (an
+    already-fitted tensor)."""
+    Pia = _f64(Pia)
+    if Pia.ndim != 3:
+        raise ValueError(f"(P|ia) must be [naux, nvirt, nocc], got shape {Pia.shape}")
+    naux, nvirt, nocc = Pia.shape
+    if V is not None:
+        V = np.asarray(V)
+        if V.shape != (naux, naux):
+            raise ValueError(f"V must be ({naux}, {naux}) for naux={naux}, got {V.shape}")
+        V = _f64(V)
+    eo, ev = np.asarray(eps_o), np.asarray(eps_v)
+    if eo.shape != (nocc,) or ev.shape != (nvirt,):
+        raise ValueError(f"eps_o must hold nocc={nocc} and eps_v nvirt={nvirt} values, got {eo.shape} and {ev.shape}")
+    return Pia, V, _f64(eo), _f64(ev)
+
+
 def _mode(m):
     if isinstance(m, str):
         if m not in MODES:
@@ -218,22 +238,28 @@ class Context:
     # ---- energy path ----
     def metric(self, V):
         V = _f64(V)
+        if V.ndim != 2 or V.shape[0] != V.shape[1]:
+            raise ValueError(f"V must be square, got shape {V.shape}")
         n = V.shape[0]
         self._chk(lib().rimp2_metric(self.h, n, _ptr(V)))
 
     def transform(self, Pia, mode="fp64"):
         Pia = _f64(Pia)
+        if Pia.ndim != 3:
+            raise ValueError(f"(P|ia) must be [naux, nvirt, nocc], got shape {Pia.shape}")
         naux, nvirt, nocc = Pia.shape
         self._chk(lib().rimp2_transform(self.h, naux, nvirt, nocc, _ptr(Pia), _mode(mode)))
 
     def load_fitted(self, Calpha, mode="fp64"):
         """Already-fitted Calpha [naux, nvirt, nocc] becomes B (no metric)."""
         Calpha = _f64(Calpha)
+        if Calpha.ndim != 3:
+            raise ValueError(f"Calpha must be [naux, nvirt, nocc], got shape {Calpha.shape}")
         naux, nvirt, nocc = Calpha.shape
         self._chk(lib().rimp2_load_fitted(self.h, naux, nvirt, nocc, _ptr(Calpha), _mode(mode)))
 
     def energy_fitted(self, Calpha, eps_o, eps_v, mode="fp64", want_pairs=True):
-        Calpha, eo, ev = _f64(Calpha), _f64(eps_o), _f64(eps_v)
+        Calpha, _, eo, ev = _fragment(Calpha, None, eps_o, eps_v)
         naux, nvirt, nocc = Calpha.shape
         ep = np.zeros((nocc, nocc), order="F") if want_pairs else None
         et = np.zeros(1)
@@ -253,7 +279,9 @@ class Context:
         return float(et[0]), ep
 
     def energy(self, Pia, V, eps_o, eps_v, mode="fp64", want_pairs=True):
-        Pia, V, eo, ev = _f64(Pia), _f64(V), _f64(eps_o), _f64(eps_v)
+        if V is None:
+            raise ValueError("V is required (use energy_fitted for an already-fitted tensor)")
+        Pia, V, eo, ev = _fragment(Pia, V, eps_o, eps_v)
         naux, nvirt, nocc = Pia.shape
         ep = np.zeros((nocc, nocc), order="F") if want_pairs else None
         et = np.zeros(1)
@@ -324,7 +352,7 @@ class Context:
         keep = []
         for k, (Pia, V, eo, ev) in enumerate(frags):
             # V None: Pia is already fitted (B = Pia), no metric
-            Pia, V, eo, ev = _f64(Pia), (None if V is None else _f64(V)), _f64(eo), _f64(ev)
+            Pia, V, eo, ev = _fragment(Pia, V, eo, ev)
             keep += [Pia, V, eo, ev]
             naux, nvirt, nocc = Pia.shape
             arr[k] = FragDesc(nocc, nvirt, naux, _ptr(Pia), _ptr(V), _ptr(eo), _ptr(ev))

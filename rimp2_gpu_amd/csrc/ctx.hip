@@ -148,21 +148,45 @@ static void check_host_denominators(const double* eo, int64_t nocc, const double
 }
 
 // Reads and clears the device status word; converts set bits to an error.
-static void check_status(Ctx* c) {
+// A set bit also invalidates the metric and the transformed B the context
+// holds: they were formed from the rejected inputs.
+void check_status(Ctx* c) {
   int st[2] = {0, 0};
   int* d = static_cast<int*>(c->status.ensure(64));
   RIMP2_HIP_CHECK(hipMemcpyAsync(st, d, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (st[0]) RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+  c->status_unread = false;
+  if (st[0]) {
+    RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+    c->have_metric = false;
+    c->have_B = false;
+  }
   if (st[0] & 4) fail(RIMP2_ERR_NUMERIC, "non-finite value in an input array");
   if (st[0] & 2) fail(RIMP2_ERR_NOT_SPD, "metric is not positive definite (pivot %d <= 0)", st[1]);
   if (st[0] & 8) fail(RIMP2_ERR_NUMERIC, "orbital-energy denominator e_i+e_j-e_a-e_b >= 0");
 }
 
+// An asynchronous entry point's input checks are read by rimp2_sync; a
+// blocking call that follows it without that sync reads them first, so an
+// error is reported (once) instead of being cleared unseen.
+void check_pending(Ctx* c) {
+  if (c->status_unread) check_status(c);
+}
+
 void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
 
-// Clears the device status word at the start of a call sequence.
-static void reset_status(Ctx* c) {
+// Clears the device status word at the start of a blocking call (after
+// reporting any error an earlier asynchronous call left unread).
+void reset_status(Ctx* c) {
+  check_pending(c);
+  RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, c->stream));
+}
+
+// The same for an asynchronous call: bits that no call has read yet are kept
+// (the new checks add theirs), so rimp2_sync reports an error of any call
+// queued since the last read, and queuing never waits for the stream.
+void reset_status_async(Ctx* c) {
+  if (c->status_unread) return;
   RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, c->stream));
 }
 
@@ -182,6 +206,33 @@ static void check_slice_bound(int64_t naux, int mode) {
   // padded to at most 16511 auxiliary rows: 5 * 127^2 * 16511 < 2^31
   if (mode_slices(mode) && naux > 16384)
     fail(RIMP2_ERR_ARG, "exact-slice modes support naux <= 16384 (int32 accumulation bound)");
+}
+
+// Refuses an energy whose device buffers cannot fit before anything is
+// allocated or read: the buffers the path needs (metric factor, inverse and
+// its packing, packed X, FP64 B, the mode's packed B and row scales, the
+// staging ring of the host path) against hipMemGetInfo's free memory plus what
+// the context already holds (grow-only buffers are reused or replaced).
+static void require_device_memory(Ctx* c, int64_t naux, int64_t nvirt, int64_t nocc, int mode, bool from_host) {
+  const size_t n2 = (size_t)naux * (size_t)naux * sizeof(double);
+  const size_t b64 = (size_t)naux * (size_t)nvirt * (size_t)nocc * sizeof(double);
+  const PackGeom g = make_geom(mode, naux, nvirt, choose_tile(mode, nvirt, nocc));
+  size_t need = 3 * n2 + mg::linv_packed_bytes(naux) + mg::x_packed_bytes(naux, nvirt, nocc) + b64 +
+                g.bytes_per_occ() * (size_t)nocc + (size_t)g.nvp * (size_t)nocc * sizeof(double);
+  if (from_host) {
+    const size_t per = (size_t)naux * (size_t)nvirt * sizeof(double);
+    need += 2 * std::max(per, std::min(b64, (size_t)32 << 20));
+  }
+  size_t held = 0;
+  for (const DevBuf* b : {&c->V, &c->Linv, &c->work, &c->linv_packed, &c->X, &c->B64, &c->packed, &c->scales,
+                          &c->x_packed})
+    held += b->bytes;
+  size_t free_b = 0, total_b = 0;
+  RIMP2_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if (need > free_b + held)
+    fail(RIMP2_ERR_NOMEM, "energy for naux=%lld nvirt=%lld nocc=%lld needs %.2f GiB of device memory; %.2f GiB are "
+         "free and the context holds %.2f GiB", (long long)naux, (long long)nvirt, (long long)nocc,
+         need / 1073741824.0, free_b / 1073741824.0, held / 1073741824.0);
 }
 
 // Packs the FP64 B held in c->B64 ([c->naux, nvirt, nocc]) for the pair kernel.
@@ -213,18 +264,23 @@ static void side_after_stream(Ctx* c) {
 // transformed on the context stream, behind whatever that stream holds by then
 // (the metric factorisation in rimp2_energy), while the host copies the next
 // chunk. A non-finite value is reported by the next check_status.
+// The raw chunks alternate between two slots of c->X: the side stream is in
+// order, so the copy of chunk k + 2 starts only after chunk k has been checked
+// and packed (the transform reads the packed copy), and the raw (P|ia) never
+// needs more than two chunks of device memory.
 static void transform_from_host(Ctx* c, int64_t nvirt, int64_t nocc, const double* Pia) {
   const int64_t n = c->naux;
   const int64_t per = n * nvirt;  // elements per occupied orbital
-  const int64_t step = std::max<int64_t>(1, ((int64_t)32 << 20) / (per * (int64_t)sizeof(double)));
-  double* dX = static_cast<double*>(c->X.ensure((size_t)(per * nocc) * sizeof(double)));
+  const int64_t step = std::min(nocc, std::max<int64_t>(1, ((int64_t)32 << 20) / (per * (int64_t)sizeof(double))));
+  const int64_t slots = std::min<int64_t>(2, ceil_div(nocc, step));
+  double* dX = static_cast<double*>(c->X.ensure((size_t)(per * step * slots) * sizeof(double)));
   double* B = static_cast<double*>(c->B64.ensure((size_t)(per * nocc) * sizeof(double)));
   const MetricView m = ctx_metric_view(c);
   metric_reserve_x(m, n, nvirt, nocc);
   int* status = static_cast<int*>(c->status.ensure(64));
-  for (int64_t i0 = 0; i0 < nocc; i0 += step) {
+  for (int64_t i0 = 0, k = 0; i0 < nocc; i0 += step, k++) {
     const int64_t ni = std::min(step, nocc - i0);
-    double* dXi = dX + i0 * per;
+    double* dXi = dX + (k % slots) * step * per;
     h2d_copy(c, c->side_pack, dXi, Pia + i0 * per, (size_t)(ni * per) * sizeof(double));
     check_finite_on(c->side_pack, status, dXi, ni * per, 512);
     metric_pack_x_range(m, c->side_pack, n, nvirt, i0, ni, dXi);
@@ -413,7 +469,7 @@ int rimp2_energy_fitted_device(void* ctx, int64_t naux, int64_t nvirt, int64_t n
   validate_dims(naux, nvirt, nocc);
   validate_mode(mode);
   if (!d_Calpha || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
-  reset_status(c);
+  reset_status_async(c);
   check_inputs_device(c, d_Calpha, naux * nvirt * nocc, "Calpha");
   check_inputs_device(c, d_eps_o, nocc, "eps_o");
   check_inputs_device(c, d_eps_v, nvirt, "eps_v");
@@ -421,6 +477,7 @@ int rimp2_energy_fitted_device(void* ctx, int64_t naux, int64_t nvirt, int64_t n
   load_fitted_device(c, naux, nvirt, nocc, d_Calpha, mode, hipMemcpyDeviceToDevice);
   pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,
                        d_e_tot, nullptr, 0);
+  c->status_unread = true;
   RIMP2_API_END(ctx)
 }
 
@@ -428,6 +485,7 @@ int rimp2_pair_energies(void* ctx, int64_t nocc, int64_t nvirt, const double* ep
                         double* eps_pair, double* e_tot) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
+  check_pending(c);
   validate_mode(mode);
   if (!eps_o || !eps_v || !e_tot) fail(RIMP2_ERR_ARG, "null eps_o / eps_v / e_tot");
   if (!c->have_B) fail(RIMP2_ERR_STATE, "rimp2_transform or rimp2_load_fitted must run before rimp2_pair_energies");
@@ -464,6 +522,7 @@ int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const dou
   validate_mode(mode);
   if (!Pia || !V || !eps_o || !eps_v || !e_tot) fail(RIMP2_ERR_ARG, "null Pia / V / eps_o / eps_v / e_tot");
   check_slice_bound(naux, mode);
+  require_device_memory(c, naux, nvirt, nocc, mode, true);
   check_host_finite(eps_o, nocc, "eps_o");
   check_host_finite(eps_v, nvirt, "eps_v");
   check_host_denominators(eps_o, nocc, eps_v, nvirt);
@@ -506,7 +565,8 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   validate_mode(mode);
   if (!d_Pia || !d_V || !d_eps_o || !d_eps_v || !d_e_tot) fail(RIMP2_ERR_ARG, "null device pointer");
   check_slice_bound(naux, mode);
-  reset_status(c);
+  require_device_memory(c, naux, nvirt, nocc, mode, false);
+  reset_status_async(c);
   // (P|ia) depends on nothing computed here: its check and its packing into
   // transform order run on a side stream, under the Cholesky
   int* status = static_cast<int*>(c->status.ensure(64));
@@ -529,12 +589,14 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   pack_current_B(c, nvirt, nocc, mode);
   pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,
                        d_e_tot, nullptr, 0);
+  c->status_unread = true;
   RIMP2_API_END(ctx)
 }
 
 int rimp2_get_cholesky(void* ctx, double* L) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
+  check_pending(c);
   if (!c->have_metric) fail(RIMP2_ERR_STATE, "no metric factored");
   if (!L) fail(RIMP2_ERR_ARG, "null L");
   RIMP2_HIP_CHECK(hipMemcpyAsync(L, c->V.ptr, (size_t)(c->naux * c->naux) * sizeof(double), hipMemcpyDeviceToHost,
@@ -546,6 +608,7 @@ int rimp2_get_cholesky(void* ctx, double* L) {
 int rimp2_get_metric_inverse(void* ctx, double* Linv) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
+  check_pending(c);
   if (!c->have_metric) fail(RIMP2_ERR_STATE, "no metric factored");
   if (!Linv) fail(RIMP2_ERR_ARG, "null Linv");
   RIMP2_HIP_CHECK(hipMemcpyAsync(Linv, c->Linv.ptr, (size_t)(c->naux * c->naux) * sizeof(double),
@@ -557,6 +620,7 @@ int rimp2_get_metric_inverse(void* ctx, double* Linv) {
 int rimp2_get_B(void* ctx, double* B) {
   RIMP2_API_BEGIN
   Ctx* c = as_ctx(ctx);
+  check_pending(c);
   if (!c->have_B) fail(RIMP2_ERR_STATE, "no transform done");
   if (!B) fail(RIMP2_ERR_ARG, "null B");
   RIMP2_HIP_CHECK(hipMemcpyAsync(B, c->B64.ptr, (size_t)(c->naux * c->nvirt * c->nocc) * sizeof(double),

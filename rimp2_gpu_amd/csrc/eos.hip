@@ -864,8 +864,16 @@ int rimp2_eos_arrays_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t nocc
   if (!d_Calpha || !d_EVocc || !d_EVvirt || !d_UoccEOST || !d_UvirtT || !d_UoccT || !d_UvirtEOST || !d_toccEOS ||
       !d_tvirtEOS || !d_goccEOS || !d_gvirtEOS)
     fail(RIMP2_ERR_ARG, "null device pointer");
+  reset_status_async(c);
+  {
+    const int64_t nv = nvirt, no = nocc;
+    const int64_t nin[7] = {nba * nv * no, no, nv, no * noccEOS, nv * nv, no * no, nv * nvirtEOS};
+    const double* din[7] = {d_Calpha, d_EVocc, d_EVvirt, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST};
+    for (int i = 0; i < 7; i++) check_inputs_device(c, din[i], nin[i], "eos input");
+  }
   eos_arrays_device(c, nvirt, nocc, noccEOS, nvirtEOS, nba, d_Calpha, d_EVocc, d_EVvirt, d_UoccEOST, d_UvirtT,
                     d_UoccT, d_UvirtEOST, d_toccEOS, d_tvirtEOS, d_goccEOS, d_gvirtEOS, c->eos_chunk);
+  c->status_unread = true;
   RIMP2_API_END(ctx)
 }
 
@@ -897,10 +905,13 @@ int rimp2_eos_arrays(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, in
   }
   unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
   unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
+  reset_status(c);
   for (int i = 0; i < 7; i++)
     h2d_copy(c, c->stream, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
   const double* d[7];
   for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
+  for (int i = 0; i < 7; i++) check_inputs_device(c, d[i], nin[i], "eos input");
+  check_status(c);
   double* o[4];
   for (int i = 0; i < 4; i++) o[i] = reinterpret_cast<double*>(dout + offo[i]);
   eos_arrays_device(c, nv, no, nE, vE, nba, d[0], d[1], d[2], d[3], d[4], d[5], d[6], o[0], o[1], o[2], o[3], c->eos_chunk);
@@ -921,8 +932,17 @@ int rimp2_first_order_device(void* ctx, int64_t nvirt, int64_t nocc, int64_t noc
   if (!d_Calpha || !d_CalphaOcc || !d_CalphaVV || !d_UoccEOST || !d_UvirtT || !d_UoccT || !d_UvirtEOST || !d_djik ||
       !d_blad)
     fail(RIMP2_ERR_ARG, "null device pointer");
+  reset_status_async(c);
+  {
+    const int64_t nv = nvirt, no = nocc;
+    const int64_t nin[7] = {nba * nv * no, nba * no * nocctot, nba * nv * nv, no * noccEOS, nv * nv, no * no,
+                            nv * nvirtEOS};
+    const double* din[7] = {d_Calpha, d_CalphaOcc, d_CalphaVV, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST};
+    for (int i = 0; i < 7; i++) check_inputs_device(c, din[i], nin[i], "first-order input");
+  }
   first_order_device(c, nvirt, nocc, noccEOS, nvirtEOS, nba, d_Calpha, d_CalphaOcc, d_CalphaVV, d_UoccEOST, d_UvirtT,
                      d_UoccT, d_UvirtEOST, d_djik, d_blad);
+  c->status_unread = true;
   RIMP2_API_END(ctx)
 }
 
@@ -952,10 +972,13 @@ int rimp2_first_order(void* ctx, int64_t nvirt, int64_t nocc, int64_t noccEOS, i
   }
   unsigned char* din = static_cast<unsigned char*>(c->eos->in.ensure(bi));
   unsigned char* dout = static_cast<unsigned char*>(c->eos->out.ensure(bo));
+  reset_status(c);
   for (int i = 0; i < 7; i++)
     h2d_copy(c, c->stream, din + offi[i], hin[i], (size_t)nin[i] * sizeof(double));
   const double* d[7];
   for (int i = 0; i < 7; i++) d[i] = reinterpret_cast<const double*>(din + offi[i]);
+  for (int i = 0; i < 7; i++) check_inputs_device(c, d[i], nin[i], "first-order input");
+  check_status(c);
   double* o[2];
   for (int i = 0; i < 2; i++) o[i] = reinterpret_cast<double*>(dout + offo[i]);
   first_order_device(c, nv, no, nE, vE, nba, d[0], d[1], d[2], d[3], d[4], d[5], d[6], o[0], o[1]);

@@ -471,6 +471,7 @@ static int64_t features_device(Ctx* c, int64_t nocc, int64_t nvirt, const double
 }
 
 static void check_B(Ctx* c, int64_t nocc, int64_t nvirt) {
+  check_pending(c);
   if (!c->have_B) fail(RIMP2_ERR_STATE, "rimp2_transform must run first");
   if (c->nocc != nocc || c->nvirt != nvirt)
     fail(RIMP2_ERR_STATE, "features requested for (nocc=%lld, nvirt=%lld) but the transform holds (%lld, %lld)",

@@ -93,6 +93,9 @@ struct Ctx {
   int tile = 0;  // pair tile (virtual block) used for the packed layout
   // pair state
   DevBuf eps_o, eps_v, partials, eps_pair, etot, status;
+  // an asynchronous entry point queued input checks whose status word no
+  // call has read yet (rimp2_sync, or the next blocking call, reads it)
+  bool status_unread = false;
   // timing
   bool timing = false;
   Timer timers[RIMP2_NUM_KERNELS];
@@ -205,6 +208,16 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
                           int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
                           double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
+// Device status word of the context (ctx.hip): check_status waits for the
+// stream and turns set bits into an error (clearing them); check_pending does
+// so only when an asynchronous call left them unread; reset_status clears the
+// word at the start of a call (after check_pending).
+// reset_status_async is the reset of an asynchronous call: it keeps bits no
+// call has read yet.
+void check_status(Ctx* c);
+void check_pending(Ctx* c);
+void reset_status(Ctx* c);
+void reset_status_async(Ctx* c);
 // Amplitude slab of occupied orbitals [i0, i0 + nchunk) from the FP64 or
 // i8s5 packing (tile 64; sigma: i8s5 row scales, else null):
 // T[a + nv b + nv^2 (j - i0) + nv^2 (nocc - i0) (i - i0)] =

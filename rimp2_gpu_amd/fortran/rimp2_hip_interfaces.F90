@@ -10,8 +10,15 @@ module rimp2_hip_interfaces
   implicit none
 
   integer(C_INT), parameter :: RIMP2_OK = 0
+  integer(C_INT), parameter :: RIMP2_ERR_ARG = 1, RIMP2_ERR_HIP = 2, RIMP2_ERR_NOT_SPD = 3, RIMP2_ERR_STATE = 4
+  integer(C_INT), parameter :: RIMP2_ERR_NOMEM = 5, RIMP2_ERR_NUMERIC = 6
   integer(C_INT), parameter :: RIMP2_MODE_FP64 = 0, RIMP2_MODE_FP32 = 1, RIMP2_MODE_BF16X3 = 2
   integer(C_INT), parameter :: RIMP2_MODE_I8S4 = 3, RIMP2_MODE_I8S5 = 4
+  ! kernel ids of the event timers (rimp2_timing_get)
+  integer(C_INT), parameter :: RIMP2_K_PAIR = 0, RIMP2_K_TRANSFORM = 1, RIMP2_K_PACK = 2, RIMP2_K_CHOLESKY = 3
+  integer(C_INT), parameter :: RIMP2_K_TRTRI = 4, RIMP2_K_REDUCE = 5, RIMP2_K_BATCH = 6, RIMP2_K_MLP = 7
+  integer(C_INT), parameter :: RIMP2_K_FEATURES = 8, RIMP2_K_EOS = 9, RIMP2_K_EOS_AMPL = 10
+  integer(C_INT), parameter :: RIMP2_NFEAT = 16
 
   type, bind(C) :: rimp2_frag_desc
      integer(C_INT64_T) :: nocc, nvirt, naux
@@ -161,6 +168,272 @@ module rimp2_hip_interfaces
        integer(C_INT64_T), value :: m, n, k, lda, ldb, ldc
        real(C_DOUBLE) :: alpha, beta
      end function rimp2_dgemm
+
+     ! ---- device-resident energy path (device pointers, asynchronous) ----
+
+     integer(C_INT) function rimp2_energy_device(ctx, naux, nvirt, nocc, d_Pia, d_V, d_eps_o, d_eps_v, mode, &
+          & d_eps_pair, d_e_tot) bind(C, name="rimp2_energy_device")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: naux, nvirt, nocc
+       type(C_PTR), value :: d_Pia, d_V, d_eps_o, d_eps_v
+       integer(C_INT), value :: mode
+       type(C_PTR), value :: d_eps_pair, d_e_tot
+     end function rimp2_energy_device
+
+     integer(C_INT) function rimp2_energy_fitted_device(ctx, naux, nvirt, nocc, d_Calpha, d_eps_o, d_eps_v, mode, &
+          & d_eps_pair, d_e_tot) bind(C, name="rimp2_energy_fitted_device")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: naux, nvirt, nocc
+       type(C_PTR), value :: d_Calpha, d_eps_o, d_eps_v
+       integer(C_INT), value :: mode
+       type(C_PTR), value :: d_eps_pair, d_e_tot
+     end function rimp2_energy_fitted_device
+
+     ! ---- intermediate results (host arrays, blocking) ----
+
+     integer(C_INT) function rimp2_get_cholesky(ctx, L) bind(C, name="rimp2_get_cholesky")
+       import :: C_INT, C_PTR, C_DOUBLE
+       type(C_PTR), value :: ctx
+       real(C_DOUBLE) :: L(*)
+     end function rimp2_get_cholesky
+
+     integer(C_INT) function rimp2_get_metric_inverse(ctx, Linv) bind(C, name="rimp2_get_metric_inverse")
+       import :: C_INT, C_PTR, C_DOUBLE
+       type(C_PTR), value :: ctx
+       real(C_DOUBLE) :: Linv(*)
+     end function rimp2_get_metric_inverse
+
+     integer(C_INT) function rimp2_get_B(ctx, B) bind(C, name="rimp2_get_B")
+       import :: C_INT, C_PTR, C_DOUBLE
+       type(C_PTR), value :: ctx
+       real(C_DOUBLE) :: B(*)
+     end function rimp2_get_B
+
+     ! ---- batched fragments ----
+
+     integer(C_INT) function rimp2_batch_prepare(ctx, nfrag, frags, mode) bind(C, name="rimp2_batch_prepare")
+       import :: C_INT, C_PTR, C_INT64_T, rimp2_frag_desc
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nfrag
+       type(rimp2_frag_desc) :: frags(*)
+       integer(C_INT), value :: mode
+     end function rimp2_batch_prepare
+
+     integer(C_INT) function rimp2_batch_prepare_device(ctx, nfrag, frags, mode) &
+          & bind(C, name="rimp2_batch_prepare_device")
+       import :: C_INT, C_PTR, C_INT64_T, rimp2_frag_desc
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nfrag
+       type(rimp2_frag_desc) :: frags(*)
+       integer(C_INT), value :: mode
+     end function rimp2_batch_prepare_device
+
+     !> Replays the captured pair stage; d_e_frag (device, may be C_NULL_PTR)
+     integer(C_INT) function rimp2_batch_run(ctx, d_e_frag) bind(C, name="rimp2_batch_run")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx, d_e_frag
+     end function rimp2_batch_run
+
+     integer(C_INT) function rimp2_batch_result(ctx, e_frag, eps_pair_packed) bind(C, name="rimp2_batch_result")
+       import :: C_INT, C_PTR, C_DOUBLE
+       type(C_PTR), value :: ctx
+       real(C_DOUBLE) :: e_frag(*)
+       type(C_PTR), value :: eps_pair_packed
+     end function rimp2_batch_result
+
+     integer(C_INT) function rimp2_batch_tile(ctx, tile) bind(C, name="rimp2_batch_tile")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T) :: tile
+     end function rimp2_batch_tile
+
+     integer(C_INT) function rimp2_batch_tuning(ctx, n, tiles, ms) bind(C, name="rimp2_batch_tuning")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT) :: n
+       integer(C_INT64_T) :: tiles(*)
+       real(C_DOUBLE) :: ms(*)
+     end function rimp2_batch_tuning
+
+     ! ---- pair-screening surrogate ----
+
+     !> feats(16, npairs): the C array [npairs, 16] seen column-major
+     integer(C_INT) function rimp2_pair_features(ctx, nocc, nvirt, eps_o, eps_v, feats) &
+          & bind(C, name="rimp2_pair_features")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE, C_FLOAT
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nocc, nvirt
+       real(C_DOUBLE) :: eps_o(*), eps_v(*)
+       real(C_FLOAT) :: feats(*)
+     end function rimp2_pair_features
+
+     integer(C_INT) function rimp2_mlp_init(ctx, seed) bind(C, name="rimp2_mlp_init")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: seed
+     end function rimp2_mlp_init
+
+     integer(C_INT) function rimp2_mlp_set_norm(ctx, mean, stdev, tmean, tstd) bind(C, name="rimp2_mlp_set_norm")
+       import :: C_INT, C_PTR, C_FLOAT
+       type(C_PTR), value :: ctx
+       real(C_FLOAT) :: mean(*), stdev(*)
+       real(C_FLOAT), value :: tmean, tstd
+     end function rimp2_mlp_set_norm
+
+     integer(C_INT) function rimp2_mlp_train_step(ctx, n, feats, target, lr, loss) bind(C, name="rimp2_mlp_train_step")
+       import :: C_INT, C_PTR, C_INT64_T, C_FLOAT
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: n
+       real(C_FLOAT) :: feats(*), target(*)
+       real(C_FLOAT), value :: lr
+       real(C_FLOAT) :: loss
+     end function rimp2_mlp_train_step
+
+     integer(C_INT) function rimp2_mlp_predict(ctx, n, feats, pred) bind(C, name="rimp2_mlp_predict")
+       import :: C_INT, C_PTR, C_INT64_T, C_FLOAT
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: n
+       real(C_FLOAT) :: feats(*), pred(*)
+     end function rimp2_mlp_predict
+
+     integer(C_INT) function rimp2_mlp_get_params(ctx, params, n) bind(C, name="rimp2_mlp_get_params")
+       import :: C_INT, C_PTR, C_INT64_T, C_FLOAT
+       type(C_PTR), value :: ctx
+       real(C_FLOAT) :: params(*)
+       integer(C_INT64_T), value :: n
+     end function rimp2_mlp_get_params
+
+     integer(C_INT) function rimp2_mlp_set_params(ctx, params, n) bind(C, name="rimp2_mlp_set_params")
+       import :: C_INT, C_PTR, C_INT64_T, C_FLOAT
+       type(C_PTR), value :: ctx
+       real(C_FLOAT) :: params(*)
+       integer(C_INT64_T), value :: n
+     end function rimp2_mlp_set_params
+
+     integer(C_INT64_T) function rimp2_mlp_num_params() bind(C, name="rimp2_mlp_num_params")
+       import :: C_INT64_T
+     end function rimp2_mlp_num_params
+
+     integer(C_INT) function rimp2_mlp_grad(ctx, n, feats, target, loss, grad) bind(C, name="rimp2_mlp_grad")
+       import :: C_INT, C_PTR, C_INT64_T, C_FLOAT
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: n
+       real(C_FLOAT) :: feats(*), target(*), loss, grad(*)
+     end function rimp2_mlp_grad
+
+     integer(C_INT) function rimp2_screened_energy(ctx, nocc, nvirt, eps_o, eps_v, mode, thresh, eps_pair, e_tot, &
+          & nskipped) bind(C, name="rimp2_screened_energy")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nocc, nvirt
+       real(C_DOUBLE) :: eps_o(*), eps_v(*)
+       integer(C_INT), value :: mode
+       real(C_DOUBLE), value :: thresh
+       real(C_DOUBLE) :: eps_pair(*), e_tot
+       integer(C_INT64_T) :: nskipped
+     end function rimp2_screened_energy
+
+     ! ---- device memory, copies, stream ----
+
+     integer(C_INT) function rimp2_device_alloc(ctx, bytes, dptr) bind(C, name="rimp2_device_alloc")
+       import :: C_INT, C_PTR, C_SIZE_T
+       type(C_PTR), value :: ctx
+       integer(C_SIZE_T), value :: bytes
+       type(C_PTR) :: dptr
+     end function rimp2_device_alloc
+
+     integer(C_INT) function rimp2_device_free(ctx, dptr) bind(C, name="rimp2_device_free")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx, dptr
+     end function rimp2_device_free
+
+     integer(C_INT) function rimp2_copy_h2d(ctx, dst, src, bytes) bind(C, name="rimp2_copy_h2d")
+       import :: C_INT, C_PTR, C_SIZE_T
+       type(C_PTR), value :: ctx, dst, src
+       integer(C_SIZE_T), value :: bytes
+     end function rimp2_copy_h2d
+
+     integer(C_INT) function rimp2_copy_d2h(ctx, dst, src, bytes) bind(C, name="rimp2_copy_d2h")
+       import :: C_INT, C_PTR, C_SIZE_T
+       type(C_PTR), value :: ctx, dst, src
+       integer(C_SIZE_T), value :: bytes
+     end function rimp2_copy_d2h
+
+     integer(C_INT) function rimp2_sync(ctx) bind(C, name="rimp2_sync")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx
+     end function rimp2_sync
+
+     ! ---- event timers ----
+
+     integer(C_INT) function rimp2_timing_enable(ctx, on) bind(C, name="rimp2_timing_enable")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx
+       integer(C_INT), value :: on
+     end function rimp2_timing_enable
+
+     integer(C_INT) function rimp2_timing_get(ctx, kid, total_ms, count) bind(C, name="rimp2_timing_get")
+       import :: C_INT, C_PTR, C_DOUBLE, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT), value :: kid
+       real(C_DOUBLE) :: total_ms
+       integer(C_INT64_T) :: count
+     end function rimp2_timing_get
+
+     integer(C_INT) function rimp2_timing_reset(ctx) bind(C, name="rimp2_timing_reset")
+       import :: C_INT, C_PTR
+       type(C_PTR), value :: ctx
+     end function rimp2_timing_reset
+
+     ! ---- seeded synthetic inputs ----
+
+     integer(C_INT) function rimp2_gen_fragment_local(seed, naux, nvirt, nocc, lam, Pia, V, eps_o, eps_v) &
+          & bind(C, name="rimp2_gen_fragment_local")
+       import :: C_INT, C_INT64_T, C_DOUBLE
+       integer(C_INT64_T), value :: seed, naux, nvirt, nocc
+       real(C_DOUBLE), value :: lam
+       real(C_DOUBLE) :: Pia(*), V(*), eps_o(*), eps_v(*)
+     end function rimp2_gen_fragment_local
+
+     integer(C_INT) function rimp2_gen_fragment_device(ctx, seed, naux, nvirt, nocc, lam, d_Pia, d_V, d_eps_o, &
+          & d_eps_v) bind(C, name="rimp2_gen_fragment_device")
+       import :: C_INT, C_PTR, C_INT64_T, C_DOUBLE
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: seed, naux, nvirt, nocc
+       real(C_DOUBLE), value :: lam
+       type(C_PTR), value :: d_Pia, d_V, d_eps_o, d_eps_v
+     end function rimp2_gen_fragment_device
+
+     integer(C_INT) function rimp2_gen_batch_shapes(seed, nfrag, nocc, nvirt, naux) &
+          & bind(C, name="rimp2_gen_batch_shapes")
+       import :: C_INT, C_INT64_T
+       integer(C_INT64_T), value :: seed, nfrag
+       integer(C_INT64_T) :: nocc(*), nvirt(*), naux(*)
+     end function rimp2_gen_batch_shapes
+
+     ! ---- 4-index arrays and first order, device pointers ----
+
+     integer(C_INT) function rimp2_eos_arrays_device(ctx, nvirt, nocc, noccEOS, nvirtEOS, nba, d_Calpha, d_EVocc, &
+          & d_EVvirt, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST, d_toccEOS, d_tvirtEOS, d_goccEOS, d_gvirtEOS) &
+          & bind(C, name="rimp2_eos_arrays_device")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nvirt, nocc, noccEOS, nvirtEOS, nba
+       type(C_PTR), value :: d_Calpha, d_EVocc, d_EVvirt, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST
+       type(C_PTR), value :: d_toccEOS, d_tvirtEOS, d_goccEOS, d_gvirtEOS
+     end function rimp2_eos_arrays_device
+
+     integer(C_INT) function rimp2_first_order_device(ctx, nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot, d_Calpha, &
+          & d_CalphaOcc, d_CalphaVV, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST, d_djik, d_blad) &
+          & bind(C, name="rimp2_first_order_device")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: nvirt, nocc, noccEOS, nvirtEOS, nba, nocctot
+       type(C_PTR), value :: d_Calpha, d_CalphaOcc, d_CalphaVV, d_UoccEOST, d_UvirtT, d_UoccT, d_UvirtEOST
+       type(C_PTR), value :: d_djik, d_blad
+     end function rimp2_first_order_device
 
   end interface
 

@@ -105,26 +105,6 @@ def test_batch_errors(ctx, rimp2):
     assert ef[0] < 0
 
 
-def test_dec_batch_256_fragments(ctx, oracle, rimp2):
-    """The 256-fragment DEC batch (nocc~U{4..32}, nvirt~U{32..256},
-    naux=4 nvirt): fp32 and i8s5 through the persistent launch; a seeded
-    subset of fragments is checked against the oracle."""
-    frags = make_batch(rimp2, 3, 256)
-    say("dec batch 256")
-    e32, _ = ctx.batch(frags, "fp32")
-    e8, _ = ctx.batch(frags, "i8s5")
-    assert np.all(e32 < 0) and np.all(e8 < 0)
-    rel = np.abs(e32 - e8) / np.abs(e8)
-    assert rel.max() < 2e-7
-    idx = [0, 17, 101, 255]
-    for k in idx:
-        P, V, eo, ev = frags[k]
-        B = oracle.metric_transform(V, P)
-        Eref, _ = oracle.pair_energies(B, eo, ev)
-        say("frag", k, P.shape, "dE_i8s5 uEh", (e8[k] - Eref) * 1e6)
-        assert abs(e8[k] - Eref) < 1e-6
-
-
 @pytest.mark.parametrize("mode", ["fp64", "i8s5"])
 def test_batch_mixes_fitted_and_metric_fragments(ctx, oracle, rimp2, mode):
     """A descriptor with V = NULL carries fitted coefficients (B = Pia)."""

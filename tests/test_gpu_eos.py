@@ -255,3 +255,46 @@ def test_dgemm_rejects_bad_arguments(ctx, rimp2):
             assert e.value.code == rimp2.ERR_ARG, args
     finally:
         ctx.free(p)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5"])
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_eos_rejects_non_finite_inputs(ctx, rimp2, mode, bad):
+    """A NaN or Inf in any input fails with RIMP2_ERR_NUMERIC before an output
+    is written (the exact-slice packer would otherwise turn it into finite
+    slices), in the blocking and the device forms; the context stays usable."""
+    dims = (40, 6, 3, 5, 50)
+    nvirt, nocc, noccEOS, nvirtEOS, nba = dims
+    say("eos non-finite", mode, bad)
+    args = list(random_inputs(81, *dims))
+    ctx.eos_set_mode(mode)
+    try:
+        for which, idx in ((0, (7, 3, 1)), (1, (2,)), (4, (5, 9))):
+            a = [x.copy(order="F") for x in args]
+            a[which][idx] = bad
+            with pytest.raises(rimp2.Rimp2Error) as ei:
+                ctx.eos_arrays(*a)
+            assert ei.value.code == rimp2.ERR_NUMERIC, which
+        # device form: reported by the next sync
+        a = [x.copy(order="F") for x in args]
+        a[0][0, 0, 0] = bad
+        d_in = [ctx.to_device(x) for x in a]
+        shapes = [(nvirt, noccEOS, nvirt, noccEOS), (nvirtEOS, nocc, nvirtEOS, nocc)] * 2
+        d_out = [ctx.alloc(int(np.prod(s)) * 8) for s in shapes]
+        ctx.eos_arrays_device(*dims, d_in, d_out)
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.sync()
+        assert ei.value.code == rimp2.ERR_NUMERIC
+        for p in d_in + d_out:
+            ctx.free(p)
+        # first order: NaN in CalphaVV
+        fo = list(first_order_inputs(np.random.default_rng(82), nba, nvirt, nocc, noccEOS, nvirtEOS))
+        fo[2] = fo[2].copy(order="F")
+        fo[2][1, 2, 3] = bad
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.first_order(*fo)
+        assert ei.value.code == rimp2.ERR_NUMERIC
+        outs = ctx.eos_arrays(*args)
+        assert all(np.all(np.isfinite(o)) for o in outs)
+    finally:
+        ctx.eos_set_mode("fp64")

@@ -225,3 +225,83 @@ def test_first_order_record_file(tmp_path, oracle):
     write_records(path, [hdr, C, eo, ev, UoE, Uv, Uo, UvE, Co, Cvv, to, tv, gv, go, dj, bl2])
     rc, out = run("file", path, 4, "fo")
     assert rc != 0 and "ERROR3" in out and "blad" in out
+
+
+def parse_lines(out, key):
+    return [line.split()[1:] for line in out.splitlines() if line.split()[:1] == [key]]
+
+
+@needs_exe
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 4])
+def test_batch_mode_against_the_oracle(oracle, rimp2, mode):
+    """rimp2_batch from host arrays and rimp2_batch_prepare_device / _run /
+    _result on device copies, driven from Fortran: every fragment's energy
+    against the oracle, and the two paths bit for bit."""
+    seed, nfrag = 21, 12
+    say("fortran batch", seed, nfrag, "mode", mode)
+    rc, out = run("batch", seed, nfrag, mode, timeout=300)
+    assert rc == 0, out
+    rows = parse_lines(out, "E_FRAG")
+    assert len(rows) == nfrag
+    o, v, q = rimp2.gen_batch_shapes(seed, nfrag)
+    tol = 1e-11 if mode == 0 else None
+    for k, (idx, e_host, e_dev) in enumerate(rows):
+        assert int(idx) == k
+        assert float(e_host) == float(e_dev)
+        P, V, eo, ev = rimp2.gen_fragment(seed + k, int(q[k]), int(v[k]), int(o[k]))
+        Eref, _ = oracle.pair_energies(oracle.metric_transform(V, P), eo, ev)
+        if tol:
+            assert abs(float(e_host) - Eref) <= tol * abs(Eref), (k, e_host, Eref)
+        else:
+            assert abs(float(e_host) - Eref) < 1e-6, (k, e_host, Eref)
+    tile = parse_lines(out, "BATCH_TILE")[0]
+    assert int(tile[0]) in (32, 64)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_screen_mode_keeps_exact_pair_energies(oracle, rimp2):
+    """Surrogate training and screening driven from Fortran (locality-decay
+    generator, lam = 1.5): the exact energy against the oracle, the screened
+    energy equal to the oracle's energy less the skipped pairs, and no kept pair
+    differing from the exact kernel's (the driver stops with ERROR3 otherwise)."""
+    seed, naux, nvirt, nocc, lam, ntrain, thresh = 999, 256, 64, 32, 1.5, 12, 1e-6
+    say("fortran screen")
+    rc, out = run("screen", seed, naux, nvirt, nocc, 0, lam, ntrain, thresh, timeout=300)
+    assert rc == 0, out
+    vals = {k: float(parse_lines(out, k)[0][0]) for k in ("E_EXACT", "E_SCREENED", "NSKIPPED")}
+    skips = [(int(a), int(b)) for a, b, _ in parse_lines(out, "SKIP")]
+    assert len(skips) == int(vals["NSKIPPED"]) > 0
+    P, V, eo, ev = rimp2.gen_fragment_local(seed, naux, nvirt, nocc, lam)
+    Eref, epsref = oracle.pair_energies(oracle.metric_transform(V, P), eo, ev)
+    assert abs(vals["E_EXACT"] - Eref) <= 1e-11 * abs(Eref)
+    dropped = sum((1.0 if i == j else 2.0) * epsref[i, j] for i, j in skips)
+    assert abs(vals["E_SCREENED"] - (Eref - dropped)) <= 1e-11 * abs(Eref)
+    assert max(abs(epsref[i, j]) for i, j in skips) < 100 * thresh
+    losses = [float(r[1]) for r in parse_lines(out, "EPOCH_LOSS")]
+    assert len(losses) == 6 and losses[-1] < losses[0]
+
+
+@needs_exe
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 4])
+def test_device_mode_against_the_oracle(oracle, rimp2, mode):
+    """Device buffers, the device generator, rimp2_energy_device + rimp2_sync,
+    the inspection calls and rimp2_energy_fitted_device, all from Fortran."""
+    seed, naux, nvirt, nocc = 8, 192, 40, 6
+    say("fortran device", mode)
+    rc, out = run("device", seed, naux, nvirt, nocc, mode)
+    assert rc == 0, out
+    vals = {k: float(parse_lines(out, k)[0][0]) for k in ("E_DEVICE", "E_FITTED_DEVICE", "PAIR_SUM",
+                                                          "MAX_LINV_L_MINUS_I")}
+    P, V, eo, ev = rimp2.gen_fragment(seed, naux, nvirt, nocc)
+    Eref, _ = oracle.pair_energies(oracle.metric_transform(V, P), eo, ev)
+    # the device generator agrees with the host one to rounding
+    tol = 1e-10 * abs(Eref) + (1e-6 if mode == 4 else 0.0)
+    assert abs(vals["E_DEVICE"] - Eref) <= tol
+    assert vals["E_FITTED_DEVICE"] == vals["E_DEVICE"]
+    assert abs(vals["PAIR_SUM"] - vals["E_DEVICE"]) <= 1e-12 * abs(Eref)
+    assert vals["MAX_LINV_L_MINUS_I"] < 1e-10
+    n, ms = parse_lines(out, "PAIR_KERNEL_LAUNCHES_MS")[0]
+    assert int(n) == 1 and float(ms) > 0

@@ -282,26 +282,6 @@ def test_device_path_flags_bad_input(ctx, rimp2):
     ctx.sync()  # status was cleared
 
 
-def test_c2_shape_exact_slice_within_1uEh(ctx, oracle, rimp2):
-    """64/512/2048 fragment: i8s5 vs fp64 energy within 1 uEh; a seeded subset
-    of pair energies checked against the oracle."""
-    naux, nvirt, nocc = 2048, 512, 64
-    say("c2 shape", naux, nvirt, nocc)
-    P, V, eo, ev = rimp2.gen_fragment(2, naux, nvirt, nocc)
-    E64, eps64 = ctx.energy(P, V, eo, ev, "fp64")
-    B = ctx.get_B(naux, nvirt, nocc)
-    rng = np.random.default_rng(0)
-    pi = rng.integers(0, nocc, 12)
-    pj = rng.integers(0, nocc, 12)
-    ref = oracle.pair_energies_subset(B, eo, ev, pi, pj)
-    np.testing.assert_allclose(eps64[pi, pj], ref, rtol=1e-11)
-    E8, eps8 = ctx.energy(P, V, eo, ev, "i8s5")
-    say("c2 E64", E64, "E_i8s5", E8, "dE uEh", (E8 - E64) * 1e6)
-    assert abs(E8 - E64) < 1e-6
-    np.testing.assert_array_equal(eps8, eps8.T)
-    assert np.all(eps64 < 0)
-
-
 @pytest.mark.parametrize("mode", ["fp64", "i8s5", "fp32", "bf16x3"])
 def test_fitted_coefficients_match_oracle(ctx, oracle, rimp2, mode):
     """Calpha read as the reference reads it (already fitted, B = Calpha)."""
@@ -436,3 +416,85 @@ def test_launches_beyond_32bit_work_items(ctx, oracle, rimp2):
     np.testing.assert_allclose(eps[pi, pj], ref, rtol=1e-11)
     say("E64", E64, "E_i8s5", E8, "dE uEh", (E8 - E64) * 1e6)
     assert abs(E8 - E64) < 1e-6
+
+
+def test_mismatched_shapes_raise_before_the_library(ctx, rimp2):
+    """V, eps_o and eps_v must match (P|ia) [naux, nvirt, nocc]: the library
+    reads V as naux x naux and the orbital energies as nocc / nvirt values."""
+    say("shape checks")
+    P, V, eo, ev = rimp2.gen_fragment(12, 48, 10, 3)
+    for args in ((P, V[:40, :40], eo, ev), (P, V, eo[:2], ev), (P, V, eo, ev[:9]), (P[:, :, 0], V, eo, ev),
+                 (P, V.ravel(), eo, ev)):
+        with pytest.raises(ValueError):
+            ctx.energy(*args)
+    with pytest.raises(ValueError):
+        ctx.energy_fitted(P, eo, ev[:5])
+    with pytest.raises(ValueError):
+        ctx.metric(V[:, :40])
+    with pytest.raises(ValueError):
+        ctx.batch([(P, V, eo[:1], ev)], "fp64")
+    E, _ = ctx.energy(P, V, eo, ev, "fp64")
+    assert E < 0
+
+
+def test_async_error_is_not_dropped_by_a_later_blocking_call(ctx, rimp2):
+    """energy_device with a non-SPD metric and no sync: the next blocking call
+    that uses the context's state (pair_energies) or starts a new energy reports
+    the pending error instead of clearing it; the state it left is invalid."""
+    say("pending async error")
+    naux, nvirt, nocc = 48, 12, 3
+    P, V, eo, ev = rimp2.gen_fragment(13, naux, nvirt, nocc)
+    bad = V.copy(order="F")
+    bad[7, 7] = -50.0
+    d = [ctx.to_device(x) for x in (P, bad, eo, ev)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    try:
+        ctx.energy_device(naux, nvirt, nocc, *d, "fp64", dpe, det)
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.pair_energies(eo, ev, "fp64")
+        assert ei.value.code == rimp2.ERR_NOT_SPD
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.pair_energies(eo, ev, "fp64")  # the state is invalid now, not silently reused
+        assert ei.value.code == rimp2.ERR_STATE
+        ctx.energy_device(naux, nvirt, nocc, *d, "fp64", dpe, det)
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.energy(P, V, eo, ev, "fp64")
+        assert ei.value.code == rimp2.ERR_NOT_SPD
+    finally:
+        for p in d + [dpe, det]:
+            ctx.free(p)
+    E, _ = ctx.energy(P, V, eo, ev, "fp64")
+    assert E < 0
+
+
+def test_energy_larger_than_free_memory_is_refused_before_allocation(ctx, rimp2):
+    """A request whose device buffers exceed hipMemGetInfo's free memory fails
+    with RIMP2_ERR_NOMEM before any allocation or read of the (P|ia) and V
+    arrays (which are deliberately one element long here): naux = nvirt =
+    16384, nocc = 4096 needs ~17 TB."""
+    import ctypes
+    say("nomem precheck")
+    naux, nvirt, nocc = 16384, 16384, 4096
+    t0, f0 = rimp2.get_dev_mem()
+    one = np.zeros(1)
+    eo = np.linspace(-2.0, -0.3, nocc)
+    ev = np.linspace(0.1, 3.0, nvirt)
+    e = np.zeros(1)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L = rimp2.lib()
+    for mode in (0, 4):
+        st = L.rimp2_energy(ctx.h, naux, nvirt, nocc, one.ctypes.data_as(dp), one.ctypes.data_as(dp),
+                            eo.ctypes.data_as(dp), ev.ctypes.data_as(dp), mode, None, e.ctypes.data_as(dp))
+        assert st == rimp2.ERR_NOMEM, (st, rimp2.last_error(ctx.h))
+        assert "GiB" in rimp2.last_error(ctx.h)
+    dummy = ctx.alloc(8)
+    try:
+        st = L.rimp2_energy_device(ctx.h, naux, nvirt, nocc, dummy, dummy, dummy, dummy, 4, None, dummy)
+        assert st == rimp2.ERR_NOMEM, (st, rimp2.last_error(ctx.h))
+    finally:
+        ctx.free(dummy)
+    t1, f1 = rimp2.get_dev_mem()
+    assert f1 >= f0 - (64 << 20)  # nothing was allocated
+    P, V, eo2, ev2 = rimp2.gen_fragment(14, 40, 8, 2)
+    E, _ = ctx.energy(P, V, eo2, ev2, "fp64")
+    assert E < 0
