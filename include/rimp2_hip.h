@@ -124,6 +124,23 @@ int rimp2_energy_fitted_device(void* ctx, int64_t naux, int64_t nvirt, int64_t n
                                const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair,
                                double* d_e_tot);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rimp2_energy_fitted_device with every buffer beyond the
+ * inputs and outputs (packed operands, row scales, partials, status word) in
+ * one static device array of RIMP2_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; the call allocates nothing.  A fragment whose buffers do
+ * not fit is refused with RIMP2_ERR_NOMEM before anything is queued (the
+ * 16/128/512 fragment needs ~9 MiB in fp64, a 32/256/1024 one ~67 MiB).
+ * Asynchronous on the context stream; input checks are reported by
+ * rimp2_sync.  All calls on one GPU share its array and the last call's
+ * contents stay there: calls that overlap on one GPU (other streams, contexts
+ * or threads) race. */
+#define RIMP2_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int rimp2_energy_fitted_device_noalloc(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const double* d_Calpha,
+                                       const double* d_eps_o, const double* d_eps_v, int mode, double* d_eps_pair,
+                                       double* d_e_tot);
+
 /* ---- inspection of intermediate results (host pointers, blocking) ---- */
 int rimp2_get_cholesky(void* ctx, double* L);      /* L [naux,naux], lower, zeros above */
 int rimp2_get_metric_inverse(void* ctx, double* Linv); /* L^{-1} [naux,naux] */

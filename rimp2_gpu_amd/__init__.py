@@ -23,6 +23,7 @@ KERNELS = {"pair": 0, "transform": 1, "pack": 2, "cholesky": 3, "trtri": 4, "red
            "batch": 6, "mlp": 7, "features": 8, "eos": 9,
            "eos_ampl": 10}
 NFEAT = 16
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 _i64 = ctypes.c_int64
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -58,6 +59,8 @@ SIGNATURES = {
     "rimp2_load_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, ctypes.c_int]),
     "rimp2_energy_fitted": (ctypes.c_int, [_vp, _i64, _i64, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "rimp2_energy_fitted_device": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "rimp2_energy_fitted_device_noalloc": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                                          _vp]),
     "rimp2_get_cholesky": (ctypes.c_int, [_vp, _dp]),
     "rimp2_get_metric_inverse": (ctypes.c_int, [_vp, _dp]),
     "rimp2_get_B": (ctypes.c_int, [_vp, _dp]),
@@ -269,6 +272,12 @@ class Context:
 
     def energy_fitted_device(self, naux, nvirt, nocc, dB, deo, dev, mode, dpairs, detot):
         self._chk(lib().rimp2_energy_fitted_device(self.h, naux, nvirt, nocc, dB, deo, dev, _mode(mode), dpairs, detot))
+
+    def energy_fitted_device_noalloc(self, naux, nvirt, nocc, dB, deo, dev, mode, dpairs, detot):
+        """rimp2_energy_fitted_device with no device allocation: the scratch
+        is the device's static array (NOALLOC_SCRATCH_BYTES)."""
+        self._chk(lib().rimp2_energy_fitted_device_noalloc(self.h, naux, nvirt, nocc, dB, deo, dev, _mode(mode),
+                                                           dpairs, detot))
 
     def pair_energies(self, eps_o, eps_v, mode="fp64", want_pairs=True):
         eo, ev = _f64(eps_o), _f64(eps_v)

@@ -150,14 +150,36 @@ static void check_host_denominators(const double* eo, int64_t nocc, const double
 // Reads and clears the device status word; converts set bits to an error.
 // A set bit also invalidates the metric and the transformed B the context
 // holds: they were formed from the rejected inputs.
-void check_status(Ctx* c) {
-  int st[2] = {0, 0};
+// The context's device status word, zeroed when first allocated (a read
+// before any reset must see no error bits).
+static int* status_word(Ctx* c) {
+  const bool fresh = !c->status.ptr;
   int* d = static_cast<int*>(c->status.ensure(64));
-  RIMP2_HIP_CHECK(hipMemcpyAsync(st, d, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  if (fresh) RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+  return d;
+}
+
+// The allocation-free entry point keeps its status word in the device's static
+// scratch; both words are read while its checks are pending.
+void check_status(Ctx* c) {
+  int st[2] = {0, 0}, sx[2] = {0, 0};
+  int* d = (c->status_scratch && !c->status.ptr) ? nullptr : status_word(c);
+  int* dx = c->status_scratch;
+  if (d) RIMP2_HIP_CHECK(hipMemcpyAsync(st, d, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  if (c->status_scratch)
+    RIMP2_HIP_CHECK(hipMemcpyAsync(sx, c->status_scratch, sizeof(sx), hipMemcpyDeviceToHost, c->stream));
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
   c->status_unread = false;
+  c->status_scratch = nullptr;
+  if (!st[0] && sx[0]) {
+    st[0] = sx[0];
+    st[1] = sx[1];
+  } else {
+    st[0] |= sx[0];
+  }
   if (st[0]) {
-    RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+    if (d) RIMP2_HIP_CHECK(hipMemsetAsync(d, 0, 64, c->stream));
+    if (dx && sx[0]) RIMP2_HIP_CHECK(hipMemsetAsync(dx, 0, 64, c->stream));
     c->have_metric = false;
     c->have_B = false;
   }
@@ -179,7 +201,7 @@ void denom_check_device(Ctx* c, const double* eo, int64_t nocc, const double* ev
 // reporting any error an earlier asynchronous call left unread).
 void reset_status(Ctx* c) {
   check_pending(c);
-  RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, c->stream));
+  RIMP2_HIP_CHECK(hipMemsetAsync(status_word(c), 0, 64, c->stream));
 }
 
 // The same for an asynchronous call: bits that no call has read yet are kept
@@ -187,7 +209,7 @@ void reset_status(Ctx* c) {
 // queued since the last read, and queuing never waits for the stream.
 void reset_status_async(Ctx* c) {
   if (c->status_unread) return;
-  RIMP2_HIP_CHECK(hipMemsetAsync(c->status.ensure(64), 0, 64, c->stream));
+  RIMP2_HIP_CHECK(hipMemsetAsync(status_word(c), 0, 64, c->stream));
 }
 
 static void validate_mode(int mode) {
@@ -344,6 +366,13 @@ int rimp2_ctx_create(int device, void** ctx) {
     destroy_streams(c);
     delete c;
     fail(RIMP2_ERR_HIP, "stream or event creation failed: %s", hipGetErrorString(e));
+  }
+  try {
+    c->scratch = noalloc_scratch(device);
+  } catch (...) {
+    destroy_streams(c);
+    delete c;
+    throw;
   }
   *ctx = c;
   RIMP2_API_END(nullptr)

@@ -209,24 +209,33 @@ static void dispatch_pair(int mode, int tile, hipStream_t s, const PairParams& p
   fail(RIMP2_ERR_ARG, "no pair kernel for mode %d tile %d", mode, tile);
 }
 
-// Runs the pair kernel + reductions on ctx->stream.  d_pair_list (optional)
-// restricts the work to the listed unique pairs (screening).
-void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const double* scales, int64_t nocc,
-                          int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
-                          double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed) {
-  hipStream_t s = c->stream;
-  const int tile = c->tile;
+// Device bytes pair_energies_core needs for its partials and padded virtual
+// energies (both 256-byte aligned, partials first).
+size_t pair_scratch_bytes(const PackGeom& g, int tile, int64_t npairs, size_t* evp_offset) {
+  const int64_t ntp = pair_units(g.nvp / tile, pair_packs_diagonal(g.mode, tile));
+  const size_t part = (size_t)round_up((int64_t)((size_t)(npairs * ntp + npairs + 1) * sizeof(double)), 256);
+  if (evp_offset) *evp_offset = part;
+  return part + (size_t)round_up(g.nvp * (int64_t)sizeof(double), 256);
+}
+
+// The pair kernel + reductions on stream s with caller-provided scratch (evp:
+// nvp doubles; part: npairs * ntp + npairs + 1 doubles); allocates nothing.
+// Every eps_pair entry is written when all pairs run; with a pair list the
+// matrix is cleared first (skipped pairs stay zero).
+void pair_energies_core(Ctx* timer, hipStream_t s, int tile, const PackGeom& g, const void* packed,
+                        const double* scales, int64_t nocc, int64_t nvirt, const double* d_eps_o,
+                        const double* d_eps_v, double* d_eps_pair, double* d_etot, const int64_t* d_pair_list,
+                        int64_t npairs_listed, double* evp, double* part) {
   const int64_t nb = g.nvp / tile;
   const int64_t ntp = pair_units(nb, pair_packs_diagonal(g.mode, tile));
   const int64_t npairs = d_pair_list ? npairs_listed : nocc * (nocc + 1) / 2;
-  double* evp = static_cast<double*>(c->eps_v.ensure((size_t)g.nvp * sizeof(double)));
   hipLaunchKernelGGL(pad_eps_v_kernel, dim3((unsigned)ceil_div(g.nvp, 256)), dim3(256), 0, s, d_eps_v, nvirt, g.nvp,
                      evp);
   RIMP2_HIP_CHECK(hipGetLastError());
   const size_t npart = (size_t)(npairs * ntp);
-  double* part = static_cast<double*>(c->partials.ensure((npart + (size_t)npairs + 1) * sizeof(double)));
   double* contrib = part + npart;
-  if (d_eps_pair) RIMP2_HIP_CHECK(hipMemsetAsync(d_eps_pair, 0, (size_t)(nocc * nocc) * sizeof(double), s));
+  if (d_eps_pair && d_pair_list)
+    RIMP2_HIP_CHECK(hipMemsetAsync(d_eps_pair, 0, (size_t)(nocc * nocc) * sizeof(double), s));
   if (npairs > 0) {
     PairParams p;
     p.f.packed = static_cast<const unsigned char*>(packed);
@@ -245,16 +254,30 @@ void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const d
     p.nwu = npairs * ntp;
     p.chunk = ceil_div(p.nwu, 8);
     {
-      ScopedTimer t(c, RIMP2_K_PAIR);
+      ScopedTimer t(timer, RIMP2_K_PAIR);
       dispatch_pair(g.mode, tile, s, p);
     }
-    ScopedTimer t(c, RIMP2_K_REDUCE);
+    ScopedTimer t(timer, RIMP2_K_REDUCE);
     hipLaunchKernelGGL(pair_reduce_kernel, dim3((unsigned)ceil_div(npairs, 256)), dim3(256), 0, s, part, ntp, npairs,
                        d_pair_list, nocc, d_eps_pair, contrib);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(total_kernel, dim3(1), dim3(256), 0, s, contrib, npairs, d_etot);
   RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+// Runs the pair kernel + reductions on ctx->stream, with the context's
+// partials and padded-energy buffers.  d_pair_list (optional) restricts the
+// work to the listed unique pairs (screening).
+void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const double* scales, int64_t nocc,
+                          int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
+                          double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed) {
+  const int64_t npairs = d_pair_list ? npairs_listed : nocc * (nocc + 1) / 2;
+  const int64_t ntp = pair_units(g.nvp / c->tile, pair_packs_diagonal(g.mode, c->tile));
+  double* evp = static_cast<double*>(c->eps_v.ensure((size_t)g.nvp * sizeof(double)));
+  double* part = static_cast<double*>(c->partials.ensure((size_t)(npairs * ntp + npairs + 1) * sizeof(double)));
+  pair_energies_core(c, c->stream, c->tile, g, packed, scales, nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair, d_etot,
+                     d_pair_list, npairs_listed, evp, part);
 }
 
 // Amplitude slab of one chunk (pair kernel, 64-wide virtual blocks) on

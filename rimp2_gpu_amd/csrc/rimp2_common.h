@@ -96,6 +96,11 @@ struct Ctx {
   // an asynchronous entry point queued input checks whose status word no
   // call has read yet (rimp2_sync, or the next blocking call, reads it)
   bool status_unread = false;
+  // status word of the allocation-free entry point (in the device's static
+  // scratch) while its checks are unread; null otherwise
+  int* status_scratch = nullptr;
+  // this device's static scratch array (set by rimp2_ctx_create)
+  unsigned char* scratch = nullptr;
   // timing
   bool timing = false;
   Timer timers[RIMP2_NUM_KERNELS];
@@ -207,6 +212,15 @@ bool pair_packs_diagonal(int mode, int tile);
 void pair_energies_device(Ctx* c, const PackGeom& g, const void* packed, const double* scales, int64_t nocc,
                           int64_t nvirt, const double* d_eps_o, const double* d_eps_v, double* d_eps_pair,
                           double* d_etot, const int64_t* d_pair_list, int64_t npairs_listed);
+// The same on stream s with caller-provided scratch (allocates nothing):
+// evp holds nvp doubles, part npairs * units + npairs + 1 doubles;
+// pair_scratch_bytes gives their total (partials first, then evp at
+// *evp_offset).  timer may be null (no event timers).
+size_t pair_scratch_bytes(const PackGeom& g, int tile, int64_t npairs, size_t* evp_offset);
+void pair_energies_core(Ctx* timer, hipStream_t s, int tile, const PackGeom& g, const void* packed,
+                        const double* scales, int64_t nocc, int64_t nvirt, const double* d_eps_o,
+                        const double* d_eps_v, double* d_eps_pair, double* d_etot, const int64_t* d_pair_list,
+                        int64_t npairs_listed, double* evp, double* part);
 void check_inputs_device(Ctx* c, const double* p, int64_t n, const char* what);
 // Device status word of the context (ctx.hip): check_status waits for the
 // stream and turns set bits into an error (clearing them); check_pending does
@@ -229,6 +243,11 @@ void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed
 // 8 = max(eo) >= min(ev) (some pair denominator >= 0).
 void check_finite_on(hipStream_t s, int* status, const double* p, int64_t n, unsigned max_blocks = 8192);
 void check_denom_on(hipStream_t s, int* status, const double* eo, int64_t nocc, const double* ev, int64_t nvirt);
+
+// --- allocation-free entry point (noalloc.hip) ---
+// Device address of this device's static scratch array, looked up once per
+// device (the calling thread must have made `device` current).
+unsigned char* noalloc_scratch(int device);
 
 // --- batch / mlp cleanup ---
 void batch_destroy(BatchState* b);

@@ -191,6 +191,19 @@ module rimp2_hip_interfaces
        type(C_PTR), value :: d_eps_pair, d_e_tot
      end function rimp2_energy_fitted_device
 
+     !> No counterpart in the reference: rimp2_energy_fitted_device with no
+     !> device allocation (scratch in a static array of 128 MiB per GPU), for
+     !> callers that cannot allocate (graph capture, serving loops).
+     integer(C_INT) function rimp2_energy_fitted_device_noalloc(ctx, naux, nvirt, nocc, d_Calpha, d_eps_o, d_eps_v, &
+          & mode, d_eps_pair, d_e_tot) bind(C, name="rimp2_energy_fitted_device_noalloc")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: naux, nvirt, nocc
+       type(C_PTR), value :: d_Calpha, d_eps_o, d_eps_v
+       integer(C_INT), value :: mode
+       type(C_PTR), value :: d_eps_pair, d_e_tot
+     end function rimp2_energy_fitted_device_noalloc
+
      ! ---- intermediate results (host arrays, blocking) ----
 
      integer(C_INT) function rimp2_get_cholesky(ctx, L) bind(C, name="rimp2_get_cholesky")
