@@ -12,9 +12,12 @@
 //   - the status words: a non-positive pivot, a non-finite input, a
 //     denominator >= 0 each flag exactly their fragment.
 // Prints "OK <launches>" and exits 0, or prints the first failure and exits 1.
+// With the argument --small only the fragments up to 129 auxiliary functions
+// take part (every panel and doubling case, a few seconds under a sanitizer).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <unordered_map>
@@ -210,12 +213,13 @@ struct Frag {
 
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
   // (n, nvirt, nocc): below, at and above the 64-wide panel; doubling levels
   // with several full groups and ragged last groups
   const int64_t dims[][3] = {{1, 3, 1},   {5, 17, 2},   {64, 40, 1},  {65, 17, 3},  {100, 130, 2},
                              {128, 40, 1}, {129, 3, 2},  {200, 17, 1}, {257, 40, 2}, {320, 130, 1}};
-  const int nd = sizeof(dims) / sizeof(dims[0]);
+  const bool small = argc > 1 && std::strcmp(argv[1], "--small") == 0;
+  const int nd = small ? 7 : (int)(sizeof(dims) / sizeof(dims[0]));
   std::vector<Frag> F(nd + 1);
   for (int f = 0; f <= nd; f++) {
     Frag& x = F[f];

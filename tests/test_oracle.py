@@ -359,6 +359,66 @@ def test_gelu_and_bf16_closed_forms(oracle):
     assert oracle.bf16(-2.5) == -2.5
 
 
+def _bf16_np(a):
+    """Round-to-nearest-even to bf16 on float32 bit patterns (NaN-free inputs)."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _mlp_forward_np(P, F, T, norm, bf16=False):
+    """Independent numpy restatement of the network, from the documented
+    parameter order W1[16][64] | b1[64] | W2[64][64] (input k -> output m) |
+    b2[64] | w3[64] | b3 and norm = mean[16] | std[16] | tmean | tstd:
+      x  = clip((F - mean)/std, -4, 4)
+      y  = w3 . gelu(W2^T gelu(W1^T x + b1) + b2) + b3
+      pred = F[:,15] + y tstd + tmean,  loss = mean (y - (T - F[:,15] - tmean)/tstd)^2
+    With bf16=True, x, the weights and both hidden activations are rounded to
+    bf16 (the MFMA operands); biases and sums stay FP64."""
+    from scipy.special import erf
+    r = _bf16_np if bf16 else (lambda v: np.asarray(v, dtype=np.float64))
+    P = P.astype(np.float64)
+    W1 = r(P[:1024].reshape(16, 64))
+    b1 = P[1024:1088]
+    W2 = r(P[1088:5184].reshape(64, 64))
+    b2 = P[5184:5248]
+    w3 = r(P[5248:5312])
+    b3 = P[5312]
+    nm = norm.astype(np.float64)
+    x = r(np.clip((F.astype(np.float64) - nm[:16]) / nm[16:32], -4.0, 4.0))
+    gelu = lambda z: 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    h1 = r(gelu(x @ W1 + b1))
+    h2 = r(gelu(h1 @ W2 + b2))
+    y = h2 @ w3 + b3
+    base = F[:, 15].astype(np.float64)
+    pred = base + y * nm[33] + nm[32]
+    loss = np.mean((y - (T.astype(np.float64) - base - nm[32]) / nm[33]) ** 2)
+    return loss, pred
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_mlp_oracle_forward_matches_numpy_restatement(oracle, bf16):
+    P, F, T, norm = mlp_case(5, n=200)
+    F[:7, 3] = 40.0  # standardised input beyond +4: saturates
+    F[7:9, 5] = -40.0
+    loss, _, pred = oracle.mlp_loss_grad(F, T, P, norm, emulate_bf16=bf16, want_grad=False)
+    lref, pref = _mlp_forward_np(P, F, T, norm, bf16=bf16)
+    np.testing.assert_allclose(pred, pref, rtol=1e-12, atol=1e-12)
+    assert abs(loss - lref) <= 1e-12 * lref
+    # a transposed W2 (the layout mistake both implementations could share)
+    # moves the prediction far beyond that tolerance; an input scale that
+    # saturates a feature for every sample is clamped alike in both
+    Pt = P.copy()
+    Pt[1088:5184] = P[1088:5184].reshape(64, 64).T.ravel()
+    _, pt = _mlp_forward_np(Pt, F, T, norm, bf16=bf16)
+    assert np.max(np.abs(pt - pred)) > 1e-3
+    nm2 = norm.copy()
+    nm2[16 + 3] = 1e-3  # input 3 now saturates for every sample
+    _, _, psat = oracle.mlp_loss_grad(F, T, P, nm2, emulate_bf16=bf16, want_grad=False)
+    _, psat_ref = _mlp_forward_np(P, F, T, nm2, bf16=bf16)
+    np.testing.assert_allclose(psat, psat_ref, rtol=1e-12, atol=1e-12)
+
+
 def test_mlp_bf16_emulation_is_close_to_fp64(oracle):
     P, F, T, norm = mlp_case(3, n=64)
     l64, g64, p64 = oracle.mlp_loss_grad(F, T, P, norm)
