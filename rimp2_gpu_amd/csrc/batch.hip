@@ -87,21 +87,49 @@ __global__ __launch_bounds__(PAIR_THREADS) void batch_pair_kernel(BatchParams p)
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
   const int64_t end = p.range[xcd + 1];
-  for (int64_t wu = p.range[xcd] + slot; wu < end; wu += nslot) {
-    // fragment holding wu: last fragment with wu_begin <= wu
-    int64_t lo = 0, hi = p.nfrag - 1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if (p.frags[mid].wu_begin <= wu) lo = mid;
-      else hi = mid - 1;
+  int64_t wu = p.range[xcd] + slot;
+  if (wu >= end) return;
+  // fragment holding the first unit: last fragment with wu_begin <= wu
+  int64_t lo = 0, hi = p.nfrag - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (uni(gload(&p.frags[mid].wu_begin)) <= wu) lo = mid;
+    else hi = mid - 1;
+  }
+  // The units of this workgroup increase, so the fragment index only moves
+  // forward: the fragment's fields are read again only when wu passes the
+  // next fragment's first unit.
+  FragView f;
+  int64_t nb = 0, ntp = 1, begin = 0, next = 0;
+  auto load_frag = [&](int64_t k) {
+    const BatchFrag* q = p.frags + k;
+    f.packed = uni(gload(&q->f.packed));
+    f.sigma = uni(gload(&q->f.sigma));
+    f.eps_o = uni(gload(&q->f.eps_o));
+    f.eps_v = uni(gload(&q->f.eps_v));
+    f.nvp = uni(gload(&q->f.nvp));
+    f.nks = uni(gload(&q->f.nks));
+    f.nrb = uni(gload(&q->f.nrb));
+    nb = uni(gload(&q->nb));
+    ntp = uni(gload(&q->ntp));
+    begin = uni(gload(&q->wu_begin));
+    next = k + 1 < p.nfrag ? uni(gload(&p.frags[k + 1].wu_begin)) : INT64_MAX;
+  };
+  load_frag(lo);
+  for (; wu < end; wu += nslot) {
+    if (wu >= next) {
+      while (lo + 1 < p.nfrag && uni(gload(&p.frags[lo + 1].wu_begin)) <= wu) lo++;
+      load_frag(lo);
     }
-    const BatchFrag& fr = p.frags[lo];
-    const int64_t local = wu - fr.wu_begin;
-    const int64_t pidx = local / fr.ntp, tp = local % fr.ntp;
+    const int64_t local = wu - begin;
+    const int64_t pidx = local / ntp, tp = local % ntp;
     int64_t i, j, A, Bk;
     tri_decode(pidx, i, j);
-    const int kind = unit_decode(tp, fr.nb, packs_diagonal<Pol, TM>(), A, Bk);
-    const double e = pair_tile<Pol, TM>(fr.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+    i = uni(i);
+    j = uni(j);
+    const int kind = uni_int(unit_decode(tp, nb, packs_diagonal<Pol, TM>(), A, Bk));
+    const double e = pair_tile<Pol, TM, false, true>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1,
+                                                     rimp2_buf2, rimp2_buf3);
     if (threadIdx.x == 0) p.partials[wu] = e;
   }
 }

@@ -53,9 +53,10 @@ __global__ __launch_bounds__(PAIR_THREADS) void pair_kernel(PairParams p) {
     if (wu >= p.nwu) break;
     const int64_t pidx = wu / p.ntp, tp = wu % p.ntp;
     int64_t i, j, A, Bk;
-    tri_decode(p.pair_list ? p.pair_list[pidx] : pidx, i, j);
-    const int kind = unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk);
-    const double e = pair_tile<Pol, TM>(p.f, i, j, A, Bk, kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+    tri_decode(p.pair_list ? uni(gload(p.pair_list + pidx)) : pidx, i, j);
+    const int kind = uni_int(unit_decode(tp, p.nb, packs_diagonal<Pol, TM>(), A, Bk));
+    const double e = pair_tile<Pol, TM>(p.f, uni(i), uni(j), uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1,
+                                        rimp2_buf2, rimp2_buf3);
     if (threadIdx.x == 0) p.partials[wu] = e;
   }
 }

@@ -264,6 +264,30 @@ __device__ inline int unit_decode(int64_t u64, int64_t nb64, bool packed, int64_
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+// Load through a global-address-space pointer: a global load counts on the
+// vector-memory counter only and cannot alias LDS, where a load through a
+// generic pointer (a pointer read from memory, as in the batch's fragment
+// table) is a flat load that the compiler must order against every LDS access.
+template <class T>
+__device__ inline T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// Wave-uniform copies (scalar registers) of values the compiler cannot prove
+// uniform (read from memory at a uniform address, or decoded from a uniform
+// unit index in vector arithmetic): a buffer resource built from a value in
+// vector registers costs a readfirstlane loop around every DMA.
+__device__ inline int uni_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <class T>
+__device__ inline T* uni(T* p) {
+  return reinterpret_cast<T*>(uni((int64_t)reinterpret_cast<uintptr_t>(p)));
+}
+
 // Workgroup barrier for the operand pipeline.  __syncthreads() carries a
 // workgroup release/acquire fence, and the release half makes the compiler wait
 // for every LDS DMA still in flight (vmcnt(0)), which would drain the ring at
@@ -325,7 +349,7 @@ constexpr int PAIR_THREADS = 512;
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
 // offset, so each transfer is one buffer load plus the M0 write.
 // AMPL: the amplitude form (AmplOut above) instead of the energy epilogue.
-template <class Pol, int TM, bool AMPL = false>
+template <class Pol, int TM, bool AMPL = false, bool PREFETCH = false>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
                             unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3,
@@ -455,6 +479,52 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     }
   };
 
+  // Epilogue inputs.  Local row l of this wave group's row block and column
+  // block (rows A and cols Bk for a tile pair, with the scales sigma_i x
+  // sigma_j in tile 1 and sigma_j x sigma_i in tile 2; rows and cols the
+  // group's own block for a diagonal unit, A for group 0 and Bk for group 1,
+  // Bk = A for a lone block, with sigma_i x sigma_j).
+  // PREFETCH: lane l loads the virtual energy and the exact-slice row scale of
+  // local row l once, before the operand prologue, so that the load latency
+  // is spent under the main loop, and the epilogue takes the value of row a
+  // from lane a with a wave shuffle (TM <= 64; for TM = 32 the upper
+  // half-wave repeats the lower one).  The persistent batch kernel, whose
+  // short units made these per-element loads a large share of a unit, uses
+  // it; the single-fragment kernel's long units measured faster with the
+  // loads in the epilogue.
+  const int64_t rowblk = (tile_pair || grp == 0) ? A : Bk;
+  const int64_t colblk = tile_pair ? Bk : rowblk;
+  const int64_t orbR = (tile_pair && grp == 1) ? j : i, orbC = (tile_pair && grp == 1) ? i : j;
+  double evR = 0.0, evC = 0.0, sR = 1.0, sC = 1.0, ei = 0.0, ej = 0.0;
+  if constexpr (!AMPL && PREFETCH) {
+    const int l = lane & (TM - 1);
+    evR = gload(f.eps_v + rowblk * TM + l);
+    evC = gload(f.eps_v + colblk * TM + l);
+    if constexpr (is_i8<Pol>::value) {
+      sR = gload(f.sigma + orbR * f.nvp + rowblk * TM + l);
+      sC = gload(f.sigma + orbC * f.nvp + colblk * TM + l);
+    }
+    ei = gload(f.eps_o + i);
+    ej = gload(f.eps_o + j);
+  }
+  // (shuffles are executed by whole waves, inside wave-uniform branches only)
+  auto rowE = [&](int al) -> double {
+    if constexpr (PREFETCH) return __shfl(evR, al, 64);
+    else return gload(f.eps_v + rowblk * TM + al);
+  };
+  auto colE = [&](int bl) -> double {
+    if constexpr (PREFETCH) return __shfl(evC, bl, 64);
+    else return gload(f.eps_v + colblk * TM + bl);
+  };
+  auto rowS = [&](int al) -> double {
+    if constexpr (PREFETCH) return __shfl(sR, al, 64);
+    else return gload(f.sigma + orbR * f.nvp + rowblk * TM + al);
+  };
+  auto colS = [&](int bl) -> double {
+    if constexpr (PREFETCH) return __shfl(sC, bl, 64);
+    else return gload(f.sigma + orbC * f.nvp + colblk * TM + bl);
+  };
+
   Ops ra, rb;
   issue(0, buf0);
   issue(1, buf1);
@@ -512,7 +582,10 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   }
 
   // ---- epilogue: energy terms in FP64 ----
-  const double ei = f.eps_o[i], ej = f.eps_o[j];
+  if constexpr (!PREFETCH) {
+    ei = gload(f.eps_o + i);
+    ej = gload(f.eps_o + j);
+  }
   double s = 0.0, c = 0.0;
   auto buf_of = [&](int k) { return k == 0 ? buf0 : k == 1 ? buf1 : k == 2 ? buf2 : buf3; };
   __syncthreads();  // every wave's stage DMA has landed and its reads are done
@@ -528,10 +601,10 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
         for (int y = 0; y < FW; y++)
 #pragma unroll
           for (int r = 0; r < Pol::NE; r++) {
-            const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
-            const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
+            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             double v2 = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) v2 *= f.sigma[j * f.nvp + a] * f.sigma[i * f.nvp + b];
+            if (is_i8<Pol>::value) v2 *= rowS(al) * colS(bl);  // sigma_j[a] sigma_i[b]
             X[((x * FW + y) * Pol::NE + r) * 64 + lane] = v2;
           }
     }
@@ -543,12 +616,12 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
         for (int y = 0; y < FW; y++)
 #pragma unroll
           for (int r = 0; r < Pol::NE; r++) {
-            const int64_t a = A * TM + wr * WT + x * Pol::RB + Pol::row(lane, r);
-            const int64_t b = Bk * TM + wc * WT + y * Pol::RB + Pol::col(lane, r);
+            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
+            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             double v1 = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) v1 *= f.sigma[i * f.nvp + a] * f.sigma[j * f.nvp + b];
+            if (is_i8<Pol>::value) v1 *= rowS(al) * colS(bl);  // sigma_i[a] sigma_j[b]
             const double v2 = same_occ ? v1 : X[((x * FW + y) * Pol::NE + r) * 64 + lane];
-            const double inv = 1.0 / (ei + ej - f.eps_v[a] - f.eps_v[b]);
+            const double inv = 1.0 / (ei + ej - rowE(al) - colE(bl));
             double e;
             if (same_blk) e = v1 * (2.0 * v1 - v2);
             else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
@@ -561,7 +634,6 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     // from an LDS copy of the tile in stage buffer grp (XOR-swizzled columns:
     // the row writes and the transposed reads are both bank-conflict free)
     const bool active = !(grp == 1 && kind == UNIT_ONE_DIAG);
-    const int64_t X = grp ? Bk : A;
     double* T = reinterpret_cast<double*>(grp ? buf1 : buf0);
     auto swz = [](int a, int b) { return a * TM + (b ^ (a & 31)); };
     double v[FW][FW][Pol::NE];
@@ -575,7 +647,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             double w = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) w *= f.sigma[i * f.nvp + X * TM + al] * f.sigma[j * f.nvp + X * TM + bl];
+            if (is_i8<Pol>::value) w *= rowS(al) * colS(bl);  // sigma_i[a] sigma_j[b]
             v[x][y][r] = w;
             T[swz(al, bl)] = w;
           }
@@ -591,7 +663,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             const double vab = v[x][y][r], vba = T[swz(bl, al)];
-            const double inv = 1.0 / (ei + ej - f.eps_v[X * TM + al] - f.eps_v[X * TM + bl]);
+            const double inv = 1.0 / (ei + ej - rowE(al) - colE(bl));
             kahan_add(s, c, vab * (2.0 * vab - vba) * inv);
           }
     }
