@@ -83,7 +83,8 @@ struct BatchParams {
 };
 
 template <class Pol, int TM>
-__global__ __launch_bounds__(PAIR_THREADS) void batch_pair_kernel(BatchParams p) {
+__global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<Pol, TM>())))
+void batch_pair_kernel(BatchParams p) {
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
   const int64_t end = p.range[xcd + 1];

@@ -39,7 +39,8 @@ struct PairParams {
 };
 
 template <class Pol, int TM>
-__global__ __launch_bounds__(PAIR_THREADS) void pair_kernel(PairParams p) {
+__global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<Pol, TM>())))
+void pair_kernel(PairParams p) {
   RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
   // XCD-aware mapping: consecutive workgroups land on different XCDs
   // (round-robin dispatch), so each XCD owns a contiguous run of work units

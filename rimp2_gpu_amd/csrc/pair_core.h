@@ -39,6 +39,7 @@ struct PolF64 {
   static constexpr int RB = 16, KK = 4, P = 1, LB = 8, ND = 1, NE = 4, KSTEPS = 4, MMA_PER_CALL = 1;
   using frag_t = double;
   using acc_t = v4d;
+  using xpose_t = double;  // element type of the diagonal unit's LDS transpose (holds V exactly)
   __device__ static void zero(acc_t& a) { a = v4d{0, 0, 0, 0}; }
   __device__ static void mma(acc_t (&acc)[ND], const frag_t (&a)[P], const frag_t (&b)[P]) {
     acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], acc[0], 0, 0, 0);
@@ -52,6 +53,7 @@ struct PolF32 {
   static constexpr int RB = 32, KK = 2, P = 1, LB = 4, ND = 1, NE = 16, KSTEPS = 8, MMA_PER_CALL = 1;
   using frag_t = float;
   using acc_t = v16f;
+  using xpose_t = float;
   __device__ static void zero(acc_t& a) {
 #pragma unroll
     for (int r = 0; r < 16; r++) a[r] = 0.f;
@@ -71,6 +73,7 @@ struct PolF32R16 {
   static constexpr int RB = 16, KK = 4, P = 1, LB = 4, ND = 1, NE = 4, KSTEPS = 4, MMA_PER_CALL = 1;
   using frag_t = float;
   using acc_t = v4f;
+  using xpose_t = float;
   __device__ static void zero(acc_t& a) { a = v4f{0.f, 0.f, 0.f, 0.f}; }
   __device__ static void mma(acc_t (&acc)[ND], const frag_t (&a)[P], const frag_t (&b)[P]) {
     acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc[0], 0, 0, 0);
@@ -84,6 +87,7 @@ struct PolBF16x3 {
   static constexpr int RB = 32, KK = 16, P = 2, LB = 16, ND = 1, NE = 16, KSTEPS = 1, MMA_PER_CALL = 3;
   using frag_t = v8bf;
   using acc_t = v16f;
+  using xpose_t = float;
   __device__ static void zero(acc_t& a) { PolF32::zero(a); }
   __device__ static void mma(acc_t (&acc)[ND], const frag_t (&a)[P], const frag_t (&b)[P]) {
     // small terms first: hi*lo + lo*hi, then hi*hi
@@ -105,6 +109,7 @@ struct PolI8 {
   static constexpr int MMA_PER_CALL = (DMAX + 1) * (DMAX + 2) / 2 - (DMAX >= S ? (DMAX - S + 1) * (DMAX - S + 2) : 0);
   using frag_t = v4i;
   using acc_t = v16i;
+  using xpose_t = double;  // scaled slice sums are not float-exact
   __device__ static void zero(acc_t& a) {
 #pragma unroll
     for (int r = 0; r < 16; r++) a[r] = 0;
@@ -178,14 +183,15 @@ constexpr int pair_stage_bytes() {
 }
 
 // A diagonal block's exchange tile V^{ji}[A,A] is the transpose of V^{ij}[A,A],
-// so where the FP64 tile fits one stage buffer (for the epilogue's LDS
-// transpose) two diagonal blocks share one work unit and no MFMA forms the
-// transpose.  Units per occupied pair over nb virtual blocks: the nb(nb-1)/2
+// so where the tile fits one stage buffer (for the epilogue's LDS transpose;
+// FP64 elements, or FP32 for the modes that accumulate in FP32, whose values
+// FP32 holds exactly) two diagonal blocks share one work unit and no MFMA
+// forms the transpose.  Units per occupied pair over nb virtual blocks: the nb(nb-1)/2
 // off-diagonal tile pairs A < B, then ceil(nb/2) units of two diagonal
 // blocks; without packing, the nb(nb+1)/2 tile pairs A <= B.
 template <class Pol, int TM>
 constexpr bool packs_diagonal() {
-  return TM * TM * 8 <= pair_stage_bytes<Pol, TM>();
+  return TM * TM * (int)sizeof(typename Pol::xpose_t) <= pair_stage_bytes<Pol, TM>();
 }
 
 __host__ __device__ inline int64_t pair_units(int64_t nb, bool packed) {
@@ -315,6 +321,16 @@ __device__ inline __amdgpu_buffer_rsrc_t slab_rsrc(const unsigned char* base, si
 
 // Threads per pair workgroup: eight waves, four per tile.
 constexpr int PAIR_THREADS = 512;
+
+// Waves per SIMD the pair kernels are compiled for: as many workgroups as
+// the four stage buffers leave room for in the CU's 160 KB of LDS (two per
+// CU below 80 KB of stages), each with two waves per SIMD.  The register
+// budget per wave follows (512 / waves), so a second workgroup that the LDS
+// admits is not turned away by the register count.
+template <class Pol, int TM>
+constexpr int pair_waves_per_eu() {
+  return 4 * pair_stage_bytes<Pol, TM>() <= 80 * 1024 ? 4 : 2;
+}
 
 // One work unit of occupied pair (i <= j) (blocks A, Bk and the unit kind from
 // unit_decode).  All PAIR_THREADS threads call it; the return value (the
@@ -634,7 +650,8 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     // from an LDS copy of the tile in stage buffer grp (XOR-swizzled columns:
     // the row writes and the transposed reads are both bank-conflict free)
     const bool active = !(grp == 1 && kind == UNIT_ONE_DIAG);
-    double* T = reinterpret_cast<double*>(grp ? buf1 : buf0);
+    using xt = typename Pol::xpose_t;
+    xt* T = reinterpret_cast<xt*>(grp ? buf1 : buf0);
     auto swz = [](int a, int b) { return a * TM + (b ^ (a & 31)); };
     double v[FW][FW][Pol::NE];
     if (active) {
@@ -649,7 +666,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
             double w = Pol::value(acc[x][y], r);
             if (is_i8<Pol>::value) w *= rowS(al) * colS(bl);  // sigma_i[a] sigma_j[b]
             v[x][y][r] = w;
-            T[swz(al, bl)] = w;
+            T[swz(al, bl)] = (xt)w;
           }
     }
     __syncthreads();
@@ -662,7 +679,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            const double vab = v[x][y][r], vba = T[swz(bl, al)];
+            const double vab = v[x][y][r], vba = (double)T[swz(bl, al)];
             const double inv = 1.0 / (ei + ej - rowE(al) - colE(bl));
             kahan_add(s, c, vab * (2.0 * vab - vba) * inv);
           }
