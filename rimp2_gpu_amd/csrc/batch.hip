@@ -129,8 +129,8 @@ void batch_pair_kernel(BatchParams p) {
     i = uni(i);
     j = uni(j);
     const int kind = uni_int(unit_decode(tp, nb, packs_diagonal<Pol, TM>(), A, Bk));
-    const double e = pair_tile<Pol, TM, false, true>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1,
-                                                     rimp2_buf2, rimp2_buf3);
+    const double e = pair_tile<Pol, TM>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2,
+                                        rimp2_buf3);
     if (threadIdx.x == 0) p.partials[wu] = e;
   }
 }

@@ -365,7 +365,7 @@ constexpr int pair_waves_per_eu() {
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
 // offset, so each transfer is one buffer load plus the M0 write.
 // AMPL: the amplitude form (AmplOut above) instead of the energy epilogue.
-template <class Pol, int TM, bool AMPL = false, bool PREFETCH = false>
+template <class Pol, int TM, bool AMPL = false>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
                             unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3,
@@ -495,51 +495,18 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     }
   };
 
-  // Epilogue inputs.  Local row l of this wave group's row block and column
-  // block (rows A and cols Bk for a tile pair, with the scales sigma_i x
-  // sigma_j in tile 1 and sigma_j x sigma_i in tile 2; rows and cols the
-  // group's own block for a diagonal unit, A for group 0 and Bk for group 1,
-  // Bk = A for a lone block, with sigma_i x sigma_j).
-  // PREFETCH: lane l loads the virtual energy and the exact-slice row scale of
-  // local row l once, before the operand prologue, so that the load latency
-  // is spent under the main loop, and the epilogue takes the value of row a
-  // from lane a with a wave shuffle (TM <= 64; for TM = 32 the upper
-  // half-wave repeats the lower one).  The persistent batch kernel, whose
-  // short units made these per-element loads a large share of a unit, uses
-  // it; the single-fragment kernel's long units measured faster with the
-  // loads in the epilogue.
+  // Epilogue inputs: the virtual energies and exact-slice row scales of the
+  // rows and columns this lane's output elements sit on (rows A and cols Bk
+  // for a tile pair, with the scales sigma_i x sigma_j in tile 1 and sigma_j x
+  // sigma_i in tile 2; rows and cols the group's own block for a diagonal
+  // unit, A for group 0 and Bk for group 1, Bk = A for a lone block, with
+  // sigma_i x sigma_j).  They are loaded after the main loop, all at once
+  // behind a scheduling barrier, so their latency is paid once per unit and
+  // not once per element.
   const int64_t rowblk = (tile_pair || grp == 0) ? A : Bk;
   const int64_t colblk = tile_pair ? Bk : rowblk;
   const int64_t orbR = (tile_pair && grp == 1) ? j : i, orbC = (tile_pair && grp == 1) ? i : j;
-  double evR = 0.0, evC = 0.0, sR = 1.0, sC = 1.0, ei = 0.0, ej = 0.0;
-  if constexpr (!AMPL && PREFETCH) {
-    const int l = lane & (TM - 1);
-    evR = gload(f.eps_v + rowblk * TM + l);
-    evC = gload(f.eps_v + colblk * TM + l);
-    if constexpr (is_i8<Pol>::value) {
-      sR = gload(f.sigma + orbR * f.nvp + rowblk * TM + l);
-      sC = gload(f.sigma + orbC * f.nvp + colblk * TM + l);
-    }
-    ei = gload(f.eps_o + i);
-    ej = gload(f.eps_o + j);
-  }
-  // (shuffles are executed by whole waves, inside wave-uniform branches only)
-  auto rowE = [&](int al) -> double {
-    if constexpr (PREFETCH) return __shfl(evR, al, 64);
-    else return gload(f.eps_v + rowblk * TM + al);
-  };
-  auto colE = [&](int bl) -> double {
-    if constexpr (PREFETCH) return __shfl(evC, bl, 64);
-    else return gload(f.eps_v + colblk * TM + bl);
-  };
-  auto rowS = [&](int al) -> double {
-    if constexpr (PREFETCH) return __shfl(sR, al, 64);
-    else return gload(f.sigma + orbR * f.nvp + rowblk * TM + al);
-  };
-  auto colS = [&](int bl) -> double {
-    if constexpr (PREFETCH) return __shfl(sC, bl, 64);
-    else return gload(f.sigma + orbC * f.nvp + colblk * TM + bl);
-  };
+  double eRow[FW][Pol::NE], sRow[FW][Pol::NE], eCol[FW], sCol[FW], ei = 0.0, ej = 0.0;
 
   Ops ra, rb;
   issue(0, buf0);
@@ -555,6 +522,39 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
     step(st + 1, rb, ra, buf1, buf2);
     step(st + 2, ra, rb, buf2, buf3);
     step(st + 3, rb, ra, buf3, buf0);
+  }
+  // the accumulators as FP64 values first: the loads below then reuse their
+  // registers
+  double val[FW][FW][Pol::NE];
+  if constexpr (!AMPL) {
+#pragma unroll
+    for (int x = 0; x < FW; x++)
+#pragma unroll
+      for (int y = 0; y < FW; y++)
+#pragma unroll
+        for (int r = 0; r < Pol::NE; r++) val[x][y][r] = Pol::value(acc[x][y], r);
+    __builtin_amdgcn_sched_barrier(0);
+    const double* evR = f.eps_v + rowblk * TM + wr * WT;
+    const double* evC = f.eps_v + colblk * TM + wc * WT;
+#pragma unroll
+    for (int x = 0; x < FW; x++) {
+#pragma unroll
+      for (int r = 0; r < Pol::NE; r++) eRow[x][r] = gload(evR + x * Pol::RB + Pol::row(lane, r));
+      eCol[x] = gload(evC + x * Pol::RB + Pol::col(lane, 0));
+    }
+    if constexpr (is_i8<Pol>::value) {
+      const double* sgR = f.sigma + orbR * f.nvp + rowblk * TM + wr * WT;
+      const double* sgC = f.sigma + orbC * f.nvp + colblk * TM + wc * WT;
+#pragma unroll
+      for (int x = 0; x < FW; x++) {
+#pragma unroll
+        for (int r = 0; r < Pol::NE; r++) sRow[x][r] = gload(sgR + x * Pol::RB + Pol::row(lane, r));
+        sCol[x] = gload(sgC + x * Pol::RB + Pol::col(lane, 0));
+      }
+    }
+    ei = gload(f.eps_o + i);
+    ej = gload(f.eps_o + j);
+    __builtin_amdgcn_sched_barrier(0);
   }
   RIMP2_WAIT_VMCNT_LGKM0(0);  // trailing re-fetches and the unused last read are done before LDS is reused
 
@@ -598,10 +598,6 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   }
 
   // ---- epilogue: energy terms in FP64 ----
-  if constexpr (!PREFETCH) {
-    ei = gload(f.eps_o + i);
-    ej = gload(f.eps_o + j);
-  }
   double s = 0.0, c = 0.0;
   auto buf_of = [&](int k) { return k == 0 ? buf0 : k == 1 ? buf1 : k == 2 ? buf2 : buf3; };
   __syncthreads();  // every wave's stage DMA has landed and its reads are done
@@ -619,8 +615,8 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            double v2 = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) v2 *= rowS(al) * colS(bl);  // sigma_j[a] sigma_i[b]
+            double v2 = val[x][y][r];
+            if (is_i8<Pol>::value) v2 *= sRow[x][r] * sCol[y];  // sigma_j[a] sigma_i[b]
             X[((x * FW + y) * Pol::NE + r) * 64 + lane] = v2;
           }
     }
@@ -634,10 +630,10 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            double v1 = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) v1 *= rowS(al) * colS(bl);  // sigma_i[a] sigma_j[b]
+            double v1 = val[x][y][r];
+            if (is_i8<Pol>::value) v1 *= sRow[x][r] * sCol[y];  // sigma_i[a] sigma_j[b]
             const double v2 = same_occ ? v1 : X[((x * FW + y) * Pol::NE + r) * 64 + lane];
-            const double inv = 1.0 / (ei + ej - rowE(al) - colE(bl));
+            const double inv = 1.0 / (ei + ej - eRow[x][r] - eCol[y]);
             double e;
             if (same_blk) e = v1 * (2.0 * v1 - v2);
             else e = 2.0 * (v1 * v1 + v2 * v2 - v1 * v2);
@@ -663,8 +659,8 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           for (int r = 0; r < Pol::NE; r++) {
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
-            double w = Pol::value(acc[x][y], r);
-            if (is_i8<Pol>::value) w *= rowS(al) * colS(bl);  // sigma_i[a] sigma_j[b]
+            double w = val[x][y][r];
+            if (is_i8<Pol>::value) w *= sRow[x][r] * sCol[y];  // sigma_i[a] sigma_j[b]
             v[x][y][r] = w;
             T[swz(al, bl)] = (xt)w;
           }
@@ -680,7 +676,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
             const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
             const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             const double vab = v[x][y][r], vba = (double)T[swz(bl, al)];
-            const double inv = 1.0 / (ei + ej - rowE(al) - colE(bl));
+            const double inv = 1.0 / (ei + ej - eRow[x][r] - eCol[y]);
             kahan_add(s, c, vab * (2.0 * vab - vba) * inv);
           }
     }
