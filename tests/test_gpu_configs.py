@@ -159,6 +159,36 @@ def test_c4_surrogate_trained_on_512_fragments_screens_with_exact_energies(ctx, 
         assert nskip == 0 and Es == E
 
 
+def test_c4_shape_locality_variant_network_screen_beats_the_network_free_screen(ctx, rimp2):
+    """The same pipeline on the locality-decay variant of the generator (orbitals
+    on a chain, lambda = 1.5), where small pairs exist: trained on 128 seeded
+    64/512/2048 fragments, applied to a 128/1024/3072 fragment.  At 1e-6 Eh the
+    network skips about the pairs an oracle screen (one that knows every exact
+    pair energy) skips, predicts log(-eps_ij) better than the base estimate
+    (feature 15) and loses less energy than the network-free screen on that
+    estimate; the skipped pairs are what the screened energy leaves out, and
+    the screened pair stage is faster than the exact one."""
+    from rimp2_gpu_amd import surrogate
+    lam, thresh = 1.5, 1e-6
+    say("c4 locality: training set from 128 seeded 64/512/2048 fragments, lambda", lam)
+    F, T = surrogate.training_set(ctx, 128, seed0=100000, lam=lam, mode="i8s5", log=say)
+    hist = surrogate.train(ctx, F, T, epochs=4, batch=4096, lr=2e-3, log=say)
+    assert hist[-1] < hist[0]
+    rep = surrogate.screen_report(ctx, 128, 1024, 3072, 4, lam, thresh, mode="i8s5")
+    say("c4 locality", {k: rep[k] for k in ("npairs", "nskipped", "mlp_skipped", "bound_skipped", "ideal_skipped",
+                                            "dE_screened", "dE_bound_screen", "dE_ideal_screen", "pred_log_rmse",
+                                            "base_log_rmse", "t_exact_s", "t_screened_s")})
+    assert rep["E_exact_again"] == rep["E_exact"]
+    assert rep["ideal_skipped"] > rep["npairs"] // 2
+    assert rep["nskipped"] == rep["mlp_skipped"]
+    assert rep["mlp_skipped"] >= 0.9 * rep["ideal_skipped"]
+    assert rep["pred_log_rmse"] < rep["base_log_rmse"]
+    # E_screened - E_exact = the (positive) energy of the skipped pairs
+    assert rep["dE_screened"] >= 0.0
+    assert rep["dE_screened"] < rep["dE_bound_screen"]
+    assert rep["t_screened_s"] < 0.5 * rep["t_exact_s"]
+
+
 # ---------------------------------------------------------------------------
 # C5: 256/2048/6144, B beyond 2^31 elements
 # ---------------------------------------------------------------------------
