@@ -147,6 +147,31 @@ __device__ inline void tri_decode(int64_t p, int64_t& i, int64_t& j) {
   i = p - jj * (jj + 1) / 2;
 }
 
+// Sum of v over the 64 lanes of a wave, in a fixed order, without LDS round
+// trips: two quad permutes and two row rotates (DPP) leave every lane of a
+// 16-lane row holding the row sum, and the four row sums are read from lanes
+// 0, 16, 32 and 48.  The result is wave-uniform.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ inline double lane_value(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ inline double wave_sum(double v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 __device__ inline void kahan_add(double& s, double& c, double x) {
   double y = x - c;
   double t = s + y;
@@ -613,8 +638,6 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
         for (int y = 0; y < FW; y++)
 #pragma unroll
           for (int r = 0; r < Pol::NE; r++) {
-            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
-            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             double v2 = val[x][y][r];
             if (is_i8<Pol>::value) v2 *= sRow[x][r] * sCol[y];  // sigma_j[a] sigma_i[b]
             X[((x * FW + y) * Pol::NE + r) * 64 + lane] = v2;
@@ -628,8 +651,6 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
         for (int y = 0; y < FW; y++)
 #pragma unroll
           for (int r = 0; r < Pol::NE; r++) {
-            const int al = wr * WT + x * Pol::RB + Pol::row(lane, r);
-            const int bl = wc * WT + y * Pol::RB + Pol::col(lane, r);
             double v1 = val[x][y][r];
             if (is_i8<Pol>::value) v1 *= sRow[x][r] * sCol[y];  // sigma_i[a] sigma_j[b]
             const double v2 = same_occ ? v1 : X[((x * FW + y) * Pol::NE + r) * 64 + lane];
@@ -681,12 +702,13 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
           }
     }
   }
-  double tot = s - c;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-  // the wave sums go through buf3 (unused by both epilogues' second phase)
+  const double tot = wave_sum(s - c);
+  // the wave sums go through the first bytes of buf0: after this barrier no
+  // wave reads the hand-over or transpose data any more (every stage buffer
+  // holds at least 8 doubles; the final barrier protects them from the next
+  // unit's DMA)
   __syncthreads();
-  double* wsum = reinterpret_cast<double*>(buf3) + 1024;
+  double* wsum = reinterpret_cast<double*>(buf0);
   if (lane == 0) wsum[wave] = tot;
   __syncthreads();
   double res = 0.0;
