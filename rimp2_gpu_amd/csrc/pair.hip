@@ -15,7 +15,7 @@
 // which is RIMP2_calc_toccA (RIMP2.F90:399-404: gmocont, deltaEPS, t = g/Delta)
 // combined with RIMP2_calc_gocc (RIMP2.F90:577-581) at U = identity.  For
 // i == j the matrix V^{ii} is symmetric and tile 2 equals tile 1 (skipped).
-// Per-lane Kahan sums are reduced by wave shuffles and LDS to one FP64 partial
+// Per-lane Kahan sums are reduced by DPP lane moves and LDS to one FP64 partial
 // per workgroup; the partials are summed per pair and then over pairs in a
 // fixed order with Neumaier compensation (deterministic).
 #include <algorithm>
