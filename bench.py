@@ -465,6 +465,37 @@ def c2_extra(r, steps, warmup):
     return out
 
 
+def sustained_i8(iters=4_000_000):
+    """The int8 MFMA rate this card sustains on random operands, from the probe
+    in rimp2_gpu_amd/measure/mfma_ceiling.hip (one launch of ~2.5 s each): the
+    pair kernel's 15 slice products per k-step with operands in registers
+    only, and with the next step's operands read from LDS at the pair kernel's
+    ratio (10 KB per wave per 15 MFMAs).  The clock is the one the matrix
+    pipes would need to give the measured rate at 100% busy, a lower bound on
+    the engine clock of the run.  None where the probe is not built."""
+    import ctypes
+    path = os.path.join(ROOT, "rimp2_gpu_amd", "librimp2_measure.so")
+    if not os.path.exists(path):
+        log("[bench] librimp2_measure.so is not built: no sustained-rate probe")
+        return None
+    f = ctypes.CDLL(path).rimp2_measure_i8_mfma
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 3
+    out = {}
+    for name, lds in (("register_operands", 0), ("with_lds_reads", 1)):
+        s, o, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if f(0, iters, lds, ctypes.byref(s), ctypes.byref(o), ctypes.byref(m)) != 0:
+            log("[bench] sustained-rate probe failed")
+            return None
+        out[name] = dict(tops=o.value / s.value / 1e12, seconds=s.value,
+                         clock_ghz_at_full_busy=m.value * 32 / s.value / 1e9)
+        log(f"[bench] sustained int8 MFMA rate, {name}: {out[name]}")
+    out["unit"] = "TOP/s"
+    out["probe"] = ("rimp2_gpu_amd/measure/mfma_ceiling.hip: one 8-wave workgroup per CU, v_mfma_i32_32x32x32_i8 "
+                    "on random operand bytes, 15 products per step into five int32 accumulators")
+    return out
+
+
 def profile_figures(mode, nocc, nvirt, naux):
     """HBM bytes per pair-kernel launch and the launch duration in the
     committed rocprofv3 summaries of this workload (counter runs scaled by a
@@ -587,6 +618,7 @@ def main():
         ref_E = E_mode if mode == a.mode else (res64 or {}).get("E")
         tte[mode]["energy_equals_device_path"] = all(E == ref_E for E in Es)
 
+    sustained = sustained_i8() if NPROD[a.mode] > 3 else None
     pair_ms = mean_ms(k_mode["pair"])
     pair_s = pair_ms * 1e-3
     # Roofline of the pair kernel: the rate of the MFMA type it executes
@@ -608,6 +640,11 @@ def main():
                 profile=prof or None,
                 live_vs_profile_duration=(pair_ms / prof["kernel_trace_mean_ms"]
                                           if prof.get("kernel_trace_mean_ms") else None))
+    if sustained:
+        # the same achieved rate against what this card holds on random int8 operands
+        roof["sustained_int8_mfma"] = sustained
+        roof["frac_of_sustained"] = {k: hw_rate / sustained[k]["tops"]
+                                     for k in ("register_operands", "with_lds_reads")}
 
     # ---- optional legs, within the wall-clock budget ----
     def left():

@@ -52,6 +52,18 @@ def build_library(jobs=8):
     return LIB
 
 
+MEASURE_LIB = os.path.join(HERE, "librimp2_measure.so")
+
+
+def build_measure():
+    """Builds librimp2_measure.so: the benchmark's probe of the int8 MFMA rate
+    the card sustains (measure/mfma_ceiling.hip); not part of the energy path."""
+    src = os.path.join(HERE, "measure", "mfma_ceiling.hip")
+    _run([HIPCC] + HIP_FLAGS + ["-x", "hip", "-shared", src, "-o", MEASURE_LIB + ".tmp"])
+    os.replace(MEASURE_LIB + ".tmp", MEASURE_LIB)
+    return MEASURE_LIB
+
+
 def build_fortran():
     """Builds the Fortran host driver (rimp2_driver.x) against the library."""
     fdir = os.path.join(HERE, "fortran")
@@ -73,6 +85,7 @@ def build_fortran():
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     print("built", build_library())
+    print("built", build_measure())
     if "--no-fortran" not in argv:
         exe = build_fortran()
         if exe:

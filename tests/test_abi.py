@@ -126,3 +126,12 @@ def test_reference_interface_module_links_get_dev_mem(tmp_path):
     out = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120).stdout
     w = out.split()
     assert w[0] == "mem" and w[3] == "T", out
+
+
+def test_measure_probe_exports_its_entry_point():
+    """The benchmark's MFMA-rate probe is a library of its own, outside the
+    energy path's C ABI."""
+    L = ctypes.CDLL(os.path.join(ROOT, "rimp2_gpu_amd", "librimp2_measure.so"))
+    assert hasattr(L, "rimp2_measure_i8_mfma")
+    header = open(os.path.join(ROOT, "include", "rimp2_hip.h")).read()
+    assert "rimp2_measure_i8_mfma" not in header
