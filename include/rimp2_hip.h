@@ -176,12 +176,24 @@ int rimp2_batch_prepare(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, 
 int rimp2_batch_prepare_device(void* ctx, int64_t nfrag, const rimp2_frag_desc* frags, int mode);
 int rimp2_batch_run(void* ctx, double* d_e_frag);
 int rimp2_batch_result(void* ctx, double* e_frag, double* eps_pair_packed);
-/* Tile size chosen by the autotuner for the prepared batch (32 or 64). */
+/* Tile candidate of the prepared batch: 32 or 64 (every fragment with that
+ * pair tile, i.e. virtual block width) or -1 (mixed: per fragment the 32-wide
+ * tile where the 64-wide one's padding executes at least 1.5x the MFMA work,
+ * i.e. ceil64(nvirt)^2 >= 1.5 ceil32(nvirt)^2, else the 64-wide; one launch
+ * carries both).  fp64 and fp32 have all three candidates, bf16x3 and
+ * the exact-slice modes 64 only. */
 int rimp2_batch_tile(void* ctx, int64_t* tile);
-/* The autotuner's measurements for the prepared batch: *n (<= 4) candidate
- * tiles and the time of one pair-stage launch with each (ms).  *n = 0 when
- * the choice came from the per-(mode, size class) cache. */
+/* The autotuner's measurements for the prepared batch: *n (<= 4) candidates
+ * (encoded as rimp2_batch_tile) and the time of one pair-stage launch with
+ * each (ms).  *n = 0 when the choice came from the per-(mode, size class)
+ * cache or from rimp2_batch_set_tile. */
 int rimp2_batch_tuning(void* ctx, int* n, int64_t* tiles, double* ms);
+/* Fixes the tile candidate of the following rimp2_batch_prepare(_device)
+ * calls on ctx (32, 64 or -1 as rimp2_batch_tile; 0 returns to the
+ * autotuner).  A candidate the mode does not have makes the prepare fail with
+ * RIMP2_ERR_ARG.  No counterpart in the reference (its fragment loop,
+ * RIMP2.F90:36, has no tiles); for callers that need a reproducible layout. */
+int rimp2_batch_set_tile(void* ctx, int64_t tile);
 
 /* ---- pair-screening surrogate (BF16 MLP 16 -> 64 -> 64 -> 1, GELU) ---- */
 #define RIMP2_NFEAT 16

@@ -71,6 +71,7 @@ SIGNATURES = {
     "rimp2_batch_result": (ctypes.c_int, [_vp, _dp, _dp]),
     "rimp2_batch_tile": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
     "rimp2_batch_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_i64), _dp]),
+    "rimp2_batch_set_tile": (ctypes.c_int, [_vp, _i64]),
     "rimp2_pair_features": (ctypes.c_int, [_vp, _i64, _i64, _dp, _dp, _fp]),
     "rimp2_mlp_init": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "rimp2_mlp_set_norm": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_float, ctypes.c_float]),
@@ -402,14 +403,21 @@ class Context:
         return ef, pp
 
     def batch_tuning(self):
-        """The autotuner's last measurements: [(tile, ms), ...] ([] when cached)."""
+        """The autotuner's last measurements: [(candidate, ms), ...] ([] when
+        cached or fixed); candidates as batch_tile."""
         n = ctypes.c_int()
         tiles = (_i64 * 4)()
         ms = (ctypes.c_double * 4)()
         self._chk(lib().rimp2_batch_tuning(self.h, ctypes.byref(n), tiles, ms))
         return [(int(tiles[k]), float(ms[k])) for k in range(n.value)]
 
+    def batch_set_tile(self, tile):
+        """Fixes the batch tile candidate (32, 64, -1 = mixed per fragment);
+        0 returns to the autotuner."""
+        self._chk(lib().rimp2_batch_set_tile(self.h, int(tile)))
+
     def batch_tile(self):
+        """Tile candidate of the prepared batch: 32, 64 or -1 (mixed)."""
         t = _i64()
         self._chk(lib().rimp2_batch_tile(self.h, ctypes.byref(t)))
         return t.value

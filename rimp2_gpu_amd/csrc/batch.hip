@@ -18,30 +18,29 @@
 // then sums each fragment's partials per pair (Neumaier) and the pairs per
 // fragment (fixed tree order).
 //
-// Autotuner: the pair tile (virtual block width) is chosen per (mode, size
-// class of the batch) by timing the persistent launch for each candidate tile
-// on the batch itself; the winner is cached in the context.
+// Tile choice: every fragment has its own pair tile (virtual block width).
+// For the modes with two tiles (fp64, fp32: 32 and 64 wide) a candidate is
+// all fragments at 32, all at 64, or per fragment the 32-wide tile where the
+// 64-wide one pads the most (mixed, encoded -1; rule in frag_tile);
+// the mixed launch carries both tile paths and branches per unit on the
+// fragment's tile (uniform within a workgroup's unit).  The autotuner times
+// the persistent launch for each candidate on the batch itself and caches
+// the winner per (mode, size class of the batch); rimp2_batch_set_tile fixes
+// a candidate instead.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
 
-#include "pair_core.h"
+#include "batch_kernel.h"
 
 namespace rimp2 {
 
-struct BatchFrag {
-  pairk::FragView f;
-  int64_t nocc, nb, ntp, npairs;
-  int64_t wu_begin;  // first global work unit (launch order)
-  int64_t pair_off;  // offset of this fragment's [nocc, nocc] pair matrix
-  int64_t orig;      // index of the fragment in the caller's list
-};
-
 struct BatchState {
   int mode = -1;
-  int tile = 0;
+  int tile = 0;         // tile candidate of the layout: 32, 64, or -1 (mixed per fragment)
+  int forced_tile = 0;  // rimp2_batch_set_tile: 0 = autotune
   int64_t nfrag = 0, total_wu = 0, total_pairs_sq = 0;
   std::vector<int64_t> nocc, nvirt, naux;
   std::vector<size_t> b64_off;  // FP64 B of each fragment in the b64 arena (doubles)
@@ -55,15 +54,15 @@ struct BatchState {
   int tune_tiles[4] = {0, 0, 0, 0};
   int ntune = 0;
   MetricGroup group;  // grouped metric pipeline of the prepare
-  DevBuf pack_jobs;   // job tables of the grouped packing
-  std::vector<unsigned char> pack_blob;
+  DevBuf pack_jobs[2];  // job tables of the grouped packing (one per tile class)
+  std::vector<unsigned char> pack_blob[2];
   DevBuf stage;       // host inputs staged on the device (V, then X, of the fragments with a metric)
   DevBuf fstatus;     // two status ints per fragment (bits as the context's status word)
   ~BatchState() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (DevBuf* b : {&b64, &packed, &scales, &eps_o, &eps_v, &partials, &pairs, &e_frag, &frags, &stage, &fstatus,
-                      &pack_jobs})
+                      &pack_jobs[0], &pack_jobs[1]})
       b->release();
     group.release();
   }
@@ -74,66 +73,6 @@ void batch_destroy(BatchState* b) { delete b; }
 namespace {
 
 using namespace pairk;
-
-struct BatchParams {
-  const BatchFrag* frags;
-  int64_t nfrag, total;
-  int64_t range[9];  // XCD x owns work units [range[x], range[x+1])
-  double* partials;
-};
-
-template <class Pol, int TM>
-__global__ __launch_bounds__(PAIR_THREADS) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<Pol, TM>())))
-void batch_pair_kernel(BatchParams p) {
-  RIMP2_PAIR_STAGE_BUFFERS(Pol, TM);
-  const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
-  const int64_t end = p.range[xcd + 1];
-  int64_t wu = p.range[xcd] + slot;
-  if (wu >= end) return;
-  // fragment holding the first unit: last fragment with wu_begin <= wu
-  int64_t lo = 0, hi = p.nfrag - 1;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi + 1) >> 1;
-    if (uni(gload(&p.frags[mid].wu_begin)) <= wu) lo = mid;
-    else hi = mid - 1;
-  }
-  // The units of this workgroup increase, so the fragment index only moves
-  // forward: the fragment's fields are read again only when wu passes the
-  // next fragment's first unit.
-  FragView f;
-  int64_t nb = 0, ntp = 1, begin = 0, next = 0;
-  auto load_frag = [&](int64_t k) {
-    const BatchFrag* q = p.frags + k;
-    f.packed = uni(gload(&q->f.packed));
-    f.sigma = uni(gload(&q->f.sigma));
-    f.eps_o = uni(gload(&q->f.eps_o));
-    f.eps_v = uni(gload(&q->f.eps_v));
-    f.nvp = uni(gload(&q->f.nvp));
-    f.nks = uni(gload(&q->f.nks));
-    f.nrb = uni(gload(&q->f.nrb));
-    nb = uni(gload(&q->nb));
-    ntp = uni(gload(&q->ntp));
-    begin = uni(gload(&q->wu_begin));
-    next = k + 1 < p.nfrag ? uni(gload(&p.frags[k + 1].wu_begin)) : INT64_MAX;
-  };
-  load_frag(lo);
-  for (; wu < end; wu += nslot) {
-    if (wu >= next) {
-      while (lo + 1 < p.nfrag && uni(gload(&p.frags[lo + 1].wu_begin)) <= wu) lo++;
-      load_frag(lo);
-    }
-    const int64_t local = wu - begin;
-    const int64_t pidx = local / ntp, tp = local % ntp;
-    int64_t i, j, A, Bk;
-    tri_decode(pidx, i, j);
-    i = uni(i);
-    j = uni(j);
-    const int kind = uni_int(unit_decode(tp, nb, packs_diagonal<Pol, TM>(), A, Bk));
-    const double e = pair_tile<Pol, TM>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2,
-                                        rimp2_buf3);
-    if (threadIdx.x == 0) p.partials[wu] = e;
-  }
-}
 
 // One workgroup per fragment: pair sums (Neumaier over its tile partials, in
 // order), the fragment's [nocc,nocc] pair matrix and its total energy.
@@ -175,50 +114,60 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const BatchFrag* frag
   if (threadIdx.x == 0) e_frag[fr.orig] = ss[0] + cc[0];
 }
 
-template <class Pol, int TM>
-int occupancy_of() {
-  int blocks = 0;
-  RIMP2_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_pair_kernel<Pol, TM>, PAIR_THREADS, 0));
-  return blocks < 1 ? 1 : blocks;
-}
-
-template <class Pol, int TM>
-void launch_batch_t(hipStream_t s, const BatchParams& p, int ncu) {
-  const int grid = 8 * ((ncu * occupancy_of<Pol, TM>() + 7) / 8);  // whole XCD slots
-  hipLaunchKernelGGL((batch_pair_kernel<Pol, TM>), dim3(grid), dim3(PAIR_THREADS), 0, s, p);
-  RIMP2_HIP_CHECK(hipGetLastError());
-}
-
+// tile: the layout's candidate (32, 64, or -1 = mixed per fragment)
 void launch_batch(int mode, int tile, hipStream_t s, const BatchParams& p, int ncu) {
+  if (tile == -1) return launch_batch_mixed(mode, s, p, ncu);
   switch (mode) {
     case RIMP2_MODE_FP64:
-      if (tile == 32) return launch_batch_t<PolF64, 32>(s, p, ncu);
-      if (tile == 64) return launch_batch_t<PolF64, 64>(s, p, ncu);
+      if (tile == 32) return launch_batch_t<PolF64, 32, PolF64, 32>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolF64, 64, PolF64, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_FP32:
-      if (tile == 32) return launch_batch_t<PolF32R16, 32>(s, p, ncu);
-      if (tile == 64) return launch_batch_t<PolF32, 64>(s, p, ncu);
+      if (tile == 32) return launch_batch_t<PolF32R16, 32, PolF32R16, 32>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolF32, 64, PolF32, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_BF16X3:
-      if (tile == 64) return launch_batch_t<PolBF16x3, 64>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolBF16x3, 64, PolBF16x3, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_I8S4:
-      if (tile == 64) return launch_batch_t<PolI8<4, 4>, 64>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolI8<4, 4>, 64, PolI8<4, 4>, 64>(s, p, ncu);
       break;
     case RIMP2_MODE_I8S5:
-      if (tile == 64) return launch_batch_t<PolI8<5, 4>, 64>(s, p, ncu);
+      if (tile == 64) return launch_batch_t<PolI8<5, 4>, 64, PolI8<5, 4>, 64>(s, p, ncu);
       break;
   }
   fail(RIMP2_ERR_ARG, "no batch pair kernel for mode %d tile %d", mode, tile);
 }
 
+// Tile candidates of a mode.  bf16x3 and the exact-slice modes have the
+// 64-wide tile only: a 128-wide tile's wave groups would spill at two waves
+// per SIMD, and a 32-wide int8 tile needs the 16x16x64 shape, which pads the
+// auxiliary dimension to 256.
 std::vector<int> tile_candidates(int mode) {
   switch (mode) {
     case RIMP2_MODE_FP64:
-    case RIMP2_MODE_FP32: return {32, 64};
-    case RIMP2_MODE_BF16X3: return {64};
+    case RIMP2_MODE_FP32: return {32, 64, -1};
     default: return {64};
   }
+}
+
+bool valid_candidate(int mode, int tile) {
+  for (int t : tile_candidates(mode))
+    if (t == tile) return true;
+  return false;
+}
+
+// Pair tile of a fragment of nvirt virtuals under candidate `tile`.  The
+// mixed candidate takes the 32-wide tile where the 64-wide tile's padding
+// executes at least 1.5x its MFMA work (nvirt <= 32 and 65..96 only): per
+// useful flop the 32-wide path runs ~1.2x slower (a unit
+// does a quarter of the work against the same fixed cost), so a smaller
+// padding saving loses (measured at 256 DEC fragments: every fragment with
+// less padding on the 32-wide tile was 18% slower than all at 64).
+int frag_tile(int tile, int64_t nvirt) {
+  if (tile != -1) return tile;
+  const double w64 = (double)round_up(nvirt, 64), w32 = (double)round_up(nvirt, 32);
+  return w64 * w64 >= 1.5 * w32 * w32 ? 32 : 64;
 }
 
 int num_cus(int device) {
@@ -227,16 +176,21 @@ int num_cus(int device) {
   return n > 0 ? n : 1;
 }
 
-// Packs every fragment's FP64 B (from the b64 arena) with pair tile `tile`,
-// builds the work-unit table in cost order and uploads it.
+// Packs every fragment's FP64 B (from the b64 arena) for the pair tile that
+// candidate `tile` gives it, builds the work-unit table in cost order and
+// uploads it.
 void layout_batch(Ctx* c, BatchState* b, int tile) {
   hipStream_t s = c->stream;
   const int64_t nf = b->nfrag;
   std::vector<PackGeom> g(nf);
+  std::vector<int> tf(nf);
+  std::vector<double> ucost(nf);  // cost of one work unit: auxiliary depth x tile area
   size_t packed_bytes = 0, scale_n = 0, evp_n = 0;
   std::vector<size_t> poff(nf), soff(nf), eoff(nf);
   for (int64_t f = 0; f < nf; f++) {
-    g[f] = make_geom(b->mode, b->naux[f], b->nvirt[f], tile);
+    tf[f] = frag_tile(tile, b->nvirt[f]);
+    g[f] = make_geom(b->mode, b->naux[f], b->nvirt[f], tf[f]);
+    ucost[f] = (double)g[f].nqp * (double)tf[f] * (double)tf[f];
     poff[f] = packed_bytes;
     packed_bytes += round_up((int64_t)(g[f].bytes_per_occ() * b->nocc[f]), 256);
     soff[f] = scale_n;
@@ -263,17 +217,20 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
     }
   }
   {
-    std::vector<PackItem> items(nf);
+    // one grouped packing per tile class (the operand layouts of the two
+    // tiles differ in fp32), each with its own job table
+    std::vector<PackItem> items[2];
     for (int64_t f = 0; f < nf; f++)
-      items[f] = PackItem{b->b64.as<double>() + b->b64_off[f], b->naux[f], b->nvirt[f], b->nocc[f], g[f],
-                          pk + poff[f], sc ? sc + soff[f] : nullptr, ev_raw + voff[f], evp + eoff[f]};
+      items[tf[f] == 64].push_back(PackItem{b->b64.as<double>() + b->b64_off[f], b->naux[f], b->nvirt[f],
+                                            b->nocc[f], g[f], pk + poff[f], sc ? sc + soff[f] : nullptr,
+                                            ev_raw + voff[f], evp + eoff[f]});
     ScopedTimer tm(c, RIMP2_K_PACK);
-    pack_B_group(items, b->mode, s, b->pack_jobs, b->pack_blob);
+    for (int k = 0; k < 2; k++) pack_B_group(items[k], b->mode, s, b->pack_jobs[k], b->pack_blob[k]);
   }
-  // launch order: most expensive work units (longest auxiliary loop) first
+  // launch order: most expensive work units first
   std::vector<int64_t> order(nf);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return g[x].nqp > g[y].nqp; });
+  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return ucost[x] > ucost[y]; });
   std::vector<BatchFrag> fr(nf);
   int64_t wu = 0, psq = 0;
   std::vector<int64_t> pair_off(nf);
@@ -291,9 +248,10 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
     x.f.nvp = g[f].nvp;
     x.f.nks = g[f].nks;
     x.f.nrb = g[f].nrb;
+    x.tile = tf[f];
     x.nocc = b->nocc[f];
-    x.nb = g[f].nvp / tile;
-    x.ntp = pair_units(x.nb, pair_packs_diagonal(b->mode, tile));
+    x.nb = g[f].nvp / tf[f];
+    x.ntp = pair_units(x.nb, pair_packs_diagonal(b->mode, tf[f]));
     x.npairs = x.nocc * (x.nocc + 1) / 2;
     x.wu_begin = wu;
     x.pair_off = pair_off[f];
@@ -307,18 +265,18 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
   b->partials.ensure((size_t)wu * sizeof(double));
   b->pairs.ensure((size_t)psq * sizeof(double));
   b->e_frag.ensure((size_t)nf * sizeof(double));
-  // 8 contiguous ranges of equal cost (a unit costs ~ its auxiliary stages)
+  // 8 contiguous ranges of equal cost
   {
     std::vector<double> cstart(nf + 1, 0.0);
     for (int64_t k = 0; k < nf; k++)
-      cstart[k + 1] = cstart[k] + (double)(fr[k].npairs * fr[k].ntp) * (double)fr[k].f.nks;
+      cstart[k + 1] = cstart[k] + (double)(fr[k].npairs * fr[k].ntp) * ucost[order[k]];
     b->range[0] = 0;
     int64_t k = 0;
     for (int x = 1; x < 8; x++) {
       const double target = cstart[nf] * x / 8.0;
       while (k + 1 < nf && cstart[k + 1] <= target) k++;
       const int64_t n = fr[k].npairs * fr[k].ntp;
-      int64_t u = (int64_t)std::ceil((target - cstart[k]) / (double)fr[k].f.nks);
+      int64_t u = (int64_t)std::ceil((target - cstart[k]) / ucost[order[k]]);
       u = std::min<int64_t>(std::max<int64_t>(u, 0), n);
       b->range[x] = std::max<int64_t>(fr[k].wu_begin + u, b->range[x - 1]);
     }
@@ -493,13 +451,18 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
   // the single-fragment state no longer matches the context buffers
   c->have_metric = false;
   c->have_B = false;
-  // autotune the pair tile for (mode, size class)
+  // autotune the pair tile for (mode, size class), unless the caller fixed it
   const std::pair<int, int> key(mode, size_class(b));
   auto it = b->tuned.find(key);
   b->ntune = 0;
   int best = 0;
-  if (it != b->tuned.end()) best = it->second;
-  else {
+  if (b->forced_tile) {
+    if (!valid_candidate(mode, b->forced_tile))
+      fail(RIMP2_ERR_ARG, "tile candidate %d is not available in precision mode %d", b->forced_tile, mode);
+    best = b->forced_tile;
+  } else if (it != b->tuned.end()) {
+    best = it->second;
+  } else {
     double best_ms = 1e300;
     for (int t : tile_candidates(mode)) {
       layout_batch(c, b, t);
@@ -591,6 +554,16 @@ int rimp2_batch_tuning(void* ctx, int* n, int64_t* tiles, double* ms) {
     tiles[k] = b->tune_tiles[k];
     ms[k] = b->tune_ms[k];
   }
+  RIMP2_API_END(ctx)
+}
+
+int rimp2_batch_set_tile(void* ctx, int64_t tile) {
+  RIMP2_API_BEGIN
+  Ctx* c = as_ctx(ctx);
+  if (tile != 0 && tile != 32 && tile != 64 && tile != -1)
+    fail(RIMP2_ERR_ARG, "tile candidate %lld: expected 0 (autotune), 32, 64 or -1 (mixed)", (long long)tile);
+  if (!c->batch) c->batch = new BatchState();
+  c->batch->forced_tile = (int)tile;
   RIMP2_API_END(ctx)
 }
 

@@ -270,6 +270,13 @@ module rimp2_hip_interfaces
        real(C_DOUBLE) :: ms(*)
      end function rimp2_batch_tuning
 
+     !> tile candidate of later prepares: 32, 64, -1 (mixed per fragment); 0 = autotune
+     integer(C_INT) function rimp2_batch_set_tile(ctx, tile) bind(C, name="rimp2_batch_set_tile")
+       import :: C_INT, C_PTR, C_INT64_T
+       type(C_PTR), value :: ctx
+       integer(C_INT64_T), value :: tile
+     end function rimp2_batch_set_tile
+
      ! ---- pair-screening surrogate ----
 
      !> feats(16, npairs): the C array [npairs, 16] seen column-major

@@ -54,7 +54,7 @@ def test_batch_matches_oracle(ctx, oracle, rimp2, mode):
         if mode == "fp64":
             np.testing.assert_allclose(eps, epsref, rtol=1e-10, atol=1e-15)
         np.testing.assert_array_equal(eps, eps.T)
-    assert ctx.batch_tile() in (32, 64, 128)
+    assert ctx.batch_tile() in (-1, 32, 64)
 
 
 def test_batch_graph_replays_are_identical(ctx, rimp2):
@@ -194,16 +194,17 @@ def test_batch_prepare_device_errors(ctx, rimp2):
 
 
 def test_batch_autotuner_times_every_candidate(oracle, rimp2):
-    """A fresh context times both fp32 tiles (32 and 64) on the batch and keeps
-    the faster; a second prepare of the same size class uses the cached choice
-    (no measurements).  Both tiles give the oracle's fragment energies within
-    the fp32 tolerance (the 32-wide tile runs 16x16x4 MFMAs)."""
+    """A fresh context times the three fp32 candidates (every fragment 32- or
+    64-wide, and the per-fragment mix) on the batch and keeps the fastest; a
+    second prepare of the same size class uses the cached choice (no
+    measurements).  The chosen layout gives the oracle's fragment energies
+    within the fp32 tolerance (the 32-wide tile runs 16x16x4 MFMAs)."""
     frags = make_batch(rimp2, 57, 5, scale=4)
     with rimp2.Context(0) as c:
         c.batch_prepare(frags, "fp32")
         tuning = c.batch_tuning()
         say("autotune fp32", tuning, "chosen", c.batch_tile())
-        assert sorted(t for t, _ in tuning) == [32, 64]
+        assert sorted(t for t, _ in tuning) == [-1, 32, 64]
         assert all(ms > 0 for _, ms in tuning)
         assert c.batch_tile() == min(tuning, key=lambda x: x[1])[0]
         c.batch_run()
@@ -212,3 +213,68 @@ def test_batch_autotuner_times_every_candidate(oracle, rimp2):
         assert c.batch_tuning() == []
     for k, (Eref, _) in enumerate(oracle_energies(oracle, frags)):
         assert abs(ef[k] - Eref) <= TOL_REL["fp32"] * abs(Eref), (k, ef[k], Eref)
+
+
+# The mixed candidate's rule: the 32-wide tile where the 64-wide one executes
+# at least 1.5x its MFMA work.  nvirt 20 and 70 (32 / 96 wide against 64 / 128:
+# 4x and 1.78x) take it; 150 (160 against 192: 1.44x) and 40, 100, 200 (the
+# same width either way) keep the 64-wide tile.
+MIXED_SHAPES = [(5, 20, 60), (3, 40, 100), (4, 70, 130), (2, 100, 260), (3, 150, 170), (2, 200, 140)]
+
+
+def mixed_tile(nvirt):
+    w32, w64 = -(-nvirt // 32) * 32, -(-nvirt // 64) * 64
+    return 32 if w64 * w64 >= 1.5 * w32 * w32 else 64
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_batch_mixed_tiles_per_fragment(oracle, rimp2, mode):
+    """Each tile candidate fixed by rimp2_batch_set_tile, on fragments of both
+    tile classes: every fragment's energy and pair matrix match the oracle in
+    each layout, and under the mixed candidate each fragment reproduces, bit
+    for bit, the one-tile layout of its own class (same kernel path, same
+    unit order, same reduction)."""
+    frags = [rimp2.gen_fragment(300 + k, q, v, o) for k, (o, v, q) in enumerate(MIXED_SHAPES)]
+    assert [mixed_tile(v) for _, v, _ in MIXED_SHAPES] == [32, 64, 32, 64, 64, 64]
+    ref = oracle_energies(oracle, frags)
+    res = {}
+    with rimp2.Context(0) as c:
+        for cand in (32, 64, -1):
+            c.batch_set_tile(cand)
+            c.batch_prepare(frags, mode)
+            assert c.batch_tile() == cand and c.batch_tuning() == []
+            c.batch_run()
+            res[cand] = c.batch_result()
+            say("batch", mode, "candidate", cand, res[cand][0])
+        c.batch_set_tile(0)
+    for cand, (ef, pp) in res.items():
+        off = 0
+        for k, (P, V, eo, ev) in enumerate(frags):
+            Eref, epsref = ref[k]
+            n = eo.size
+            eps = pp[off:off + n * n].reshape(n, n, order="F")
+            off += n * n
+            assert abs(ef[k] - Eref) <= TOL_REL[mode] * abs(Eref), (cand, k, ef[k], Eref)
+            if mode == "fp64":
+                np.testing.assert_allclose(eps, epsref, rtol=1e-10, atol=1e-15)
+    off = 0
+    for k, (o, v, q) in enumerate(MIXED_SHAPES):
+        same = res[mixed_tile(v)]
+        assert res[-1][0][k] == same[0][k], (k, v)
+        np.testing.assert_array_equal(res[-1][1][off:off + o * o], same[1][off:off + o * o])
+        off += o * o
+
+
+def test_batch_set_tile_rejects_unavailable_candidates(rimp2):
+    frags = [rimp2.gen_fragment(400, 64, 20, 3)]
+    with rimp2.Context(0) as c:
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            c.batch_set_tile(7)
+        assert ei.value.code == rimp2.ERR_ARG
+        c.batch_set_tile(-1)
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            c.batch_prepare(frags, "i8s5")  # the exact-slice modes have the 64-wide tile only
+        assert ei.value.code == rimp2.ERR_ARG and "not available" in str(ei.value)
+        c.batch_set_tile(0)
+        c.batch_prepare(frags, "i8s5")
+        assert c.batch_tile() == 64
