@@ -96,40 +96,9 @@ __device__ inline double bget(const PackArgs& p, int64_t q, int64_t a, int64_t i
   return (q < p.naux && a < p.nvirt) ? p.B[q + p.naux * (a + p.nvirt * i)] : 0.0;
 }
 
-// One packed element t (fragment f = t >> 6 = (i * nks + ks) * nrb + rb, lane
-// t & 63) per mode; the split-BF16 and exact-slice modes write every plane of
-// the fragment.
-template <int MODE>
-__device__ inline void pack_one(const PackArgs& p, int64_t t, int64_t nvp);
-
-template <>
-__device__ inline void pack_one<RIMP2_MODE_FP64>(const PackArgs& p, int64_t t, int64_t) {
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-  static_cast<double*>(p.dst)[t] = bget(p, q, a, i);
-}
-
-template <>
-__device__ inline void pack_one<RIMP2_MODE_FP32>(const PackArgs& p, int64_t t, int64_t) {
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q = ks * 2 + (lane >> 5);
-  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
-}
-
-// FP32 in 16x16x4 operand order (the fp64 fragment shape with 4-byte lanes).
-constexpr int PACK_F32R16 = 16;  // packer id beside the mode numbers
-template <>
-__device__ inline void pack_one<PACK_F32R16>(const PackArgs& p, int64_t t, int64_t) {
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 16 + (lane & 15), q = ks * 4 + (lane >> 4);
-  static_cast<float*>(p.dst)[t] = (float)bget(p, q, a, i);
-}
+// FP32 in 16x16x4 operand order (the fp64 fragment shape with 4-byte lanes):
+// packer id beside the mode numbers.
+constexpr int PACK_F32R16 = 16;
 
 // bf16 round-to-nearest-even of a float (no NaN inputs reach here).
 __device__ inline uint16_t f2bf(float x) {
@@ -139,128 +108,147 @@ __device__ inline uint16_t f2bf(float x) {
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// planes: 0 = hi = bf16(b), 1 = lo = bf16(b - hi)
+// Packing by tiles: one workgroup per (occupied orbital i, 32 virtual rows,
+// TQ auxiliary values).  The tile is read with coalesced loads (each wave
+// instruction reads 64 consecutive auxiliary values of one row) into LDS; each
+// thread then forms whole lanes of the MFMA fragments the tile covers and
+// writes them in fragment order, where the 64 lanes of a fragment and the
+// fragments of consecutive row blocks are contiguous, so a wave's writes are
+// contiguous too.  Packed element of lane l of fragment f = (i nks + ks) nrb +
+// rb, plane k (P planes of 64 lanes x lane bytes per fragment):
+//   fp64 / fp32 16x16x4: row 16 rb + l % 16, aux 4 ks + l / 16
+//   fp32 32x32x2:        row 32 rb + l % 32, aux 2 ks + l / 32
+//   bf16x3 32x32x16:     row 32 rb + l % 32, aux 16 ks + 8 (l / 32) + e, e < 8;
+//                        plane 0 = hi = bf16(b), plane 1 = lo = bf16(b - hi)
+//   int8 32x32x32:       row 32 rb + l % 32, aux 32 ks + 16 (l / 32) + e, e < 16;
+//                        plane k = slice k of 127 b / sigma (s_k = rint(254 r_{k-1}))
+// Tiles at the padded edges may hold fewer k-steps or row blocks; values past
+// naux or nvirt read as zero.
+template <int MODE>
+struct PackT;
 template <>
-__device__ inline void pack_one<RIMP2_MODE_BF16X3>(const PackArgs& p, int64_t t, int64_t) {
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 16 + 8 * (lane >> 5);
-  uint16_t hi[8], lo[8];
-#pragma unroll
-  for (int e = 0; e < 8; e++) {
-    float b = (float)bget(p, q0 + e, a, i);
-    hi[e] = f2bf(b);
-    lo[e] = f2bf(b - bf2f(hi[e]));
-  }
-  uint4 vh, vl;
-  vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
-  vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
-  vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
-  vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
-  uint4* dst = static_cast<uint4*>(p.dst);
-  dst[(f * 2 + 0) * 64 + lane] = vh;
-  dst[(f * 2 + 1) * 64 + lane] = vl;
+struct PackT<RIMP2_MODE_FP64> { static constexpr int RB = 16, KK = 4, P = 1, TQ = 64; };
+template <>
+struct PackT<PACK_F32R16> { static constexpr int RB = 16, KK = 4, P = 1, TQ = 64; };
+template <>
+struct PackT<RIMP2_MODE_FP32> { static constexpr int RB = 32, KK = 2, P = 1, TQ = 64; };
+template <>
+struct PackT<RIMP2_MODE_BF16X3> { static constexpr int RB = 32, KK = 16, P = 2, TQ = 64; };
+template <>
+struct PackT<RIMP2_MODE_I8S4> { static constexpr int RB = 32, KK = 32, P = 4, TQ = 128; };
+template <>
+struct PackT<RIMP2_MODE_I8S5> { static constexpr int RB = 32, KK = 32, P = 5, TQ = 128; };
+
+constexpr int TILE_ROWS = 32;
+
+// Tiles of one fragment's packing: nocc x (tiles of k-steps) x (tiles of row blocks).
+template <int MODE>
+__host__ __device__ inline int64_t pack_tiles(const PackArgs& p) {
+  constexpr int KT = PackT<MODE>::TQ / PackT<MODE>::KK, RT = TILE_ROWS / PackT<MODE>::RB;
+  return p.nocc * ((p.nks + KT - 1) / KT) * ((p.nrb + RT - 1) / RT);
 }
 
+// Exact-slice planes of one lane: 16 values x (127 / sigma) split into S int8
+// slices, s_0 = rint(x), s_k = rint(r_k) with r_k = 254 (r_{k-1} - s_{k-1}).
+// No slice leaves [-127, 127]: |x| <= 127 (1 + 2^-53)^2 because sigma is the
+// row's largest |B|, and every later |r_k| <= 254 / 2.  rint(r) is formed as
+// (r + 1.5 2^52) - 1.5 2^52 (exact for |r| < 2^51 under round-to-nearest-even,
+// the rounding of rint); the low 32 bits of r + 1.5 2^52 then hold rint(r) in
+// two's complement, so its low byte is the slice.
 template <int S>
-__device__ inline void pack_i8_one(const PackArgs& p, int64_t t, int64_t nvp) {
-  const int lane = (int)(t & 63);
-  const int64_t f = t >> 6;
-  const int64_t rb = f % p.nrb, ks = (f / p.nrb) % p.nks, i = f / (p.nrb * p.nks);
-  const int64_t a = rb * 32 + (lane & 31), q0 = ks * 32 + 16 * (lane >> 5);
-  const double sg = p.sigma[i * nvp + a];
-  const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
-  int8_t sl[S][16];
+__device__ inline void slice_lane(const double* v, double inv, uint4 (&out)[S]) {
+#pragma clang fp contract(off)  // each product and sum rounded on its own, as written
+  constexpr double M = 6755399441055744.0;  // 1.5 * 2^52
+  uint32_t w[S][4];
+#pragma unroll
+  for (int k = 0; k < S; k++)
+#pragma unroll
+    for (int d = 0; d < 4; d++) w[k][d] = 0;
 #pragma unroll
   for (int e = 0; e < 16; e++) {
-    double r = bget(p, q0 + e, a, i) * inv;
+    double x = v[e] * inv;
 #pragma unroll
     for (int k = 0; k < S; k++) {
-      double s = rint(r);
-      s = fmin(127.0, fmax(-127.0, s));
-      sl[k][e] = (int8_t)s;
-      r = (r - s) * 254.0;
+      const double t = x + M;
+      const uint32_t lo = (uint32_t)__builtin_bit_cast(uint64_t, t);
+      w[k][e >> 2] |= (lo & 0xFFu) << (8 * (e & 3));
+      if (k + 1 < S) x = (x - (t - M)) * 254.0;
     }
   }
-  uint4* dst = static_cast<uint4*>(p.dst);
 #pragma unroll
-  for (int k = 0; k < S; k++) {
-    uint4 v;
-    memcpy(&v, sl[k], 16);
-    dst[(f * S + k) * 64 + lane] = v;
-  }
-}
-template <>
-__device__ inline void pack_one<RIMP2_MODE_I8S4>(const PackArgs& p, int64_t t, int64_t nvp) {
-  pack_i8_one<4>(p, t, nvp);
-}
-template <>
-__device__ inline void pack_one<RIMP2_MODE_I8S5>(const PackArgs& p, int64_t t, int64_t nvp) {
-  pack_i8_one<5>(p, t, nvp);
+  for (int k = 0; k < S; k++) out[k] = make_uint4(w[k][0], w[k][1], w[k][2], w[k][3]);
 }
 
-// Exact-slice packing by tiles: one workgroup per (i, rb, 4 k-steps) = 32 rows
-// x 128 auxiliary values.  The tile is read with coalesced loads (one row of
-// 1 KB per wave instruction) into LDS, then thread (k-step f, lane) quantises the
-// 16 values of its fragment element from LDS and writes the S planes (each
-// wave: 1 KB per plane).  The per-element path reads 16 strided bytes from 64
-// rows per load instead.
-template <int S>
-__global__ __launch_bounds__(256) void pack_i8_tile_kernel(PackArgs p, int64_t nvp) {
-  __shared__ double tile[32][129];
-  const int64_t kt = p.nks / 4, ntiles = p.nocc * kt * p.nrb;
-  const int tid = threadIdx.x;
-  for (int64_t u = blockIdx.x; u < ntiles; u += gridDim.x) {
-    const int64_t rb = u % p.nrb, kq = (u / p.nrb) % kt, i = u / (p.nrb * kt);
-    const int64_t q0 = kq * 128;
-    {  // wave w reads rows 8w .. 8w+7, one row (128 q = 1 KB) per instruction, lane l q = 2l, 2l+1
-      const int w = tid >> 6, l = tid & 63;
-#pragma unroll
-      for (int rr = 0; rr < 8; rr++) {
-        const int row = 8 * w + rr;
-        tile[row][2 * l] = bget(p, q0 + 2 * l, rb * 32 + row, i);
-        tile[row][2 * l + 1] = bget(p, q0 + 2 * l + 1, rb * 32 + row, i);
-      }
-    }
-    __syncthreads();
-    const int f = tid >> 6, lane = tid & 63, row = lane & 31;
-    const int64_t a = rb * 32 + row;
-    const double sg = p.sigma[i * nvp + a];
-    const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
-    int8_t sl[S][16];
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      double x = tile[row][f * 32 + 16 * (lane >> 5) + e] * inv;
-#pragma unroll
-      for (int k = 0; k < S; k++) {
-        double sv = rint(x);
-        sv = fmin(127.0, fmax(-127.0, sv));
-        sl[k][e] = (int8_t)sv;
-        x = (x - sv) * 254.0;
-      }
-    }
-    const int64_t frag = (i * p.nks + kq * 4 + f) * p.nrb + rb;
-    uint4* dst = static_cast<uint4*>(p.dst);
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-      uint4 v;
-      memcpy(&v, sl[k], 16);
-      dst[(frag * S + k) * 64 + lane] = v;
-    }
-    __syncthreads();  // the tile is reused by the next iteration
-  }
-}
-
-// one thread per (i, ks, rb, lane)
+// Tile u of p's packing; all 256 threads call it.  tile: LDS [TILE_ROWS][TQ + 1].
 template <int MODE>
-__global__ __launch_bounds__(256) void pack_kernel(PackArgs p, int64_t nvp) {
-  RIMP2_GRID_STRIDE(t, p.nocc * p.nks * p.nrb * 64) pack_one<MODE>(p, t, nvp);
+__device__ void pack_tile(const PackArgs& p, int64_t nvp, int64_t u, double* tile) {
+  using T = PackT<MODE>;
+  constexpr int TQ = T::TQ, PITCH = TQ + 1, KT = TQ / T::KK, RT = TILE_ROWS / T::RB;
+  const int64_t nat = (p.nrb + RT - 1) / RT, nqt = (p.nks + KT - 1) / KT;
+  const int64_t at = u % nat, qt = (u / nat) % nqt, i = u / (nat * nqt);
+  const int64_t a0 = at * TILE_ROWS, q0 = qt * TQ;
+  const int tid = threadIdx.x;
+  {  // wave w loads rows w, w + 4, ...: 64 consecutive auxiliary values per instruction
+    const int w = tid >> 6, l = tid & 63;
+    for (int r = w; r < TILE_ROWS; r += 4)
+#pragma unroll
+      for (int h = 0; h < TQ / 64; h++) tile[r * PITCH + h * 64 + l] = bget(p, q0 + h * 64 + l, a0 + r, i);
+  }
+  __syncthreads();
+  const int nks_t = (int)min((int64_t)KT, p.nks - qt * KT), nrb_t = (int)min((int64_t)RT, p.nrb - at * RT);
+  const int items = nks_t * nrb_t * 64;
+  for (int it = tid; it < items; it += 256) {
+    const int lane = it & 63, fl = it >> 6;
+    const int rbl = fl % nrb_t, ksl = fl / nrb_t;
+    const int64_t f = (i * p.nks + qt * KT + ksl) * p.nrb + at * RT + rbl;
+    if constexpr (MODE == RIMP2_MODE_FP64 || MODE == PACK_F32R16) {
+      const double v = tile[(rbl * 16 + (lane & 15)) * PITCH + ksl * 4 + (lane >> 4)];
+      if constexpr (MODE == RIMP2_MODE_FP64) static_cast<double*>(p.dst)[f * 64 + lane] = v;
+      else static_cast<float*>(p.dst)[f * 64 + lane] = (float)v;
+    } else if constexpr (MODE == RIMP2_MODE_FP32) {
+      const double v = tile[(lane & 31) * PITCH + ksl * 2 + (lane >> 5)];
+      static_cast<float*>(p.dst)[f * 64 + lane] = (float)v;
+    } else if constexpr (MODE == RIMP2_MODE_BF16X3) {
+      const double* v = tile + (lane & 31) * PITCH + ksl * 16 + 8 * (lane >> 5);
+      uint16_t hi[8], lo[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float b = (float)v[e];
+        hi[e] = f2bf(b);
+        lo[e] = f2bf(b - bf2f(hi[e]));
+      }
+      uint4 vh, vl;
+      vh.x = hi[0] | ((uint32_t)hi[1] << 16); vh.y = hi[2] | ((uint32_t)hi[3] << 16);
+      vh.z = hi[4] | ((uint32_t)hi[5] << 16); vh.w = hi[6] | ((uint32_t)hi[7] << 16);
+      vl.x = lo[0] | ((uint32_t)lo[1] << 16); vl.y = lo[2] | ((uint32_t)lo[3] << 16);
+      vl.z = lo[4] | ((uint32_t)lo[5] << 16); vl.w = lo[6] | ((uint32_t)lo[7] << 16);
+      uint4* dst = static_cast<uint4*>(p.dst);
+      dst[(f * 2 + 0) * 64 + lane] = vh;
+      dst[(f * 2 + 1) * 64 + lane] = vl;
+    } else {
+      constexpr int S = T::P;
+      const double sg = p.sigma[i * nvp + a0 + (lane & 31)];
+      const double inv = (sg > 0.0) ? 127.0 / sg : 0.0;
+      uint4 out[S];
+      slice_lane<S>(tile + (lane & 31) * PITCH + ksl * 32 + 16 * (lane >> 5), inv, out);
+      uint4* dst = static_cast<uint4*>(p.dst);
+#pragma unroll
+      for (int k = 0; k < S; k++) dst[(f * S + k) * 64 + lane] = out[k];
+    }
+  }
+  __syncthreads();  // the tile is reused by the next one
 }
 
-// Grouped packing of many fragments (one launch per step for a batch).  Job j
-// owns the global items [first, next job's first); a chunk of 256 consecutive
-// items finds its first job once (uniform), each thread steps on from there.
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_tile_kernel(PackArgs p, int64_t nvp) {
+  __shared__ double tile[TILE_ROWS * (PackT<MODE>::TQ + 1)];
+  const int64_t n = pack_tiles<MODE>(p);
+  for (int64_t u = blockIdx.x; u < n; u += gridDim.x) pack_tile<MODE>(p, nvp, u, tile);
+}
+
+// Grouped packing of many fragments (one launch for a batch).  Job j owns the
+// global tiles [first, next job's first); a workgroup finds the job of its
+// tile by bisection (uniform).
 struct PackJob {
   PackArgs p;
   double* sigma;  // row scales written by rowmax_group_kernel (p.sigma reads them)
@@ -289,13 +277,12 @@ __device__ inline int pack_job_find(const Job* __restrict__ jobs, int n, int64_t
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void pack_group_kernel(const PackJob* __restrict__ jobs, int njobs, int64_t total) {
-  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
-    int j = pack_job_find(jobs, njobs, base);
-    const int64_t t = base + threadIdx.x;
-    if (t >= total) continue;
-    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
-    pack_one<MODE>(jobs[j].p, t - jobs[j].first, jobs[j].nvp);
+__global__ __launch_bounds__(256) void pack_tile_group_kernel(const PackJob* __restrict__ jobs, int njobs,
+                                                              int64_t total) {
+  __shared__ double tile[TILE_ROWS * (PackT<MODE>::TQ + 1)];
+  for (int64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const int j = pack_job_find(jobs, njobs, u);
+    pack_tile<MODE>(jobs[j].p, jobs[j].nvp, u - jobs[j].first, tile);
   }
 }
 
@@ -329,41 +316,48 @@ __global__ __launch_bounds__(256) void pad_group_kernel(const PadJob* __restrict
 
 }  // namespace
 
+template <int MODE>
+static void launch_pack_tiles(const PackArgs& p, int64_t nvp, hipStream_t s, unsigned max_blocks) {
+  unsigned blocks = grid_blocks(pack_tiles<MODE>(p), 1);
+  if (max_blocks && blocks > max_blocks) blocks = max_blocks;  // the kernel loops over its tiles
+  hipLaunchKernelGGL(pack_tile_kernel<MODE>, dim3(blocks), dim3(256), 0, s, p, nvp);
+}
+
 void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
             double* dscales, hipStream_t s, unsigned max_blocks) {
   ScopedTimer tm(c, RIMP2_K_PACK);
   PackArgs p{dB, naux, nvirt, nocc, g.nks, g.nrb, dscales, dst};
-  const int64_t threads = nocc * g.nks * g.nrb * 64;
-  unsigned blocks = grid_blocks(threads, 256);
-  if (max_blocks && blocks > max_blocks) blocks = max_blocks;  // grid-stride kernels
   switch (g.mode) {
-    case RIMP2_MODE_FP64:
-      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP64>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
-      break;
+    case RIMP2_MODE_FP64: launch_pack_tiles<RIMP2_MODE_FP64>(p, g.nvp, s, max_blocks); break;
     case RIMP2_MODE_FP32:
-      if (g.rb == 16) hipLaunchKernelGGL(pack_kernel<PACK_F32R16>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
-      else hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_FP32>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
+      if (g.rb == 16) launch_pack_tiles<PACK_F32R16>(p, g.nvp, s, max_blocks);
+      else launch_pack_tiles<RIMP2_MODE_FP32>(p, g.nvp, s, max_blocks);
       break;
-    case RIMP2_MODE_BF16X3:
-      hipLaunchKernelGGL(pack_kernel<RIMP2_MODE_BF16X3>, dim3(blocks), dim3(256), 0, s, p, g.nvp);
-      break;
+    case RIMP2_MODE_BF16X3: launch_pack_tiles<RIMP2_MODE_BF16X3>(p, g.nvp, s, max_blocks); break;
     case RIMP2_MODE_I8S4:
-    case RIMP2_MODE_I8S5: {
+    case RIMP2_MODE_I8S5:
       hipLaunchKernelGGL(rowmax_kernel, dim3((unsigned)ceil_div(g.nvp * nocc, 4)), dim3(256), 0, s, dB, naux, nvirt,
                          nocc, g.nvp, dscales);
       RIMP2_HIP_CHECK(hipGetLastError());
-      // tiles of 4 k-steps (the auxiliary padding is a multiple of four stages of 32)
-      if (g.nks % 4) fail(RIMP2_ERR_ARG, "exact-slice packing: %lld k-steps not a multiple of 4", (long long)g.nks);
-      const unsigned tb = grid_blocks(nocc * (g.nks / 4) * g.nrb * 64, 64);  // tiles, capped
-      if (g.mode == RIMP2_MODE_I8S4)
-        hipLaunchKernelGGL(pack_i8_tile_kernel<4>, dim3(tb), dim3(256), 0, s, p, g.nvp);
-      else
-        hipLaunchKernelGGL(pack_i8_tile_kernel<5>, dim3(tb), dim3(256), 0, s, p, g.nvp);
+      if (g.mode == RIMP2_MODE_I8S4) launch_pack_tiles<RIMP2_MODE_I8S4>(p, g.nvp, s, max_blocks);
+      else launch_pack_tiles<RIMP2_MODE_I8S5>(p, g.nvp, s, max_blocks);
       break;
-    }
     default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", g.mode);
   }
   RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+// tiles of one item's packing (the packer of mode and fragment shape rb)
+static int64_t tiles_of(int mode, int rb, const PackArgs& p) {
+  switch (mode) {
+    case RIMP2_MODE_FP64: return pack_tiles<RIMP2_MODE_FP64>(p);
+    case RIMP2_MODE_FP32: return rb == 16 ? pack_tiles<PACK_F32R16>(p) : pack_tiles<RIMP2_MODE_FP32>(p);
+    case RIMP2_MODE_BF16X3: return pack_tiles<RIMP2_MODE_BF16X3>(p);
+    case RIMP2_MODE_I8S4: return pack_tiles<RIMP2_MODE_I8S4>(p);
+    case RIMP2_MODE_I8S5: return pack_tiles<RIMP2_MODE_I8S5>(p);
+  }
+  fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
+  return 0;
 }
 
 void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, DevBuf& jobs,
@@ -381,7 +375,7 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
     j.nvp = it.g.nvp;
     j.first = ne;
     pj.push_back(j);
-    ne += it.nocc * it.g.nks * it.g.nrb * 64;
+    ne += tiles_of(mode, it.g.rb, j.p);
     dj.push_back(PadJob{it.ev, it.ev_padded, it.nvirt, np});
     np += it.g.nvp;
   }
@@ -409,19 +403,19 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
                        (const PackJob*)(d + orj), nj, nr);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
-  const dim3 gb(grid_blocks(ne, 256));
+  const dim3 gb(grid_blocks(ne, 1));  // one workgroup per tile (capped; the kernel loops)
   const PackJob* jp = (const PackJob*)(d + opj);
   switch (mode) {
-    case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP64>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_FP64: hipLaunchKernelGGL(pack_tile_group_kernel<RIMP2_MODE_FP64>, gb, dim3(256), 0, s, jp, nj, ne); break;
     case RIMP2_MODE_FP32:
-      if (items[0].g.rb == 16) hipLaunchKernelGGL(pack_group_kernel<PACK_F32R16>, gb, dim3(256), 0, s, jp, nj, ne);
-      else hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_FP32>, gb, dim3(256), 0, s, jp, nj, ne);
+      if (items[0].g.rb == 16) hipLaunchKernelGGL(pack_tile_group_kernel<PACK_F32R16>, gb, dim3(256), 0, s, jp, nj, ne);
+      else hipLaunchKernelGGL(pack_tile_group_kernel<RIMP2_MODE_FP32>, gb, dim3(256), 0, s, jp, nj, ne);
       break;
     case RIMP2_MODE_BF16X3:
-      hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_BF16X3>, gb, dim3(256), 0, s, jp, nj, ne);
+      hipLaunchKernelGGL(pack_tile_group_kernel<RIMP2_MODE_BF16X3>, gb, dim3(256), 0, s, jp, nj, ne);
       break;
-    case RIMP2_MODE_I8S4: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_I8S4>, gb, dim3(256), 0, s, jp, nj, ne); break;
-    case RIMP2_MODE_I8S5: hipLaunchKernelGGL(pack_group_kernel<RIMP2_MODE_I8S5>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_I8S4: hipLaunchKernelGGL(pack_tile_group_kernel<RIMP2_MODE_I8S4>, gb, dim3(256), 0, s, jp, nj, ne); break;
+    case RIMP2_MODE_I8S5: hipLaunchKernelGGL(pack_tile_group_kernel<RIMP2_MODE_I8S5>, gb, dim3(256), 0, s, jp, nj, ne); break;
     default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", mode);
   }
   RIMP2_HIP_CHECK(hipGetLastError());
