@@ -68,6 +68,22 @@ PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile) {
 
 namespace {
 
+// This lane's share of max_q |col[q]| over q = lane, lane + 64, ... < n: four
+// independent maxima, so four loads of the wave are in flight at once (one
+// chain of load-then-max per lane leaves the pass latency-bound).
+__device__ inline double col_absmax(const double* __restrict__ col, int64_t n, int lane) {
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  int64_t q = lane;
+  for (; q + 192 < n; q += 256) {
+    m0 = fmax(m0, fabs(col[q]));
+    m1 = fmax(m1, fabs(col[q + 64]));
+    m2 = fmax(m2, fabs(col[q + 128]));
+    m3 = fmax(m3, fabs(col[q + 192]));
+  }
+  for (; q < n; q += 64) m0 = fmax(m0, fabs(col[q]));
+  return fmax(fmax(m0, m1), fmax(m2, m3));
+}
+
 // sigma[i*nvp + a] = max_Q |B[Q,a,i]| (0 for padding rows); one wave per row.
 __global__ __launch_bounds__(256) void rowmax_kernel(const double* __restrict__ B, int64_t naux, int64_t nvirt,
                                                      int64_t nocc, int64_t nvp, double* __restrict__ sigma) {
@@ -76,10 +92,7 @@ __global__ __launch_bounds__(256) void rowmax_kernel(const double* __restrict__ 
   if (row >= nvp * nocc) return;
   const int64_t i = row / nvp, a = row % nvp;
   double m = 0.0;
-  if (a < nvirt) {
-    const double* col = B + naux * (a + nvirt * i);
-    for (int64_t q = lane; q < naux; q += 64) m = fmax(m, fabs(col[q]));
-  }
+  if (a < nvirt) m = col_absmax(B + naux * (a + nvirt * i), naux, lane);
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   if (lane == 0) sigma[row] = m;
 }
@@ -294,10 +307,7 @@ __global__ __launch_bounds__(256) void rowmax_group_kernel(const PackJob* __rest
     const PackJob& J = jobs[pack_job_find(jobs, njobs, row)];
     const int64_t r = row - J.first, i = r / J.nvp, a = r % J.nvp;
     double m = 0.0;
-    if (a < J.p.nvirt) {
-      const double* col = J.p.B + J.p.naux * (a + J.p.nvirt * i);
-      for (int64_t q = lane; q < J.p.naux; q += 64) m = fmax(m, fabs(col[q]));
-    }
+    if (a < J.p.nvirt) m = col_absmax(J.p.B + J.p.naux * (a + J.p.nvirt * i), J.p.naux, lane);
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
     if (lane == 0) J.sigma[r] = m;
   }
