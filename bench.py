@@ -467,12 +467,14 @@ def c2_extra(r, steps, warmup):
 
 def sustained_i8(iters=4_000_000):
     """The int8 MFMA rate this card sustains on random operands, from the probe
-    in rimp2_gpu_amd/measure/mfma_ceiling.hip (one launch of ~2.5 s each): the
+    in rimp2_gpu_amd/measure/mfma_ceiling.hip (one launch of 1-2.5 s each): the
     pair kernel's 15 slice products per k-step with operands in registers
-    only, and with the next step's operands read from LDS at the pair kernel's
-    ratio (10 KB per wave per 15 MFMAs).  The clock is the one the matrix
-    pipes would need to give the measured rate at 100% busy, a lower bound on
-    the engine clock of the run.  None where the probe is not built."""
+    only; with the next step's operands read from LDS at the pair kernel's
+    ratio (10 KB per wave per 15 MFMAs); and with the pair kernel's whole
+    operand flow (those reads, a barrier and 40 KB of global -> LDS DMA per
+    stage and CU).  The clock is the one the matrix pipes would need to give
+    the measured rate at 100% busy, a lower bound on the engine clock of the
+    run.  None where the probe is not built."""
     import ctypes
     path = os.path.join(ROOT, "rimp2_gpu_amd", "librimp2_measure.so")
     if not os.path.exists(path):
@@ -482,9 +484,10 @@ def sustained_i8(iters=4_000_000):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 3
     out = {}
-    for name, lds in (("register_operands", 0), ("with_lds_reads", 1)):
+    for name, lds, n in (("register_operands", 0, iters), ("with_lds_reads", 1, iters),
+                         ("pair_kernel_operand_flow", 2, iters // 2)):
         s, o, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        if f(0, iters, lds, ctypes.byref(s), ctypes.byref(o), ctypes.byref(m)) != 0:
+        if f(0, n, lds, ctypes.byref(s), ctypes.byref(o), ctypes.byref(m)) != 0:
             log("[bench] sustained-rate probe failed")
             return None
         out[name] = dict(tops=o.value / s.value / 1e12, seconds=s.value,
@@ -644,7 +647,7 @@ def main():
         # the same achieved rate against what this card holds on random int8 operands
         roof["sustained_int8_mfma"] = sustained
         roof["frac_of_sustained"] = {k: hw_rate / sustained[k]["tops"]
-                                     for k in ("register_operands", "with_lds_reads")}
+                                     for k in ("register_operands", "with_lds_reads", "pair_kernel_operand_flow")}
 
     # ---- optional legs, within the wall-clock budget ----
     def left():

@@ -1,6 +1,8 @@
 """The benchmark's probe of the int8 MFMA rate the card sustains
-(rimp2_gpu_amd/measure/mfma_ceiling.hip): it runs, counts the operations it
-issues, and reports a rate at or below the published dense INT8 peak."""
+(rimp2_gpu_amd/measure/mfma_ceiling.hip), in its three forms (registers only,
+LDS reads, the pair kernel's operand flow with DMA): it runs, counts the
+operations it issues, and reports a rate at or below the published dense INT8
+peak."""
 import ctypes
 import os
 
@@ -11,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("lds_reads", [0, 1])
+@pytest.mark.parametrize("lds_reads", [0, 1, 2])
 def test_sustained_rate_probe(lds_reads):
     f = ctypes.CDLL(os.path.join(ROOT, "rimp2_gpu_amd", "librimp2_measure.so")).rimp2_measure_i8_mfma
     f.restype = ctypes.c_int
