@@ -47,7 +47,8 @@ def build_library(jobs=8):
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(compile_one, zip(srcs, objs)))
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-o", LIB + ".tmp"] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-o", LIB + ".tmp"] + objs +
+         ["-L" + os.path.join(ROCM, "lib"), "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -72,7 +72,12 @@ static hipEvent_t get_event(Ctx* c) {
   return e;
 }
 
+static const char* const kTimerNames[RIMP2_NUM_KERNELS] = {"pair",  "transform", "pack",     "cholesky",
+                                                            "trtri", "reduce",    "batch",    "mlp",
+                                                            "features", "eos",    "eos_ampl"};
+
 ScopedTimer::ScopedTimer(Ctx* ctx, int k) : c(ctx), kid(k) {
+  roctxRangePushA(k >= 0 && k < RIMP2_NUM_KERNELS ? kTimerNames[k] : "kernel");
   if (!c || !c->timing) return;
   e0 = get_event(c);
   e1 = get_event(c);
@@ -80,8 +85,8 @@ ScopedTimer::ScopedTimer(Ctx* ctx, int k) : c(ctx), kid(k) {
 }
 
 ScopedTimer::~ScopedTimer() {
-  if (!e0) return;
-  if (hipEventRecord(e1, c->stream) == hipSuccess) c->pending.push_back({kid, e0, e1});
+  if (e0 && hipEventRecord(e1, c->stream) == hipSuccess) c->pending.push_back({kid, e0, e1});
+  roctxRangePop();
 }
 
 static void resolve_timers(Ctx* c) {

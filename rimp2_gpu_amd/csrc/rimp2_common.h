@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdarg>
 #include <cstdint>
@@ -131,6 +132,20 @@ void h2d_copy(Ctx* c, hipStream_t s, void* dst, const void* src, size_t bytes);
 
 // Records hipEvents around a launch on ctx->stream when timing is on (no-op
 // for a null ctx).
+// A roctx range over a host scope: `rocprofv3 --marker-trace` shows it on the
+// host timeline (an asynchronous call's range covers its enqueue, a blocking
+// call's range its whole run).  With no tool attached, push and pop are a
+// dispatch-table call each.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
+
+// Event timer of one kernel class (RIMP2_K_*) around the launches of a scope,
+// on the context stream when the context's timing is on; the scope is also a
+// roctx range named after the class.
 struct ScopedTimer {
   Ctx* c;
   int kid;
@@ -273,8 +288,11 @@ inline unsigned grid_blocks(int64_t items, int64_t threads) {
 
 }  // namespace rimp2
 
-// Wraps an extern "C" body: converts rimp2::Error into a status code.
-#define RIMP2_API_BEGIN try {
+// Wraps an extern "C" body: converts rimp2::Error into a status code.  The
+// call is a roctx range named after the entry point.
+#define RIMP2_API_BEGIN                             \
+  ::rimp2::TraceRange rimp2_api_range_(__func__); \
+  try {
 #define RIMP2_API_END(ctxp)                                         \
   }                                                                 \
   catch (const ::rimp2::Error& e) {                                 \
