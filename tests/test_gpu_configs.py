@@ -112,6 +112,35 @@ def test_c3_every_fragment_against_the_oracle(ctx, c3, mode):
         assert abs(eps.sum() - ef[k]) <= 1e-12 * abs(ef[k])
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+def test_c3_every_tile_candidate_against_the_oracle(rimp2, c3, mode):
+    """The 256 fragments under each fixed tile candidate (every fragment 32-
+    or 64-wide, and the per-fragment mix, whose launch carries both tile
+    paths): every fragment's energy against the oracle, and under the mix each
+    fragment equal bit for bit to the one-tile layout of its own tile."""
+    frags, ref = c3
+    res = {}
+    with rimp2.Context(0) as c:
+        for cand in (32, 64, -1):
+            c.batch_set_tile(cand)
+            c.batch_prepare(frags, mode)
+            assert c.batch_tile() == cand
+            c.batch_run()
+            res[cand], _ = c.batch_result(want_pairs=False)
+            rel = np.abs(res[cand] - ref) / np.abs(ref)
+            say("c3", mode, "candidate", cand, "max rel dE", rel.max())
+            assert rel.max() <= TOL_E_REL[mode], (cand, int(np.argmax(rel)))
+    w32 = lambda v: -(-v // 32) * 32
+    w64 = lambda v: -(-v // 64) * 64
+    n32 = 0
+    for k, (P, V, eo, ev) in enumerate(frags):
+        nv = P.shape[1]
+        t = 32 if w64(nv) ** 2 >= 1.5 * w32(nv) ** 2 else 64
+        n32 += t == 32
+        assert res[-1][k] == res[t][k], (k, nv, t)
+    assert 0 < n32 < len(frags)
+
+
 # ---------------------------------------------------------------------------
 # C4: surrogate trained on 512 C2-shape fragments, screening a C4 fragment
 # ---------------------------------------------------------------------------
