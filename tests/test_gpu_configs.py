@@ -5,7 +5,7 @@ CPU oracle.
                          against the oracle's full FP64 energy
   C3  256 DEC fragments  (nocc ~ U{4..32}, nvirt ~ U{32..256}, naux = 4 nvirt)
                          every fragment's E in fp64, fp32 and i8s5 against the
-                         oracle
+                         oracle; in fp64 and fp32 under every tile candidate
   C4  128/1024/3072      surrogate trained on the exact pair energies of 512
                          seeded 64/512/2048 fragments, then the screened
                          energy of a 128/1024/3072 fragment: every kept pair
@@ -14,7 +14,8 @@ CPU oracle.
                          seeded subset (the first and the last units of the
                          launch included) against the oracle's pair loops on an
                          independent LAPACK metric (rel 1e-11); i8s5 within
-                         1 uEh of fp64
+                         1 uEh of fp64; fp32 and bf16x3 (the modes this
+                         configuration compares) at their tolerances below
 
 Tolerances (E_ref: the oracle's FP64 energy):
   fp64   |dE| <= 1e-11 |E_ref|, pair energies rel 1e-10
@@ -247,6 +248,14 @@ def test_c5_pair_energies_and_exact_slice_energy(ctx, oracle, rimp2):
         ctx.sync()
         E8 = float(ctx.from_device(det, (1,))[0])
         eps8 = ctx.from_device(dpe, (nocc, nocc))
+        # the modes this configuration compares with fp64: FP32 and split-BF16
+        # MFMA with FP32 accumulation (packed operands of 12.9 and 25.8 GB)
+        other = {}
+        for mode in ("fp32", "bf16x3"):
+            say("c5", mode)
+            ctx.energy_device(naux, nvirt, nocc, *b, mode, dpe, det)
+            ctx.sync()
+            other[mode] = (float(ctx.from_device(det, (1,))[0]), ctx.from_device(dpe, (nocc, nocc)))
         V = ctx.from_device(b[1], (naux, naux))
         eo, ev = ctx.from_device(b[2], (nocc,)), ctx.from_device(b[3], (nvirt,))
         X = np.empty((naux, nvirt, len(orbs)), order="F")
@@ -274,3 +283,12 @@ def test_c5_pair_energies_and_exact_slice_energy(ctx, oracle, rimp2):
     np.testing.assert_allclose(got, ref, rtol=1e-11)
     # the total is the sum of the pair matrix
     assert abs(eps64.sum() - E64) <= 1e-12 * abs(E64)
+    # FP32 and split BF16: total against the fp64 total (checked above against
+    # the oracle to 1e-11) and the sampled pairs against the oracle, at their
+    # stated relative tolerances
+    for mode, (Em, epsm) in other.items():
+        say("c5", mode, "E", Em, "dE", Em - E64, "rel", (Em - E64) / E64, "pair rel",
+            np.abs(epsm[pi, pj] - ref) / np.abs(ref))
+        check_E(mode, Em, E64)
+        check_pairs(mode, epsm[pi, pj], ref)
+        np.testing.assert_array_equal(epsm, epsm.T)
