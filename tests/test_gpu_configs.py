@@ -9,7 +9,8 @@ CPU oracle.
   C4  128/1024/3072      surrogate trained on the exact pair energies of 512
                          seeded 64/512/2048 fragments, then the screened
                          energy of a 128/1024/3072 fragment: every kept pair
-                         carries the exact kernel's energy
+                         carries the exact kernel's energy, and a seeded sample
+                         of the exact kernel's pair energies matches the oracle
   C5  256/2048/6144      B has 3.2e9 elements (> 2^31): fp64 pair energies of a
                          seeded subset (the first and the last units of the
                          launch included) against the oracle's pair loops on an
@@ -145,7 +146,8 @@ def test_c3_every_tile_candidate_against_the_oracle(rimp2, c3, mode):
 # ---------------------------------------------------------------------------
 # C4: surrogate trained on 512 C2-shape fragments, screening a C4 fragment
 # ---------------------------------------------------------------------------
-def test_c4_surrogate_trained_on_512_fragments_screens_with_exact_energies(ctx, rimp2):
+def test_c4_surrogate_trained_on_512_fragments_screens_with_exact_energies(ctx, rimp2, oracle):
+    import scipy.linalg as sl
     from rimp2_gpu_amd import surrogate
     thresh = 1e-6
     say("c4: training set from 512 seeded 64/512/2048 fragments (i8s5 exact pair energies)")
@@ -167,6 +169,15 @@ def test_c4_surrogate_trained_on_512_fragments_screens_with_exact_energies(ctx, 
         Fc = ctx.pair_features(eo, ev)
         pred = ctx.mlp_predict(Fc)
         Es, eps_s, nskip = ctx.screened_energy(eo, ev, "i8s5", thresh)
+        # inputs of a seeded sample of pairs for the oracle (the last pair included)
+        rng = np.random.default_rng(4)
+        pi = np.concatenate([[0, 127, 3], rng.integers(0, nocc, 4)])
+        pj = np.concatenate([[0, 127, 120], rng.integers(0, nocc, 4)])
+        orbs = np.unique(np.concatenate([pi, pj]))
+        V = ctx.from_device(b[1], (naux, naux))
+        X = np.empty((naux, nvirt, len(orbs)), order="F")
+        for k, i in enumerate(orbs):
+            X[:, :, k] = ctx.from_device(b[0] + int(i) * naux * nvirt * 8, (naux, nvirt))
     finally:
         for p in b + [dpe, det]:
             ctx.free(p)
@@ -177,6 +188,16 @@ def test_c4_surrogate_trained_on_512_fragments_screens_with_exact_energies(ctx, 
     base = float(np.sqrt(np.mean((Fc[:, 15].astype(np.float64) - ltrue) ** 2)))
     say("c4 E", E, "screened", Es, "skipped", nskip, "of", len(ii), "pred log rmse", rmse, "base", base,
         "min |eps|", float(np.min(-eps)))
+    # the exact kernel at this shape: sampled i8s5 pair energies against the
+    # oracle's pair loops on an independent LAPACK metric, within 1 uEh
+    L = sl.cholesky(V, lower=True)
+    B = np.asfortranarray(sl.solve_triangular(L, X.reshape(naux, -1, order="F"), lower=True).reshape(X.shape,
+                                                                                                     order="F"))
+    loc = {int(i): k for k, i in enumerate(orbs)}
+    ref = oracle.pair_energies_subset(B, eo[orbs], ev, np.array([loc[int(i)] for i in pi]),
+                                      np.array([loc[int(j)] for j in pj]))
+    say("c4 sampled pairs |d| vs oracle", np.abs(eps[pi, pj] - ref))
+    check_pairs("i8s5", eps[pi, pj], ref)
     # the network has learned the pair energies of an unseen, larger shape
     assert rmse < 0.5 and rmse < base
     # kept pairs carry the exact kernel's energies; a skipped pair contributes 0
