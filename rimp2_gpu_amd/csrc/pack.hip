@@ -197,6 +197,8 @@ template <int MODE>
 __device__ void pack_tile(const PackArgs& p, int64_t nvp, int64_t u, double* tile) {
   using T = PackT<MODE>;
   constexpr int TQ = T::TQ, PITCH = TQ + 1, KT = TQ / T::KK, RT = TILE_ROWS / T::RB;
+  static_assert(T::P >= 1 && TQ % 64 == 0 && TQ % T::KK == 0 && TILE_ROWS % T::RB == 0,
+                "a tile holds whole k-steps and row blocks and loads 64 values per wave instruction");
   const int64_t nat = (p.nrb + RT - 1) / RT, nqt = (p.nks + KT - 1) / KT;
   const int64_t at = u % nat, qt = (u / nat) % nqt, i = u / (nat * nqt);
   const int64_t a0 = at * TILE_ROWS, q0 = qt * TQ;
