@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 
 #include "rimp2_common.h"
@@ -35,16 +36,30 @@ Ctx* as_ctx(void* ctx) {
   return c;
 }
 
+// Fault injection for tests of the allocation-failure path: with
+// RIMP2_FAULT_ALLOC_BYTES=n in the environment, every allocation of the
+// library's own device buffers of at least n bytes fails as hipMalloc would.
+// Read at each allocation, so a caller can set and clear it between calls.
+static bool injected_alloc_failure(size_t want) {
+  const char* v = getenv("RIMP2_FAULT_ALLOC_BYTES");
+  if (!v || !*v) return false;
+  char* end = nullptr;
+  const unsigned long long n = strtoull(v, &end, 10);
+  return end != v && want >= n;
+}
+
 void* DevBuf::ensure(size_t need) {
   if (need <= bytes && ptr) return ptr;
   release();
   size_t want = need < 256 ? 256 : need;
-  hipError_t e = hipMalloc(&ptr, want);
+  const bool injected = injected_alloc_failure(want);
+  hipError_t e = injected ? hipErrorOutOfMemory : hipMalloc(&ptr, want);
   if (e != hipSuccess) {
     ptr = nullptr;
     bytes = 0;
     (void)hipGetLastError();
-    fail(RIMP2_ERR_NOMEM, "device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+    fail(RIMP2_ERR_NOMEM, "device allocation of %zu bytes failed: %s%s", want, hipGetErrorString(e),
+         injected ? " (injected by RIMP2_FAULT_ALLOC_BYTES)" : "");
   }
   bytes = want;
   return ptr;

@@ -498,3 +498,33 @@ def test_energy_larger_than_free_memory_is_refused_before_allocation(ctx, rimp2)
     P, V, eo2, ev2 = rimp2.gen_fragment(14, 40, 8, 2)
     E, _ = ctx.energy(P, V, eo2, ev2, "fp64")
     assert E < 0
+
+
+@pytest.mark.parametrize("limit_mib", [1, 4])
+def test_allocation_failure_is_reported_and_the_context_recovers(rimp2, limit_mib):
+    """RIMP2_FAULT_ALLOC_BYTES makes every library buffer of at least that
+    size fail as hipMalloc would. On a fresh context at naux 512, nvirt 128,
+    nocc 16 a 1 MiB limit fails the first buffer (the 2 MiB metric) and a
+    4 MiB limit fails later, after the metric buffers exist (the 8 MiB (P|ia)
+    and B). Both surface as RIMP2_ERR_NOMEM, and with the variable cleared the
+    same context then forms the golden energy."""
+    g = np.load(os.path.join(GOLD, "frag_4_512_128_16.npz"))
+    say("injected allocation failure, limit MiB", limit_mib)
+    P, V, eo, ev = rimp2.gen_fragment(int(g["seed"]), int(g["naux"]), int(g["nvirt"]), int(g["nocc"]))
+    c = rimp2.Context(0)
+    try:
+        os.environ["RIMP2_FAULT_ALLOC_BYTES"] = str(limit_mib << 20)
+        try:
+            for mode in ("fp64", "i8s5"):
+                with pytest.raises(rimp2.Rimp2Error) as ei:
+                    c.energy(P, V, eo, ev, mode)
+                assert ei.value.code == rimp2.ERR_NOMEM
+                assert "injected" in str(ei.value)
+        finally:
+            del os.environ["RIMP2_FAULT_ALLOC_BYTES"]
+        for mode in ("fp64", "i8s5"):
+            E, eps = c.energy(P, V, eo, ev, mode)
+            check_E(mode, E, float(g["E"]))
+        np.testing.assert_allclose(c.energy(P, V, eo, ev, "fp64")[1], g["eps_pair"], rtol=1e-10, atol=1e-14)
+    finally:
+        c.close()
