@@ -5,7 +5,8 @@ Adam kernels live in the HIP library (mlp.hip); this module drives them:
 
   * training_set: fragments are generated on the GPU, their exact pair
     energies computed by the pair kernel, and the 16 per-pair features taken
-    from the same transformed B (rimp2_pair_features);
+    from the same transformed B (rimp2_pair_features); optionally cached per
+    fragment, so an interrupted run resumes where it stopped;
   * train: standardises features / target (log(-eps_ij)) and runs Adam steps
     on minibatches;
   * screen_report: runs the exact energy, the MLP-screened energy and a
@@ -15,6 +16,7 @@ Adam kernels live in the HIP library (mlp.hip); this module drives them:
 No part of the energy ever comes from the network: screened energies are the
 exact kernel's energies over the kept pairs.
 """
+import os
 import time
 
 import numpy as np
@@ -32,33 +34,58 @@ def _device_fragment(ctx, seed, naux, nvirt, nocc, lam, bufs):
     return eo, ev
 
 
+def _cache_path(cache_dir, seed, nocc, nvirt, naux, lam, mode):
+    return os.path.join(cache_dir, f"frag_{seed}_{nocc}_{nvirt}_{naux}_{lam!r}_{mode}.npz")
+
+
 def training_set(ctx, nfrag, nocc=64, nvirt=512, naux=2048, seed0=100000, lam=0.0, mode="i8s5", log=None,
-                 min_log_eps=-46.0):
+                 min_log_eps=-46.0, cache_dir=None):
     """Features [npairs, 16] and targets log(-eps_ij) over nfrag
     device-generated fragments (seeds seed0 .. seed0+nfrag-1).  Pairs with
     -eps_ij below exp(min_log_eps) (1e-20 by default, far below any screening
     threshold; some underflow to zero) are left out: they cannot change a
-    screening decision and would dominate the residual's statistics."""
+    screening decision and would dominate the residual's statistics.
+
+    cache_dir: checkpoint of a long training-set run.  Each fragment's
+    features and targets are written there (one .npz per fragment, named by
+    seed, shape, lam and mode; written to a temporary name and renamed, so a
+    crash leaves no partial file) and read back instead of recomputed when
+    the run is repeated."""
     bufs = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
     dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
-    iu, ju = np.triu_indices(nocc)  # i <= j ... ordered j-major below
+    # unique pairs i <= j, ordered j-major as the features are
     jj = np.concatenate([np.full(j + 1, j) for j in range(nocc)])
     ii = np.concatenate([np.arange(j + 1) for j in range(nocc)])
+    if cache_dir:
+        os.makedirs(cache_dir, exist_ok=True)
     F, T = [], []
+    reused = 0
     try:
         for f in range(nfrag):
+            cp = _cache_path(cache_dir, seed0 + f, nocc, nvirt, naux, lam, mode) if cache_dir else None
+            if cp and os.path.exists(cp):
+                with np.load(cp) as z:
+                    F.append(z["F"])
+                    T.append(z["T"])
+                reused += 1
+                continue
             eo, ev = _device_fragment(ctx, seed0 + f, naux, nvirt, nocc, lam, bufs)
             ctx.energy_device(naux, nvirt, nocc, bufs[0], bufs[1], bufs[2], bufs[3], mode, dpe, det)
             ctx.sync()
             eps = ctx.from_device(dpe, (nocc, nocc))
             F.append(ctx.pair_features(eo, ev))
             T.append(np.log(np.maximum(-eps[ii, jj], 1e-300)).astype(np.float32))
+            if cp:
+                tmp = cp + ".tmp.npz"
+                np.savez(tmp, F=F[-1], T=T[-1])
+                os.replace(tmp, cp)
             if log and (f % 32 == 0 or f == nfrag - 1):
                 log(f"[surrogate] training fragment {f + 1}/{nfrag}")
     finally:
         for p in bufs + [dpe, det]:
             ctx.free(p)
-    del iu, ju
+    if log and reused:
+        log(f"[surrogate] {reused} of {nfrag} training fragments read from {cache_dir}")
     F, T = np.concatenate(F), np.concatenate(T)
     keep = T > min_log_eps
     return F[keep], T[keep]

@@ -278,3 +278,40 @@ def test_batch_set_tile_rejects_unavailable_candidates(rimp2):
         c.batch_set_tile(0)
         c.batch_prepare(frags, "i8s5")
         assert c.batch_tile() == 64
+
+
+class CountingCtx:
+    """Passes batch calls through to a Context and counts the fragments."""
+
+    def __init__(self, c):
+        self.c, self.nfrag = c, 0
+
+    def batch(self, frags, mode, want_pairs=True):
+        self.nfrag += len(frags)
+        return self.c.batch(frags, mode, want_pairs)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "i8s5"])
+def test_batch_results_file_resumes_to_the_straight_result(rimp2, tmp_path, mode):
+    """A run over 12 fragments whose first 7 an earlier (interrupted) run
+    recorded: only the other 5 are computed, and every fragment's energy and
+    pair matrix equals a straight batch of all 12 bit for bit (one tile
+    fixed, so a fragment's result does not depend on the batch around it)."""
+    from rimp2_gpu_amd import results
+    frags = make_batch(rimp2, 31, 12, scale=4)
+    keys = [f"seed{31 + k}" for k in range(12)]
+    say("results file", mode, [f[0].shape for f in frags])
+    p = str(tmp_path / "c3.jsonl")
+    with rimp2.Context(0) as c:
+        c.batch_set_tile(64)
+        ef, pp = c.batch(frags, mode)
+        results.run_batch(c, frags[:7], keys[:7], p, mode=mode, chunk=4)
+        cc = CountingCtx(c)
+        E, P = results.run_batch(cc, frags, keys, p, mode=mode, chunk=4)
+    assert cc.nfrag == 5
+    np.testing.assert_array_equal(E, ef)
+    off = 0
+    for k, (Pk, V, eo, ev) in enumerate(frags):
+        n = eo.size
+        np.testing.assert_array_equal(P[k], pp[off:off + n * n].reshape(n, n, order="F"))
+        off += n * n

@@ -9,6 +9,8 @@ accumulation order and the rare bf16 rounding flip, so predictions are held to
 are recomputed in numpy from the FP64 B.  Screening: the exact kernel's pair
 energies on the kept pairs are bitwise those of the unscreened run.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -145,3 +147,40 @@ def test_network_gains_over_the_base_estimate_across_shapes(ctx, rimp2):
     assert rep["pred_log_rmse"] < rep["base_log_rmse"]
     assert abs(rep["dE_screened"]) < abs(rep["dE_bound_screen"])
     assert rep["mlp_skipped"] >= 0.9 * rep["ideal_skipped"]
+
+
+def test_training_set_cache_resumes_without_recomputing(ctx, rimp2, tmp_path):
+    """With cache_dir, a repeated training-set run reads every fragment back
+    (the pair kernel is not called again) and returns the same arrays; a
+    partial cache (an interrupted run) is completed."""
+    from rimp2_gpu_amd import surrogate
+    kw = dict(nocc=8, nvirt=32, naux=128, seed0=900, lam=1.5, mode="fp64")
+    say("training-set cache")
+    F0, T0 = surrogate.training_set(ctx, 4, **kw)
+    d = str(tmp_path / "cache")
+    F1, T1 = surrogate.training_set(ctx, 2, cache_dir=d, **kw)  # the interrupted run
+    assert len(os.listdir(d)) == 2
+    np.testing.assert_array_equal(F1, F0[:len(F1)])
+    np.testing.assert_array_equal(T1, T0[:len(T1)])
+
+    class CountEnergy:
+        def __init__(self, c):
+            self.c, self.calls = c, 0
+
+        def __getattr__(self, name):
+            return getattr(self.c, name)
+
+        def energy_device(self, *a):
+            self.calls += 1
+            return self.c.energy_device(*a)
+
+    proxy = CountEnergy(ctx)
+    F2, T2 = surrogate.training_set(proxy, 4, cache_dir=d, **kw)
+    assert proxy.calls == 2 and len(os.listdir(d)) == 4
+    np.testing.assert_array_equal(F2, F0)
+    np.testing.assert_array_equal(T2, T0)
+    proxy = CountEnergy(ctx)
+    F3, T3 = surrogate.training_set(proxy, 4, cache_dir=d, **kw)
+    assert proxy.calls == 0
+    np.testing.assert_array_equal(F3, F0)
+    np.testing.assert_array_equal(T3, T0)
