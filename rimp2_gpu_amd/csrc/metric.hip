@@ -417,6 +417,13 @@ using mg::TransformArgs;
 // stage buffers (32 KB per 16-deep K stage: 16 KB of L^{-1}, 16 KB of X) fed
 // by buffer_load_dwordx4 ... lds, next-stage operands read under the current
 // stage's MFMAs, one DMA + six LDS reads per eight MFMAs.
+// A wave stops its LDS reads and MFMAs after the last stage where its 32-row
+// strip of L^{-1} has nonzeros (row q ends at column q), and does none in a
+// strip that lies wholly in the padding beyond naux; it still issues its share
+// of the DMA and meets every barrier.  The products skipped are exact zeros,
+// so B is unchanged.  Strips pair up on the SIMDs as (0, 3) and (1, 2)
+// (waves w and w + 4 share a SIMD), so in the diagonal block each SIMD keeps
+// 10 of the 16 strip-stages of a full one.
 __device__ __attribute__((always_inline)) inline void transform_unit(const TransformArgs& p, int64_t wu, unsigned char* __restrict__ buf0,
                                       unsigned char* __restrict__ buf1, unsigned char* __restrict__ buf2,
                                       unsigned char* __restrict__ buf3) {
@@ -430,13 +437,17 @@ __device__ __attribute__((always_inline)) inline void transform_unit(const Trans
   const int64_t mt = wu % p.MT, col = wu / p.MT, i = col / p.NT, nt = col % p.NT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave < 4 ? wave >> 1 : 5 - (wave >> 1), wc = wave & 1;  // strips 0 0 1 1 3 3 2 2
   const int64_t m0 = mt * TR_TM;
   // L^{-1} is lower triangular: the K loop of row tile mt stops at its last
   // row (m0 + 128 is a whole number of four-stage groups; so is KS * 4)
   int64_t kend = m0 + TR_TM;
   if (kend > p.KS * 4) kend = p.KS * 4;
   const int64_t nst = kend / 16;
+  // stages [0, wst) carry this wave's nonzeros; wst is even (strips are two stages deep)
+  int64_t wst = (m0 + 32 * (wr + 1)) / 16;
+  if (wst > nst) wst = nst;
+  if (m0 + 32 * wr >= p.naux) wst = 0;
   const size_t a_ks = (size_t)p.MB * FB, b_ks = (size_t)p.NRB * FB;
   const size_t a_off = (size_t)(mt * 8) * FB, b_off = (size_t)nt * 8 * FB;
   const __amdgpu_buffer_rsrc_t ra = pairk::slab_rsrc(p.Lp + a_off, (size_t)p.KS * a_ks - a_off);
@@ -499,6 +510,12 @@ __device__ __attribute__((always_inline)) inline void transform_unit(const Trans
       __builtin_amdgcn_sched_group_barrier(0x0008, 8, 0);
     }
   };
+  // a stage past wst: the DMA and the barrier only
+  auto idle = [&](int64_t st, unsigned char* freed) {
+    RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
+    pipeline_barrier();
+    issue(st + 4 < nst ? st + 4 : nst - 1, freed);
+  };
   Ops oa, ob;
   issue(0, buf0);
   issue(1, buf1);
@@ -506,12 +523,28 @@ __device__ __attribute__((always_inline)) inline void transform_unit(const Trans
   issue(3, buf3);
   RIMP2_WAIT_VMCNT(3 * NI);
   pipeline_barrier();
-  load(buf0, oa);
-  for (int64_t st = 0; st < nst; st += 4) {
+  if (wst > 0) load(buf0, oa);
+  // every loop bound below is uniform over the wave; all waves run nst stages
+  const int64_t wst4 = wst & ~(int64_t)3;
+  int64_t st = 0;
+  for (; st < wst4; st += 4) {
     step(st, oa, ob, buf0, buf1);
     step(st + 1, ob, oa, buf1, buf2);
     step(st + 2, oa, ob, buf2, buf3);
     step(st + 3, ob, oa, buf3, buf0);
+  }
+  if (st < wst) {  // wst - st == 2
+    step(st, oa, ob, buf0, buf1);
+    step(st + 1, ob, oa, buf1, buf2);
+    idle(st + 2, buf2);
+    idle(st + 3, buf3);
+    st += 4;
+  }
+  for (; st < nst; st += 4) {
+    idle(st, buf0);
+    idle(st + 1, buf1);
+    idle(st + 2, buf2);
+    idle(st + 3, buf3);
   }
   RIMP2_WAIT_VMCNT_LGKM0(0);
   // epilogue: D layout d[r] = D[4r + l/16][l%16]
@@ -662,17 +695,17 @@ __global__ __launch_bounds__(256) void pack_x_group_kernel(const XPackJob* __res
 
 using mg::TransformJob;
 
+// XCD x takes the units [r.b[x], r.b[x + 1]) (ranges of equal cost, from the
+// host); its workgroups stride through them.
 __global__ __launch_bounds__(512) void transform_group_kernel(const TransformJob* __restrict__ jobs, int njobs,
-                                                              int64_t total, int64_t chunk) {
+                                                              mg::XcdRanges r) {
   constexpr int SB = 32768;
   __shared__ __align__(16) unsigned char buf0[SB];
   __shared__ __align__(16) unsigned char buf1[SB];
   __shared__ __align__(16) unsigned char buf2[SB];
   __shared__ __align__(16) unsigned char buf3[SB];
   const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3, nslot = gridDim.x >> 3;
-  for (int64_t k = slot; k < chunk; k += nslot) {
-    const int64_t wu = xcd * chunk + k;
-    if (wu >= total) break;
+  for (int64_t wu = r.b[xcd] + slot; wu < r.b[xcd + 1]; wu += nslot) {
     const int j = job_find(jobs, njobs, wu);
     const TransformArgs a = jobs[j].a;
     transform_pair(a, wu - jobs[j].first, buf0, buf1, buf2, buf3);
@@ -925,10 +958,13 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
         hipLaunchKernelGGL(pack_x_group_kernel, dim3((unsigned)st.total), dim3(256), 0, s, (const XPackJob*)jp, st.njobs);
         break;
       case mg::GL_TRANSFORM: {
-        const int64_t chunk = ceil_div(st.total, (int64_t)8);
-        const int64_t nslot = chunk < kMaxGridBlocks / 8 ? chunk : kMaxGridBlocks / 8;
+        const mg::XcdRanges r = mg::transform_xcd_ranges(
+            reinterpret_cast<const TransformJob*>(G.blob.data() + st.off), st.njobs, st.total);
+        int64_t most = 0;
+        for (int x = 0; x < 8; x++) most = std::max(most, r.b[x + 1] - r.b[x]);
+        const int64_t nslot = std::max<int64_t>(1, std::min(most, kMaxGridBlocks / 8));
         hipLaunchKernelGGL(transform_group_kernel, dim3((unsigned)(8 * nslot)), dim3(512), 0, s,
-                           (const TransformJob*)jp, st.njobs, st.total, chunk);
+                           (const TransformJob*)jp, st.njobs, r);
         break;
       }
     }

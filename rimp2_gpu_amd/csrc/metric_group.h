@@ -135,6 +135,61 @@ struct TransformJob {
   int64_t first;  // first global work unit
 };
 
+// K stages (16 deep) of transform row tile mt, and of work unit t of a column
+// tile (row tiles t and MT-1-t): the unit's cost, near enough.
+inline int64_t transform_tile_stages(const TransformArgs& a, int64_t mt) {
+  return std::min((mt + 1) * kTrTM, a.KS * 4) / 16;
+}
+inline int64_t transform_unit_stages(const TransformArgs& a, int64_t t) {
+  return transform_tile_stages(a, t) + (a.MT - 1 - t != t ? transform_tile_stages(a, a.MT - 1 - t) : 0);
+}
+
+// Units [b[x], b[x + 1]) of a grouped transform go to XCD x.  The jobs list
+// fragments by decreasing naux, so equal unit counts per XCD would give XCD 0
+// the longest units; instead b[x] is the first unit at which the units before
+// it reach x/8 of the total cost.
+struct XcdRanges {
+  int64_t b[9];
+};
+inline XcdRanges transform_xcd_ranges(const TransformJob* jobs, int njobs, int64_t total) {
+  XcdRanges r{};
+  std::vector<int64_t> col_cost((size_t)njobs);
+  double all = 0.0;
+  for (int j = 0; j < njobs; j++) {
+    const TransformArgs& a = jobs[j].a;
+    int64_t c = 0;
+    for (int64_t t = 0; t < (a.MT + 1) / 2; t++) c += transform_unit_stages(a, t);
+    col_cost[(size_t)j] = c;
+    all += (double)c * (double)(a.nwu / ((a.MT + 1) / 2));
+  }
+  double acc = 0.0;  // cost of the units before job j
+  int x = 1;
+  for (int j = 0; j < njobs && x < 8; j++) {
+    const TransformArgs& a = jobs[j].a;
+    const int64_t mth = (a.MT + 1) / 2, cc = col_cost[(size_t)j];
+    const double job = (double)cc * (double)(a.nwu / mth);
+    while (x < 8 && all * x / 8.0 <= acc + job) {
+      // smallest k with (cost of the job's first k units) >= rem
+      const double rem = all * x / 8.0 - acc;
+      int64_t full = cc > 0 ? (int64_t)(rem / (double)cc) : 0;
+      if (full * mth > a.nwu) full = a.nwu / mth;
+      double got = (double)full * (double)cc;
+      int64_t k = full * mth, t = 0;
+      while (got < rem && k < a.nwu) {
+        got += (double)transform_unit_stages(a, t);
+        t++;
+        k++;
+      }
+      r.b[x++] = jobs[j].first + k;
+    }
+    acc += job;
+  }
+  for (; x < 8; x++) r.b[x] = total;
+  r.b[0] = 0;
+  r.b[8] = total;
+  return r;
+}
+
 enum GroupLaunch { GL_COPY, GL_DENOM, GL_ZERO_LINV, GL_POTRF, GL_PANEL, GL_SYRK, GL_TRI_T, GL_TRI_L, GL_LPACK,
                    GL_XPACK, GL_TRANSFORM };
 

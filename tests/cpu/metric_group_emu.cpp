@@ -373,6 +373,30 @@ int main(int argc, char** argv) {
           // tile of one occupied orbital, K up to its last row
           const TransformJob* J = reinterpret_cast<const TransformJob*>(jp);
           check_firsts(J, st.njobs, st.total);
+          {
+            // the XCD ranges: ascending, covering every unit once, each
+            // within one unit's cost of an eighth of the total
+            const XcdRanges r = transform_xcd_ranges(J, st.njobs, st.total);
+            if (r.b[0] != 0 || r.b[8] != st.total) die("XCD ranges do not cover the units", r.b[0], r.b[8]);
+            std::vector<int64_t> ucost((size_t)st.total);
+            int64_t all = 0, umax = 0;
+            for (int j = 0; j < st.njobs; j++) {
+              const TransformArgs& a = J[j].a;
+              const int64_t mth = (a.MT + 1) / 2;
+              for (int64_t w2 = 0; w2 < a.nwu; w2++) {
+                const int64_t c = transform_unit_stages(a, w2 % mth);
+                ucost[(size_t)(J[j].first + w2)] = c;
+                all += c;
+                umax = std::max(umax, c);
+              }
+            }
+            for (int x = 0; x < 8; x++) {
+              if (r.b[x + 1] < r.b[x]) die("XCD ranges not ascending", x);
+              int64_t c = 0;
+              for (int64_t u = r.b[x]; u < r.b[x + 1]; u++) c += ucost[(size_t)u];
+              if (std::fabs((double)c - all / 8.0) > (double)umax) die("XCD range cost off balance", x, c);
+            }
+          }
           for (int j = 0; j < st.njobs; j++) {
             const TransformArgs& a = J[j].a;
             if (job_end(J, st.njobs, j, st.total) - J[j].first != a.nwu) die("transform unit count", j);
