@@ -606,21 +606,46 @@ __global__ void zero_upper_kernel(double* A, int64_t n, int64_t lda) {
 // Copy src -> dst (dst == src: check only); bit 4 of status on a non-finite value.
 using mg::CopyJob;
 
-// Grid-stride over [0, total) in chunks of 256 consecutive elements: the job
-// of a chunk's first element is found once for the workgroup (uniform, scalar
-// loads); each thread then steps past the jobs that end before its element.
+// Grid-stride over [0, total) in chunks of 64 x 256 consecutive elements.
+// A chunk inside one job (the common case: a metric holds n^2 >= 16384
+// elements from n = 128) runs with the job's fields in scalar registers and 64
+// independent loads per thread; a chunk across a job boundary steps each
+// element to its job.
 __global__ __launch_bounds__(256) void copy_check_group_kernel(const CopyJob* __restrict__ jobs, int njobs,
                                                                int64_t total) {
-  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
+  constexpr int64_t CH = 64 * 256;
+  const int tid = threadIdx.x;
+  for (int64_t base = (int64_t)blockIdx.x * CH; base < total; base += (int64_t)gridDim.x * CH) {
+    const int64_t end = base + CH < total ? base + CH : total;
     int j = job_find(jobs, njobs, base);
-    const int64_t t = base + threadIdx.x;
-    if (t >= total) continue;
-    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
-    const CopyJob& J = jobs[j];
-    const int64_t e = t - J.first;
-    const double v = J.src[e];
-    if (J.dst != J.src) J.dst[e] = v;
-    if (!isfinite(v)) atomicOr(J.status, 4);
+    const int64_t jend = j + 1 < njobs ? jobs[j + 1].first : total;
+    if (end <= jend) {
+      const CopyJob J = jobs[j];
+      const double* src = J.src + (base - J.first);
+      double* dst = J.dst + (base - J.first);
+      const bool copy = J.dst != J.src;
+      const int64_t cnt = end - base;
+      bool bad = false;
+#pragma unroll 16
+      for (int r = 0; r < 64; r++) {
+        const int64_t e = (int64_t)r * 256 + tid;
+        if (e < cnt) {
+          const double v = src[e];
+          if (copy) dst[e] = v;
+          bad |= !isfinite(v);
+        }
+      }
+      if (bad) atomicOr(J.status, 4);
+      continue;
+    }
+    for (int64_t t = base + tid; t < end; t += 256) {
+      while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
+      const CopyJob& J = jobs[j];
+      const int64_t e = t - J.first;
+      const double v = J.src[e];
+      if (J.dst != J.src) J.dst[e] = v;
+      if (!isfinite(v)) atomicOr(J.status, 4);
+    }
   }
 }
 
@@ -643,19 +668,35 @@ __global__ __launch_bounds__(64) void denom_group_kernel(const DenomJob* __restr
 // L^{-1} of one fragment into A-fragment order (as pack_linv_kernel).
 using mg::LinvPackJob;
 
+// One 64-row x 64-column tile of L^{-1} per workgroup (tiles the transform
+// never reads are skipped: linv_tile_read): coalesced column reads (512
+// bytes per wave) into LDS, zeros above the diagonal and beyond n, then the
+// tile's 16 k-steps x 4 row blocks written as 512-byte fragments.
 __global__ __launch_bounds__(256) void pack_linv_group_kernel(const LinvPackJob* __restrict__ jobs, int njobs,
                                                               int64_t total) {
-  for (int64_t base = (int64_t)blockIdx.x * 256; base < total; base += (int64_t)gridDim.x * 256) {
-    int j = job_find(jobs, njobs, base);
-    const int64_t t = base + threadIdx.x;
-    if (t >= total) continue;
-    while (j + 1 < njobs && jobs[j + 1].first <= t) j++;
-    const LinvPackJob& J = jobs[j];
-    const int64_t e = t - J.first;
-    const int lane = (int)(e & 63);
-    const int64_t f = e >> 6, mb = f % J.MB, ks = f / J.MB;
-    const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4);
-    J.Lp[e] = (m < J.n && k < J.n && k <= m) ? J.Li[m + J.n * k] : 0.0;
+  __shared__ double tile[64][65];  // [column][row]
+  const int tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const LinvPackJob& J = jobs[job_find(jobs, njobs, t)];
+    const int64_t u = t - J.first, kt = J.KS / 16;
+    const int64_t mbt = u / kt, kst = u - mbt * kt;
+    if (kst >= 2 * (mbt / 2 + 1)) continue;  // not read (mg::linv_tile_read); uniform over the workgroup
+    const int64_t m0 = mbt * 64, k0 = kst * 64;
+#pragma unroll 4
+    for (int r = 0; r < 16; r++) {
+      const int cl = (tid >> 6) + 4 * r;
+      const int64_t m = m0 + (tid & 63), k = k0 + cl;
+      tile[cl][tid & 63] = (m < J.n && k < J.n && k <= m) ? J.Li[m + J.n * k] : 0.0;
+    }
+    __syncthreads();
+    double* out = J.Lp + ((kst * 16) * J.MB + mbt * 4) * 64;
+#pragma unroll 4
+    for (int r = 0; r < 16; r++) {
+      const int e = 256 * r + tid, fr = e >> 6, lane = e & 63;
+      const int ksl = fr >> 2, mbl = fr & 3;
+      out[((int64_t)ksl * J.MB + mbl) * 64 + lane] = tile[4 * ksl + (lane >> 4)][16 * mbl + (lane & 15)];
+    }
+    __syncthreads();  // the tile is reused by the next one
   }
 }
 
@@ -919,12 +960,12 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
   RIMP2_HIP_CHECK(hipMemcpyAsync(dj, G.blob.data(), G.blob.size(), hipMemcpyHostToDevice, s));
   for (const mg::GroupStep& st : steps) {
     const void* jp = dj + st.off;
-    const unsigned eb = grid_blocks(st.total, 256);
     if (st.kind >= mg::GL_PANEL && st.kind <= mg::GL_TRI_L && st.total > kMaxGridBlocks)
       fail(RIMP2_ERR_ARG, "grouped metric launch too large (%lld tiles)", (long long)st.total);
     switch (st.kind) {
       case mg::GL_COPY:
-        hipLaunchKernelGGL(copy_check_group_kernel, dim3(eb), dim3(256), 0, s, (const CopyJob*)jp, st.njobs, st.total);
+        hipLaunchKernelGGL(copy_check_group_kernel, dim3(grid_blocks(st.total, 64 * 256)), dim3(256), 0, s,
+                           (const CopyJob*)jp, st.njobs, st.total);
         break;
       case mg::GL_DENOM:
         hipLaunchKernelGGL(denom_group_kernel, dim3((unsigned)st.njobs), dim3(64), 0, s, (const DenomJob*)jp);
@@ -950,8 +991,8 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
                            0, s, (const GemmJob*)jp, st.njobs);
         break;
       case mg::GL_LPACK:
-        hipLaunchKernelGGL(pack_linv_group_kernel, dim3(eb), dim3(256), 0, s, (const LinvPackJob*)jp, st.njobs,
-                           st.total);
+        hipLaunchKernelGGL(pack_linv_group_kernel, dim3((unsigned)std::min<int64_t>(st.total, kMaxGridBlocks)),
+                           dim3(256), 0, s, (const LinvPackJob*)jp, st.njobs, st.total);
         break;
       case mg::GL_XPACK:  // one workgroup per 64 x 64 tile
         if (st.total > kMaxGridBlocks) fail(RIMP2_ERR_ARG, "grouped X packing too large (%lld tiles)", (long long)st.total);

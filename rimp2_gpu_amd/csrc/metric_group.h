@@ -114,12 +114,19 @@ struct DenomJob {  // bit 8 when max(eo) >= min(ev)
   int64_t nocc, nvirt;
   int* status;
 };
+// L^{-1} of one fragment into A-fragment order Lp[ks][mb][lane] (lane l <->
+// row 16 mb + l % 16, column 4 ks + l / 16), in tiles of 64 rows (4 row
+// blocks) x 64 columns (16 k-steps); tiles = (MB / 4) * (KS / 16).  The
+// transform's K loop for a 128-row tile stops at that tile's last row, so
+// tile (mbt, kst) is read only when kst < 2 (mbt / 2 + 1); the others are
+// never written (linv_tile_read).
 struct LinvPackJob {
   const double* Li;
   double* Lp;
-  int64_t n, MB;
-  int64_t first;
+  int64_t n, MB, KS;
+  int64_t first;  // first global tile
 };
+inline bool linv_tile_read(int64_t mbt, int64_t kst) { return kst < 2 * (mbt / 2 + 1); }
 // X of one fragment into B-fragment order Xp[g][ks][rb][lane] (lane l <->
 // q = 4 ks + l / 16, column 16 rb + l % 16 of group g), in tiles of 64 q x
 // 64 columns (16 k-steps x 4 row blocks); bit 4 of status on a non-finite value.
@@ -359,8 +366,8 @@ inline void plan(const std::vector<Item>& items, const std::vector<Copy>& copies
       a.Lp = ar.Lp + g.lpo[k];
       a.Xp = ar.Xp + g.xpo[k];
       a.B = it.B;
-      lp.push_back(LinvPackJob{Lik(k), reinterpret_cast<double*>(ar.Lp + g.lpo[k]), it.n, a.MB, el});
-      el += a.KS * a.MB * 64;
+      lp.push_back(LinvPackJob{Lik(k), reinterpret_cast<double*>(ar.Lp + g.lpo[k]), it.n, a.MB, a.KS, el});
+      el += (a.MB / 4) * (a.KS / 16);
       xp.push_back(XPackJob{it.X, reinterpret_cast<double*>(ar.Xp + g.xpo[k]), it.n, cols, a.KS, a.NRB, it.status, ex});
       ex += groups * (a.KS / 16) * (a.NRB / 4);
       tr.push_back(TransformJob{a, eu});

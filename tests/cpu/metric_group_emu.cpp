@@ -327,13 +327,24 @@ int main(int argc, char** argv) {
         case GL_LPACK: {
           const LinvPackJob* J = reinterpret_cast<const LinvPackJob*>(jp);
           check_firsts(J, st.njobs, st.total);
-          for (int j = 0; j < st.njobs; j++)
-            for (int64_t e = 0; e < job_end(J, st.njobs, j, st.total) - J[j].first; e++) {
-              const int lane = (int)(e & 63);
-              const int64_t f = e >> 6, mb = f % J[j].MB, ks = f / J[j].MB;
-              const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4), n = J[j].n;
-              wr(J[j].Lp + e, (m < n && k < n && k <= m) ? rd(J[j].Li + m + n * k, "Linv") : 0.0, "Lp");
+          // tiles of 64 rows x 64 columns; tiles the transform does not read
+          // stay unwritten (NaN in this arena, so a read of one would show)
+          for (int j = 0; j < st.njobs; j++) {
+            const LinvPackJob& x = J[j];
+            const int64_t kt = x.KS / 16, tiles = job_end(J, st.njobs, j, st.total) - x.first;
+            if (x.MB % 4 || x.KS % 16 || tiles != (x.MB / 4) * kt) die("L^-1 packing tile count", j, tiles);
+            for (int64_t u = 0; u < tiles; u++) {
+              const int64_t mbt = u / kt, kst = u % kt;
+              if (!linv_tile_read(mbt, kst)) continue;
+              for (int64_t ks = kst * 16; ks < kst * 16 + 16; ks++)
+                for (int64_t mb = mbt * 4; mb < mbt * 4 + 4; mb++)
+                  for (int lane = 0; lane < 64; lane++) {
+                    const int64_t m = mb * 16 + (lane & 15), k = ks * 4 + (lane >> 4), n = x.n;
+                    wr(x.Lp + (ks * x.MB + mb) * 64 + lane, (m < n && k < n && k <= m) ? rd(x.Li + m + n * k, "Linv") : 0.0,
+                       "Lp");
+                  }
             }
+          }
           break;
         }
         case GL_XPACK: {
