@@ -682,12 +682,14 @@ __global__ __launch_bounds__(256) void pack_linv_group_kernel(const LinvPackJob*
     const int64_t mbt = u / kt, kst = u - mbt * kt;
     if (kst >= 2 * (mbt / 2 + 1)) continue;  // not read (mg::linv_tile_read); uniform over the workgroup
     const int64_t m0 = mbt * 64, k0 = kst * 64;
-#pragma unroll 4
+    double v[16];  // all 16 loads in flight before the LDS stores
+#pragma unroll
     for (int r = 0; r < 16; r++) {
-      const int cl = (tid >> 6) + 4 * r;
-      const int64_t m = m0 + (tid & 63), k = k0 + cl;
-      tile[cl][tid & 63] = (m < J.n && k < J.n && k <= m) ? J.Li[m + J.n * k] : 0.0;
+      const int64_t m = m0 + (tid & 63), k = k0 + (tid >> 6) + 4 * r;
+      v[r] = (m < J.n && k < J.n && k <= m) ? J.Li[m + J.n * k] : 0.0;
     }
+#pragma unroll
+    for (int r = 0; r < 16; r++) tile[(tid >> 6) + 4 * r][tid & 63] = v[r];
     __syncthreads();
     double* out = J.Lp + ((kst * 16) * J.MB + mbt * 4) * 64;
 #pragma unroll 4
@@ -715,13 +717,17 @@ __global__ __launch_bounds__(256) void pack_x_group_kernel(const XPackJob* __res
   const double* Xg = J.X + J.naux * J.cols * g;
   const int tid = threadIdx.x;
   bool bad = false;
-#pragma unroll 4
+  double v[16];  // all 16 loads in flight before the LDS stores
+#pragma unroll
   for (int r = 0; r < 16; r++) {
     const int cl = (tid >> 6) + 4 * r;
     const int64_t q = q0 + (tid & 63), c = c0 + cl;
-    const double v = (q < J.naux && c < J.cols) ? Xg[q + J.naux * c] : 0.0;
-    bad |= !isfinite(v);
-    tile[cl][tid & 63] = v;
+    v[r] = (q < J.naux && c < J.cols) ? Xg[q + J.naux * c] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    bad |= !isfinite(v[r]);
+    tile[(tid >> 6) + 4 * r][tid & 63] = v[r];
   }
   __syncthreads();
   double* out = J.Xp + ((g * J.nks + kst * 16) * J.nrb + rbt * 4) * 64;

@@ -203,11 +203,18 @@ __device__ void pack_tile(const PackArgs& p, int64_t nvp, int64_t u, double* til
   const int64_t at = u % nat, qt = (u / nat) % nqt, i = u / (nat * nqt);
   const int64_t a0 = at * TILE_ROWS, q0 = qt * TQ;
   const int tid = threadIdx.x;
-  {  // wave w loads rows w, w + 4, ...: 64 consecutive auxiliary values per instruction
+  {  // wave w loads rows w, w + 4, ...: 64 consecutive auxiliary values per instruction,
+     // all of a thread's loads issued before its LDS stores
     const int w = tid >> 6, l = tid & 63;
-    for (int r = w; r < TILE_ROWS; r += 4)
+    double v[TILE_ROWS / 4][TQ / 64];
 #pragma unroll
-      for (int h = 0; h < TQ / 64; h++) tile[r * PITCH + h * 64 + l] = bget(p, q0 + h * 64 + l, a0 + r, i);
+    for (int rr = 0; rr < TILE_ROWS / 4; rr++)
+#pragma unroll
+      for (int h = 0; h < TQ / 64; h++) v[rr][h] = bget(p, q0 + h * 64 + l, a0 + w + 4 * rr, i);
+#pragma unroll
+    for (int rr = 0; rr < TILE_ROWS / 4; rr++)
+#pragma unroll
+      for (int h = 0; h < TQ / 64; h++) tile[(w + 4 * rr) * PITCH + h * 64 + l] = v[rr][h];
   }
   __syncthreads();
   const int nks_t = (int)min((int64_t)KT, p.nks - qt * KT), nrb_t = (int)min((int64_t)RT, p.nrb - at * RT);
