@@ -72,16 +72,26 @@ def append(path, records):
     fd = os.open(path, os.O_RDWR | os.O_CREAT, 0o644)
     try:
         size = os.fstat(fd).st_size
-        if size > 0:
-            data = os.pread(fd, size, 0)
-            keep = data.rfind(b"\n") + 1
-            if keep != size:
-                os.ftruncate(fd, keep)
-                size = keep
-        os.pwrite(fd, text.encode(), size)
+        keep = _after_last_newline(fd, size)
+        if keep != size:
+            os.ftruncate(fd, keep)
+        os.pwrite(fd, text.encode(), keep)
         os.fsync(fd)
     finally:
         os.close(fd)
+
+
+def _after_last_newline(fd, size, block=1 << 16):
+    """Offset just past the last newline of the first `size` bytes (0 if none),
+    reading backwards from the end one block at a time."""
+    end = size
+    while end > 0:
+        start = max(0, end - block)
+        k = os.pread(fd, end - start, start).rfind(b"\n")
+        if k >= 0:
+            return start + k + 1
+        end = start
+    return 0
 
 
 def _shape(frag):

@@ -150,3 +150,18 @@ def test_mode_shape_and_key_checks(tmp_path):
     E, P = results.run_batch(ctx, Frags(4), keys, q, mode="fp64")
     assert ctx.nfrag == 4 and all(x.shape == (s[0], s[0]) for x, s in zip(P, Frags(4).shapes))
     assert os.path.getsize(q) > 0
+
+
+def test_torn_line_longer_than_the_scan_block(tmp_path):
+    """A torn record longer than append's backward-scan block is still cut
+    back to the last whole line."""
+    p = str(tmp_path / "r.jsonl")
+    results.append(p, [rec("a", -1.5)])
+    big = rec("b", -2.5, nocc=200)  # ~40000 pair energies: several scan blocks
+    line = json.dumps(big).encode()
+    assert len(line) > 3 * (1 << 16)
+    with open(p, "ab") as fh:
+        fh.write(line[: len(line) - 10])  # no newline: torn
+    results.append(p, [rec("c", -3.5)])
+    assert set(results.load(p)) == {"a", "c"}
+    assert open(p, "rb").read().count(b"\n") == 2
