@@ -39,10 +39,16 @@ constexpr int NB = (int)mg::kPanel;  // Cholesky / inverse panel width
 // so both fragment reads are free of bank conflicts.
 // ---------------------------------------------------------------------------
 // One BM x BN tile (row tile mt, column tile nt) of C; all 256 threads.
+// LDS of one tile: As[BK * (BM + 16)] then Bs[BN * (BK + 1)] doubles.
+template <int BM, int BN>
+constexpr int gemm_tile_lds_doubles() {
+  return 16 * (BM + 16) + BN * 17;
+}
+
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 __device__ __attribute__((always_inline)) inline void gemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                                      const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int64_t mt,
-                                     int64_t nt) {
+                                     int64_t nt, double* __restrict__ lds) {
   constexpr int BK = 16;
   constexpr int PA = BM + 16;  // As[k][m]
   constexpr int PB = BK + 1;   // Bs[n][k]
@@ -50,8 +56,9 @@ __device__ __attribute__((always_inline)) inline void gemm_f64_tile(int64_t M, i
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int NA = BM * BK / 512;  // double2 loads of A per thread per K tile
   constexpr int NBL = BN * BK / 512; // double2 loads of B per thread per K tile
-  __shared__ double As[BK * PA];
-  __shared__ double Bs[BN * PB];
+  static_assert(gemm_tile_lds_doubles<BM, BN>() == BK * PA + BN * PB, "tile LDS layout");
+  double* __restrict__ As = lds;
+  double* __restrict__ Bs = lds + BK * PA;
 
   const int64_t m0 = mt * BM;
   const int64_t n0 = nt * BN;
@@ -187,6 +194,15 @@ __device__ __attribute__((always_inline)) inline void gemm_f64_tile(int64_t M, i
 }
 
 template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
+__device__ __attribute__((always_inline)) inline void gemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha,
+                                                                    const double* A, int64_t lda, const double* B,
+                                                                    int64_t ldb, double beta, double* C, int64_t ldc,
+                                                                    int64_t mt, int64_t nt) {
+  __shared__ double lds[gemm_tile_lds_doubles<BM, BN>()];
+  gemm_f64_tile<BM, BN, TRANSB, TRI_A, LOWER_C>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mt, nt, lds);
+}
+
+template <int BM, int BN, bool TRANSB, bool TRI_A, bool LOWER_C>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, double alpha,
                                                        const double* A, int64_t lda, const double* B, int64_t ldb,
                                                        double beta, double* C, int64_t ldc, int64_t sA,
@@ -269,13 +285,21 @@ __device__ inline void wave_lds_sync() {
 // substitution, the blocks below by distance on f64 MFMA; see below); X goes
 // through LDS to be stored with coalesced writes by all four waves.
 // ---------------------------------------------------------------------------
+struct PotrfLds {
+  double Ls[NB][NB + 1];
+  double Xs[NB][NB + 1];
+  __align__(16) double colb[2][NB];
+  __align__(16) double invd[NB];
+  double Ts[3][16][17];  // per-wave 16 x 16 products of the inverse
+};
+
 __device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A, int64_t lda, double* Linv, int64_t ldi, int64_t k0, int kb,
-                                        int* status) {
-  __shared__ double Ls[NB][NB + 1];
-  __shared__ double Xs[NB][NB + 1];
-  __shared__ __align__(16) double colb[2][NB];
-  __shared__ __align__(16) double invd[NB];
-  __shared__ double Ts[3][16][17];  // per-wave 16 x 16 products of the inverse
+                                        int* status, PotrfLds& sh) {
+  auto& Ls = sh.Ls;
+  auto& Xs = sh.Xs;
+  auto& colb = sh.colb;
+  auto& invd = sh.invd;
+  auto& Ts = sh.Ts;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (wave == 0) {
     const int r = lane;
@@ -374,15 +398,37 @@ __device__ __attribute__((always_inline)) inline void potrf_diag_block(double* A
 
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int64_t lda, double* Linv, int64_t ldi,
                                                          int64_t k0, int kb, int* status) {
-  potrf_diag_block(A, lda, Linv, ldi, k0, kb, status);
+  __shared__ PotrfLds sh;
+  potrf_diag_block(A, lda, Linv, ldi, k0, kb, status, sh);
+}
+
+// Trailing update of a Cholesky step, A22 -= L21 L21^T (lower 64 x 64 tiles,
+// one per workgroup), with the factor of the next diagonal block in the same
+// launch: that block is tile (0, 0), so its workgroup factors it as soon as it
+// has updated it, while the other workgroups are still updating theirs.  (The
+// diagonal factor is a ~30 us latency-bound chain; here it runs under the
+// update instead of after it.)
+// The two phases share one LDS buffer, so the kernel holds the factor's 74 KB
+// (two workgroups per CU), not that plus the tile's 19 KB.
+__global__ __launch_bounds__(256) void syrk_potrf_kernel(int64_t mr, int64_t kb, const double* L21, double* L,
+                                                         int64_t n, double* Linv, int64_t r0, int kbn, int* status) {
+  static_assert(sizeof(PotrfLds) >= gemm_tile_lds_doubles<64, 64>() * sizeof(double), "shared LDS");
+  __shared__ PotrfLds sh;
+  gemm_f64_tile<64, 64, true, false, true>(mr, mr, kb, -1.0, L21, n, L21, n, 1.0, L + r0 + n * r0, n, blockIdx.x,
+                                           blockIdx.y, &sh.Ls[0][0]);
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    __syncthreads();  // the block's update is written, and the tile's LDS is free
+    potrf_diag_block(L, n, Linv, n, r0, kbn, status, sh);
+  }
 }
 
 // Grouped: workgroup g factors the diagonal block of job g.
 using mg::PotrfJob;
 
 __global__ __launch_bounds__(256) void potrf_diag_group_kernel(const PotrfJob* __restrict__ jobs) {
+  __shared__ PotrfLds sh;
   const PotrfJob& J = jobs[blockIdx.x];
-  potrf_diag_block(J.A, J.n, J.Linv, J.n, J.k0, J.kb, J.status);
+  potrf_diag_block(J.A, J.n, J.Linv, J.n, J.k0, J.kb, J.status, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -783,22 +829,25 @@ void metric_factor(const MetricView& m, int64_t n, const double* dV_in) {
   RIMP2_HIP_CHECK(hipMemsetAsync(Li, 0, (size_t)(n * n) * sizeof(double), s));
   {
     ScopedTimer t(m.timer, RIMP2_K_CHOLESKY);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, L, n, Li, n, (int64_t)0, (int)std::min<int64_t>(n, NB),
+                       status);
+    RIMP2_HIP_CHECK(hipGetLastError());
     for (int64_t k0 = 0; k0 < n; k0 += NB) {
       const int kb = (int)((n - k0) < NB ? (n - k0) : NB);
-      hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, L, n, Li, n, k0, kb, status);
-      RIMP2_HIP_CHECK(hipGetLastError());
       const int64_t r0 = k0 + kb, mr = n - r0;
       if (mr <= 0) break;
       // panel: L21 = A21 * L11^{-T}   (in place; one N tile covers kb <= 64)
       gemm_f64<64, 64, true, false, false>(s, mr, kb, kb, 1.0, L + r0 + n * k0, n, Li + k0 + n * k0, n, 0.0,
                                            L + r0 + n * k0, n);
       // trailing: A22 -= L21 L21^T  (lower tiles only; K = 64, so small
-      // tiles: the update is latency-bound and wants many workgroups).
-      // Splitting off the part beyond the next block column onto a second
-      // stream (to run under the next diagonal block) measured slower: the
-      // extra launch and stream wait per step cost more than the 15 us hidden.
-      gemm_f64<64, 64, true, false, true>(s, mr, mr, kb, -1.0, L + r0 + n * k0, n, L + r0 + n * k0, n, 1.0,
-                                          L + r0 + n * r0, n);
+      // tiles: the update is latency-bound and wants many workgroups), and
+      // the next diagonal block's factor in the same launch.  Splitting the
+      // update over a second stream instead measured slower: the extra launch
+      // and stream wait per step cost more than the 15 us hidden.
+      const unsigned tiles = (unsigned)ceil_div(mr, (int64_t)64);
+      hipLaunchKernelGGL(syrk_potrf_kernel, dim3(tiles, tiles), dim3(256), 0, s, mr, (int64_t)kb, L + r0 + n * k0, L,
+                         n, Li, r0, (int)std::min<int64_t>(mr, NB), status);
+      RIMP2_HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_blocks(n * n, 256)), dim3(256), 0, s, L, n, n);
     RIMP2_HIP_CHECK(hipGetLastError());
