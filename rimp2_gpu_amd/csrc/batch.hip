@@ -248,6 +248,7 @@ void layout_batch(Ctx* c, BatchState* b, int tile) {
     x.f.nvp = g[f].nvp;
     x.f.nks = g[f].nks;
     x.f.nrb = g[f].nrb;
+    x.f.nvirt = b->nvirt[f];
     x.tile = tf[f];
     x.nocc = b->nocc[f];
     x.nb = g[f].nvp / tf[f];

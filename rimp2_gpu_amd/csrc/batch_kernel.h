@@ -73,6 +73,7 @@ void batch_pair_kernel(BatchParams p) {
     f.nvp = uni(gload(&q->f.nvp));
     f.nks = uni(gload(&q->f.nks));
     f.nrb = uni(gload(&q->f.nrb));
+    f.nvirt = uni(gload(&q->f.nvirt));
     tile = uni(gload(&q->tile));
     nb = uni(gload(&q->nb));
     ntp = uni(gload(&q->ntp));
@@ -94,10 +95,12 @@ void batch_pair_kernel(BatchParams p) {
     double e;
     if (TS != TL && tile == TS) {
       const int kind = uni_int(unit_decode(tp, nb, packs_diagonal<PolS, TS>(), A, Bk));
-      e = pair_tile<PolS, TS>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+      e = pair_tile<PolS, TS, false, true>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2,
+                                           rimp2_buf3);
     } else {
       const int kind = uni_int(unit_decode(tp, nb, packs_diagonal<PolL, TL>(), A, Bk));
-      e = pair_tile<PolL, TL>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2, rimp2_buf3);
+      e = pair_tile<PolL, TL, false, true>(f, i, j, uni(A), uni(Bk), kind, rimp2_buf0, rimp2_buf1, rimp2_buf2,
+                                           rimp2_buf3);
     }
     if (threadIdx.x == 0) p.partials[wu] = e;
   }

@@ -247,6 +247,7 @@ void pair_energies_core(Ctx* timer, hipStream_t s, int tile, const PackGeom& g, 
     p.f.nvp = g.nvp;
     p.f.nks = g.nks;
     p.f.nrb = g.nrb;
+    p.f.nvirt = nvirt;
     p.partials = part;
     p.pair_list = d_pair_list;
     p.nocc = nocc;
@@ -308,6 +309,7 @@ void pair_amplitudes_device(hipStream_t s, const PackGeom& g, const void* packed
   p.f.nvp = g.nvp;
   p.f.nks = g.nks;
   p.f.nrb = g.nrb;
+  p.f.nvirt = nvirt;
   p.partials = nullptr;
   p.pair_list = nullptr;
   p.nocc = nocc;

@@ -188,6 +188,7 @@ struct FragView {
   const double* eps_o;          // [nocc]
   const double* eps_v;          // [nvp], padding entries = 1.0
   int64_t nvp, nks, nrb;
+  int64_t nvirt;                // virtuals before the padding (0: treat all nvp as real)
 };
 
 // Output of the amplitude form of the pair kernel: for the unordered
@@ -390,7 +391,10 @@ constexpr int pair_waves_per_eu() {
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
 // offset, so each transfer is one buffer load plus the M0 write.
 // AMPL: the amplitude form (AmplOut above) instead of the energy epilogue.
-template <class Pol, int TM, bool AMPL = false>
+// SKIP: waves whose block is all virtual padding skip their MFMAs (the batch,
+// where most fragments' nvirt is not a multiple of TM; off elsewhere, where the
+// branch measured 0.3-0.5% slower on 256/2048/6144 with nothing to skip).
+template <class Pol, int TM, bool AMPL = false, bool SKIP = false>
 __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, int64_t Bk, int kind,
                             unsigned char* __restrict__ buf0, unsigned char* __restrict__ buf1,
                             unsigned char* __restrict__ buf2, unsigned char* __restrict__ buf3,
@@ -413,7 +417,19 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int grp = wave >> 2;  // 0: tile 1, 1: tile 2
-  const int wr = (wave >> 1) & 1, wc = wave & 1;
+  // Wave tile (wr, wc) = block k = 2 wr + wc of the group's 2 x 2.  Tile 1's
+  // waves take block k = wave.  With SKIP, a wave whose block lies wholly in
+  // the virtual padding (the last block of a fragment whose nvirt is not a
+  // multiple of TM) skips its MFMAs and LDS reads, and in a unit with a
+  // padding block tile 2's waves take block 3 - (wave - 4): with waves w and
+  // w + 4 on one SIMD, each SIMD then holds block k of tile 1 and block 3 - k
+  // of tile 2, so when the second row half or the second column half of both
+  // tiles is padding every SIMD keeps one busy wave.  Otherwise tile 2's waves
+  // take block wave - 4.
+  const int64_t nv = f.nvirt > 0 ? f.nvirt : f.nvp;
+  const bool padded = SKIP && ((Bk + 1) * TM > nv || (A + 1) * TM > nv || kind == UNIT_ONE_DIAG);
+  const int wq = (grp && padded) ? 3 - (wave & 3) : (wave & 3);
+  const int wr = wq >> 1, wc = wq & 1;
 
   typename Pol::acc_t acc[FW][FW][Pol::ND];
 #pragma unroll
@@ -533,6 +549,18 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   const int64_t orbR = (tile_pair && grp == 1) ? j : i, orbC = (tile_pair && grp == 1) ? i : j;
   double eRow[FW][Pol::NE], sRow[FW][Pol::NE], eCol[FW], sCol[FW], ei = 0.0, ej = 0.0;
 
+  // a stage of a wave with nothing to compute: its DMA share and the barrier
+  auto idle = [&](int64_t st, unsigned char* freed) {
+    RIMP2_WAIT_VMCNT_LGKM0(2 * NI);
+    pipeline_barrier();
+    issue(st + 4 < nst ? st + 4 : nst - 1, freed);
+  };
+  // this wave's block has real rows and columns (a lone diagonal block leaves
+  // group 1 without a tile)
+  const int64_t rblk = (tile_pair || grp == 0) ? A : Bk, cblk = tile_pair ? Bk : rblk;
+  const bool busy =
+      !SKIP || (rblk * TM + wr * WT < nv && cblk * TM + wc * WT < nv && !(kind == UNIT_ONE_DIAG && grp == 1));
+
   Ops ra, rb;
   issue(0, buf0);
   issue(1, buf1);
@@ -540,13 +568,23 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   issue(3, buf3);
   RIMP2_WAIT_VMCNT(3 * NI);
   pipeline_barrier();
-  load(buf0, ra);
-  // nst is a multiple of 4 (the auxiliary padding guarantees it)
-  for (int64_t st = 0; st < nst; st += 4) {
-    step(st, ra, rb, buf0, buf1);
-    step(st + 1, rb, ra, buf1, buf2);
-    step(st + 2, ra, rb, buf2, buf3);
-    step(st + 3, rb, ra, buf3, buf0);
+  // nst is a multiple of 4 (the auxiliary padding guarantees it); both
+  // branches meet every barrier (busy is uniform over the wave)
+  if (busy) {
+    load(buf0, ra);
+    for (int64_t st = 0; st < nst; st += 4) {
+      step(st, ra, rb, buf0, buf1);
+      step(st + 1, rb, ra, buf1, buf2);
+      step(st + 2, ra, rb, buf2, buf3);
+      step(st + 3, rb, ra, buf3, buf0);
+    }
+  } else {
+    for (int64_t st = 0; st < nst; st += 4) {
+      idle(st, buf0);
+      idle(st + 1, buf1);
+      idle(st + 2, buf2);
+      idle(st + 3, buf3);
+    }
   }
   // the accumulators as FP64 values first: the loads below then reuse their
   // registers
@@ -709,7 +747,7 @@ __device__ double pair_tile(const FragView& f, int64_t i, int64_t j, int64_t A, 
   // unit's DMA)
   __syncthreads();
   double* wsum = reinterpret_cast<double*>(buf0);
-  if (lane == 0) wsum[wave] = tot;
+  if (lane == 0) wsum[grp * 4 + wq] = tot;  // by block: the sum below runs in block order
   __syncthreads();
   double res = 0.0;
 #pragma unroll
