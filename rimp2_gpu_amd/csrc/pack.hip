@@ -31,13 +31,15 @@ int mode_slices(int mode) {
   }
 }
 
-// Auxiliary padding quantum per mode: four LDS stages of the pair kernel (its
-// break-free loop body runs four stage buffers against two register sets).
+// Auxiliary padding quantum per mode: one LDS stage of the pair kernel (16
+// auxiliary values; its loop runs whole groups of four stages and then the
+// last 0-3), and in the exact-slice modes four stages of 32 (their kernels
+// run whole groups only).
 static int64_t q_quantum(int mode) {
   switch (mode) {
-    case RIMP2_MODE_FP64: return 4 * 16;
-    case RIMP2_MODE_FP32: return 4 * 16;
-    case RIMP2_MODE_BF16X3: return 4 * 16;
+    case RIMP2_MODE_FP64: return 16;
+    case RIMP2_MODE_FP32: return 16;
+    case RIMP2_MODE_BF16X3: return 16;
     default: return 4 * 32;
   }
 }
