@@ -380,14 +380,16 @@ constexpr int pair_waves_per_eu() {
 // The body is unrolled four times (4 buffers, 2 register sets) so every
 // buffer and register set is named statically, and the last nst % 4 stages
 // follow as straight-line steps (the auxiliary axis is padded to one stage;
-// to four in the exact-slice modes, which have no tail steps).  The prologue's four issues may run past stage nst - 1 into the
-// next orbital's block or beyond the slab record (zeros); those buffers are
-// never read.  The issue is unconditional (the last steps re-fetch stage
-// nst-1 into a free buffer, the last read fetches a valid but unused buffer):
-// a branch around the DMA makes the compiler lose track of which buffer is in
-// flight and wait for all DMA before every LDS read.  Because the packed layout stores every fragment in lane order, a
-// wave's DMA destination is one contiguous KB and the MFMA operand reads
-// (ds_read of 64 x lane_bytes) are contiguous and bank-conflict free.
+// to four in the exact-slice modes, which have no tail steps).  The
+// prologue's four issues may run past stage nst - 1 into the next orbital's
+// block or beyond the slab record (zeros); those buffers are never read.  The
+// issue is unconditional (the last steps re-fetch stage nst-1 into a free
+// buffer, the last read fetches a valid but unused buffer): a branch around
+// the DMA makes the compiler lose track of which buffer is in flight and wait
+// for all DMA before every LDS read.  Because the packed layout stores every
+// fragment in lane order, a wave's DMA destination is one contiguous KB and
+// the MFMA operand reads (ds_read of 64 x lane_bytes) are contiguous and
+// bank-conflict free.
 //
 // DMA addressing: the slab resource and the stage offset of each transfer are
 // wave-uniform (SGPRs) and the per-lane part is a loop-invariant 32-bit
