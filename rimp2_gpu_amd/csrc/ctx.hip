@@ -261,13 +261,19 @@ static void require_device_memory(Ctx* c, int64_t naux, int64_t nvirt, int64_t n
   const PackGeom g = make_geom(mode, naux, nvirt, choose_tile(mode, nvirt, nocc));
   size_t need = 3 * n2 + mg::linv_packed_bytes(naux) + mg::x_packed_bytes(naux, nvirt, nocc) + b64 +
                 g.bytes_per_occ() * (size_t)nocc + (size_t)g.nvp * (size_t)nocc * sizeof(double);
+  if (mode_slices(mode)) {  // the int8 transform's slices (X's share the packed-X buffer)
+    need += metric_i8_linv_bytes(naux) + (size_t)(naux + 64) * sizeof(double) +
+            (size_t)(nvirt + 64) * (size_t)nocc * sizeof(double);
+    const size_t xi8 = metric_i8_x_bytes(naux, nvirt, nocc), x64 = mg::x_packed_bytes(naux, nvirt, nocc);
+    if (xi8 > x64) need += xi8 - x64;
+  }
   if (from_host) {
     const size_t per = (size_t)naux * (size_t)nvirt * sizeof(double);
     need += 2 * std::max(per, std::min(b64, (size_t)32 << 20));
   }
   size_t held = 0;
   for (const DevBuf* b : {&c->V, &c->Linv, &c->work, &c->linv_packed, &c->X, &c->B64, &c->packed, &c->scales,
-                          &c->x_packed})
+                          &c->x_packed, &c->lt_slices, &c->lt_sigma, &c->x_sigma})
     held += b->bytes;
   size_t free_b = 0, total_b = 0;
   RIMP2_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
@@ -302,7 +308,7 @@ static void side_after_stream(Ctx* c) {
 
 // (P|ia) from host memory into c->B64 = L^{-1} X, by chunks of occupied
 // orbitals (about 32 MiB each). Each chunk is copied, checked for NaN/Inf and
-// packed on the side stream (which side_after_stream has started), then
+// packed (the exact-slice modes: sliced for the int8 transform) on the side stream (which side_after_stream has started), then
 // transformed on the context stream, behind whatever that stream holds by then
 // (the metric factorisation in rimp2_energy), while the host copies the next
 // chunk. A non-finite value is reported by the next check_status.
@@ -310,7 +316,7 @@ static void side_after_stream(Ctx* c) {
 // order, so the copy of chunk k + 2 starts only after chunk k has been checked
 // and packed (the transform reads the packed copy), and the raw (P|ia) never
 // needs more than two chunks of device memory.
-static void transform_from_host(Ctx* c, int64_t nvirt, int64_t nocc, const double* Pia) {
+static void transform_from_host(Ctx* c, int64_t nvirt, int64_t nocc, const double* Pia, int mode) {
   const int64_t n = c->naux;
   const int64_t per = n * nvirt;  // elements per occupied orbital
   const int64_t step = std::min(nocc, std::max<int64_t>(1, ((int64_t)32 << 20) / (per * (int64_t)sizeof(double))));
@@ -318,17 +324,25 @@ static void transform_from_host(Ctx* c, int64_t nvirt, int64_t nocc, const doubl
   double* dX = static_cast<double*>(c->X.ensure((size_t)(per * step * slots) * sizeof(double)));
   double* B = static_cast<double*>(c->B64.ensure((size_t)(per * nocc) * sizeof(double)));
   const MetricView m = ctx_metric_view(c);
-  metric_reserve_x(m, n, nvirt, nocc);
+  const bool i8 = mode_slices(mode) != 0;
+  if (i8) {
+    metric_reserve_x_i8(c, n, nvirt, nocc);
+    metric_slice_linv(c, c->stream, n);
+  } else {
+    metric_reserve_x(m, n, nvirt, nocc);
+  }
   int* status = static_cast<int*>(c->status.ensure(64));
   for (int64_t i0 = 0, k = 0; i0 < nocc; i0 += step, k++) {
     const int64_t ni = std::min(step, nocc - i0);
     double* dXi = dX + (k % slots) * step * per;
     h2d_copy(c, c->side_pack, dXi, Pia + i0 * per, (size_t)(ni * per) * sizeof(double));
     check_finite_on(c->side_pack, status, dXi, ni * per, 512);
-    metric_pack_x_range(m, c->side_pack, n, nvirt, i0, ni, dXi);
+    if (i8) metric_slice_x_range(c, c->side_pack, n, nvirt, i0, ni, dXi);
+    else metric_pack_x_range(m, c->side_pack, n, nvirt, i0, ni, dXi);
     RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_done, c->side_pack));
     RIMP2_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_pack_done, 0));
-    metric_transform_range(m, n, nvirt, i0, ni, B);
+    if (i8) metric_transform_i8_range(c, n, nvirt, i0, ni, B);
+    else metric_transform_range(m, n, nvirt, i0, ni, B);
   }
 }
 
@@ -409,7 +423,7 @@ int rimp2_ctx_destroy(void* ctx) {
   mlp_destroy(c->mlp);
   eos_destroy(c->eos);
   for (DevBuf* b : {&c->V, &c->Linv, &c->work, &c->linv_packed, &c->X, &c->B64, &c->packed, &c->scales, &c->x_packed,
-                    &c->eps_o, &c->eps_v,
+                    &c->lt_slices, &c->lt_sigma, &c->x_sigma, &c->eps_o, &c->eps_v,
                     &c->partials, &c->eps_pair, &c->etot, &c->status})
     b->release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -484,7 +498,7 @@ int rimp2_transform(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const 
   reset_status(c);
   c->have_B = false;
   side_after_stream(c);
-  transform_from_host(c, nvirt, nocc, Pia);
+  transform_from_host(c, nvirt, nocc, Pia, mode);
   check_status(c);
   pack_current_B(c, nvirt, nocc, mode);
   RIMP2_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -584,7 +598,7 @@ int rimp2_energy(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, const dou
   h2d_copy(c, c->stream, dV, V, (size_t)(naux * naux) * sizeof(double));
   check_inputs_device(c, dV, naux * naux, "V");
   metric_factor(c, naux, dV);
-  transform_from_host(c, nvirt, nocc, Pia);
+  transform_from_host(c, nvirt, nocc, Pia, mode);
   double* deo = static_cast<double*>(c->eps_o.ensure((size_t)(nocc + nvirt) * sizeof(double)));
   double* dev = deo + nocc;
   RIMP2_HIP_CHECK(hipMemcpyAsync(deo, eps_o, (size_t)nocc * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -621,8 +635,14 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   int* status = static_cast<int*>(c->status.ensure(64));
   RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_go, c->stream));
   RIMP2_HIP_CHECK(hipStreamWaitEvent(c->side_pack, c->ev_pack_go, 0));
+  const bool i8 = mode_slices(mode) != 0;
   check_finite_on(c->side_pack, status, d_Pia, naux * nvirt * nocc, 512);
-  metric_pack_x(ctx_metric_view(c), c->side_pack, naux, nvirt, nocc, d_Pia);
+  if (i8) {
+    metric_reserve_x_i8(c, naux, nvirt, nocc);
+    metric_slice_x_range(c, c->side_pack, naux, nvirt, 0, nocc, d_Pia);
+  } else {
+    metric_pack_x(ctx_metric_view(c), c->side_pack, naux, nvirt, nocc, d_Pia);
+  }
   RIMP2_HIP_CHECK(hipEventRecord(c->ev_pack_done, c->side_pack));
   check_inputs_device(c, d_V, naux * naux, "V");
   check_inputs_device(c, d_eps_o, nocc, "eps_o");
@@ -630,10 +650,12 @@ int rimp2_energy_device(void* ctx, int64_t naux, int64_t nvirt, int64_t nocc, co
   denom_check_device(c, d_eps_o, nocc, d_eps_v, nvirt);
   metric_factor(c, naux, d_V);
   c->have_metric = true;
+  if (i8) metric_slice_linv(c, c->stream, naux);
   RIMP2_HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_pack_done, 0));
   {
     double* B = static_cast<double*>(c->B64.ensure((size_t)(naux * nvirt * nocc) * sizeof(double)));
-    metric_transform_packed(ctx_metric_view(c), naux, nvirt, nocc, B);
+    if (i8) metric_transform_i8_range(c, naux, nvirt, 0, nocc, B);
+    else metric_transform_packed(ctx_metric_view(c), naux, nvirt, nocc, B);
   }
   pack_current_B(c, nvirt, nocc, mode);
   pair_energies_device(c, c->geom, c->packed.ptr, c->scales.as<double>(), nocc, nvirt, d_eps_o, d_eps_v, d_eps_pair,

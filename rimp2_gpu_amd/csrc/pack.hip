@@ -114,6 +114,9 @@ __device__ inline double bget(const PackArgs& p, int64_t q, int64_t a, int64_t i
 // FP32 in 16x16x4 operand order (the fp64 fragment shape with 4-byte lanes):
 // packer id beside the mode numbers.
 constexpr int PACK_F32R16 = 16;
+// Six int8 slices in the exact-slice layout: the operands of the int8
+// metric transform (transform_i8.hip).
+constexpr int PACK_I8S6 = 17;
 
 // bf16 round-to-nearest-even of a float (no NaN inputs reach here).
 __device__ inline uint16_t f2bf(float x) {
@@ -153,6 +156,8 @@ template <>
 struct PackT<RIMP2_MODE_I8S4> { static constexpr int RB = 32, KK = 32, P = 4, TQ = 128; };
 template <>
 struct PackT<RIMP2_MODE_I8S5> { static constexpr int RB = 32, KK = 32, P = 5, TQ = 128; };
+template <>
+struct PackT<PACK_I8S6> { static constexpr int RB = 32, KK = 32, P = 6, TQ = 128; };
 
 constexpr int TILE_ROWS = 32;
 
@@ -395,6 +400,17 @@ void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc,
       break;
     default: fail(RIMP2_ERR_ARG, "unknown precision mode %d", g.mode);
   }
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void pack_slices6(hipStream_t s, const double* src, int64_t nq, int64_t nrows, int64_t norb, int64_t nks,
+                  int64_t nrp, double* sigma, void* dst, unsigned max_blocks) {
+  if (nks * 32 < nq || nrp < nrows || nrp % 32) fail(RIMP2_ERR_ARG, "six-slice packing geometry");
+  hipLaunchKernelGGL(rowmax_kernel, dim3((unsigned)ceil_div(nrp * norb, 4)), dim3(256), 0, s, src, nq, nrows, norb, nrp,
+                     sigma);
+  RIMP2_HIP_CHECK(hipGetLastError());
+  PackArgs p{src, nq, nrows, norb, nks, nrp / 32, sigma, dst};
+  launch_pack_tiles<PACK_I8S6>(p, nrp, s, max_blocks);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

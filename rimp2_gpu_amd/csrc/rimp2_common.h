@@ -90,6 +90,8 @@ struct Ctx {
   int mode = -1;
   bool have_B = false;
   DevBuf X, B64, packed, scales, x_packed;
+  // int8 transform operands of the exact-slice modes (transform_i8.hip)
+  DevBuf lt_slices, lt_sigma, x_sigma;
   PackGeom geom;
   int tile = 0;  // pair tile (virtual block) used for the packed layout
   // pair state
@@ -179,6 +181,22 @@ void metric_pack_x_range(const MetricView& m, hipStream_t s, int64_t n, int64_t 
                          const double* dXi);
 void metric_transform_range(const MetricView& m, int64_t n, int64_t nvirt, int64_t i0, int64_t ni, double* dB);
 void metric_factor(Ctx* c, int64_t naux, const double* dV_in);  // dV_in: device
+// Metric transform on int8 MFMA for the exact-slice modes (transform_i8.hip):
+// L^{-1} (after metric_factor) and X each split into six int8 slices per
+// element with one scale per column, B = L^{-1} X from the 21 slice products
+// of order <= 5 with an FP64 epilogue (relative error ~1e-12 of B's scale).
+// metric_slice_linv slices the context's L^{-1} on stream s (c->work is its
+// scratch); metric_reserve_x_i8 sizes the X slices of nocc orbitals;
+// metric_slice_x_range slices orbitals [i0, i0 + ni) (dXi points at orbital
+// i0 of X) on stream s; metric_transform_i8_range writes those orbitals of
+// dB (the whole B) on c->stream.
+size_t metric_i8_x_bytes(int64_t n, int64_t nvirt, int64_t nocc);
+size_t metric_i8_linv_bytes(int64_t n);
+void metric_slice_linv(Ctx* c, hipStream_t s, int64_t n);
+void metric_reserve_x_i8(Ctx* c, int64_t n, int64_t nvirt, int64_t nocc);
+void metric_slice_x_range(Ctx* c, hipStream_t s, int64_t n, int64_t nvirt, int64_t i0, int64_t ni,
+                          const double* dXi);
+void metric_transform_i8_range(Ctx* c, int64_t n, int64_t nvirt, int64_t i0, int64_t ni, double* dB);
 // Grouped metric pipeline: the Cholesky, inverse and transform of many
 // fragments with one launch per step for the whole group (metric_group.h).
 // Arenas and job tables of a grouped pipeline (grow-only, reused call after
@@ -204,6 +222,13 @@ PackGeom make_geom(int mode, int64_t naux, int64_t nvirt, int tile);
 void pack_B(Ctx* c, const double* dB, int64_t naux, int64_t nvirt, int64_t nocc, const PackGeom& g, void* dst,
             double* dscales, hipStream_t s, unsigned max_blocks = 0);
 int mode_slices(int mode);
+// Six int8 slices of the columns of src [nq, nrows, norb] (nq fastest), each
+// column scaled by its largest |value| (sigma[orb * nrp + row]; 0 on padding
+// rows): the exact-slice packing (32 rows x 32 k-values per fragment,
+// orbital-major) with nks 32-deep k-steps and nrp rows (multiple of 32).
+// max_blocks (0: no cap) caps the slicing grid.
+void pack_slices6(hipStream_t s, const double* src, int64_t nq, int64_t nrows, int64_t norb, int64_t nks,
+                  int64_t nrp, double* sigma, void* dst, unsigned max_blocks = 0);
 // One fragment of a grouped packing (device arrays).
 struct PackItem {
   const double* B;      // FP64 B [naux, nvirt, nocc]
