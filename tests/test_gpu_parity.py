@@ -291,6 +291,41 @@ def test_device_path_flags_bad_input(ctx, rimp2):
     ctx.sync()  # status was cleared
 
 
+@pytest.mark.parametrize("shape", [(33, 5, 3), (256, 129, 33)])
+def test_non_finite_pia_is_found_at_any_position(ctx, rimp2, shape):
+    """The device check of (P|ia) reads 16-byte pairs on a capped grid: a NaN
+    or Inf is found as the first, a middle and the last element (for odd
+    sizes the unpaired tail), and in a pointer shifted by one element (the
+    unaligned path); the clean array passes."""
+    naux, nvirt, nocc = shape
+    n = naux * nvirt * nocc
+    say("non-finite positions", shape, n)
+    P, V, eo, ev = rimp2.gen_fragment(13, naux, nvirt, nocc)
+    dV, deo, dev = ctx.to_device(V), ctx.to_device(eo), ctx.to_device(ev)
+    det = ctx.alloc(8)
+    try:
+        for pos, bad in [(None, 0.0), (0, np.nan), (n // 2 + 1, np.inf), (n - 1, np.nan), (n - 2, -np.inf)]:
+            for shift in (0, 1):
+                Q = np.asfortranarray(P).ravel(order="F").copy()
+                if pos is not None:
+                    Q[pos] = bad
+                dQ = ctx.to_device(np.concatenate([np.zeros(shift), Q]))
+                try:
+                    ctx.energy_device(naux, nvirt, nocc, dQ + 8 * shift, dV, deo, dev, "fp64", None, det)
+                    if pos is None:
+                        ctx.sync()
+                    else:
+                        with pytest.raises(rimp2.Rimp2Error) as ei:
+                            ctx.sync()
+                        assert ei.value.code == rimp2.ERR_NUMERIC, (pos, shift)
+                finally:
+                    ctx.free(dQ)
+    finally:
+        for p in (dV, deo, dev, det):
+            ctx.free(p)
+    ctx.sync()
+
+
 @pytest.mark.parametrize("mode", ["fp64", "i8s5", "fp32", "bf16x3"])
 def test_fitted_coefficients_match_oracle(ctx, oracle, rimp2, mode):
     """Calpha read as the reference reads it (already fitted, B = Calpha)."""
