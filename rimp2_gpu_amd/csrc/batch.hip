@@ -438,7 +438,8 @@ void prepare(Ctx* c, int64_t nfrag, const rimp2_frag_desc* frags, int mode, bool
     v += (size_t)d.nvirt;
   }
   if (!on_device) h2d_copy(c, s, eo, heps.data(), heps.size() * sizeof(double));
-  metric_group_run(b->group, s, items, copies, denoms);
+  // the exact-slice modes transform on int8 MFMA (six slices per operand)
+  metric_group_run(b->group, s, items, copies, denoms, mode_slices(mode) != 0);
   std::vector<int> hst((size_t)nfrag * 2);
   RIMP2_HIP_CHECK(hipMemcpyAsync(hst.data(), fst, hst.size() * sizeof(int), hipMemcpyDeviceToHost, s));
   RIMP2_HIP_CHECK(hipStreamSynchronize(s));

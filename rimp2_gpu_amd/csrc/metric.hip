@@ -987,15 +987,16 @@ void metric_transform_range(const MetricView& m, int64_t n, int64_t nvirt, int64
 // Grouped metric pipeline (host side)
 // ---------------------------------------------------------------------------
 void MetricGroup::release() {
-  for (DevBuf* b : {&L, &Linv, &T, &Lp, &Xp, &jobs}) b->release();
+  for (DevBuf* b : {&L, &Linv, &T, &Lp, &Xp, &jobs, &Ls, &Xs, &sig, &ijobs, &sjobs}) b->release();
   blob.clear();
 }
 
 // Lays the arenas out, plans the job tables (metric_group.h), sends them to
 // the device in one copy and queues one launch per step.
 void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>& items,
-                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms) {
+                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms, bool i8) {
   for (const mg::Item& it : items) {
+    if (i8) break;  // the int8 transform checks its own slab sizes
     const TransformArgs a = mg::transform_args(it.n, it.nvirt, it.nocc);
     if ((size_t)a.KS * a.MB * 512 > 0xFFFFFFFFull || (size_t)a.KS * a.NRB * 512 > 0xFFFFFFFFull)
       fail(RIMP2_ERR_ARG, "metric transform operands exceed 4 GiB per slab (naux=%lld nvirt=%lld)", (long long)it.n,
@@ -1006,10 +1007,10 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
   ar.L = static_cast<unsigned char*>(G.L.ensure(g.ln ? g.ln : 256));
   ar.Linv = static_cast<unsigned char*>(G.Linv.ensure(g.ln ? g.ln : 256));
   ar.T = static_cast<unsigned char*>(G.T.ensure(g.ln ? g.ln : 256));
-  ar.Lp = static_cast<unsigned char*>(G.Lp.ensure(g.lpn ? g.lpn : 256));
-  ar.Xp = static_cast<unsigned char*>(G.Xp.ensure(g.xpn ? g.xpn : 256));
+  ar.Lp = static_cast<unsigned char*>(G.Lp.ensure(g.lpn && !i8 ? g.lpn : 256));
+  ar.Xp = static_cast<unsigned char*>(G.Xp.ensure(g.xpn && !i8 ? g.xpn : 256));
   std::vector<mg::GroupStep> steps;
-  mg::plan(items, copies, denoms, g, ar, steps, G.blob);
+  mg::plan(items, copies, denoms, g, ar, steps, G.blob, !i8);
   if (steps.empty()) return;
   unsigned char* dj = static_cast<unsigned char*>(G.jobs.ensure(G.blob.size()));
   RIMP2_HIP_CHECK(hipMemcpyAsync(dj, G.blob.data(), G.blob.size(), hipMemcpyHostToDevice, s));
@@ -1065,6 +1066,16 @@ void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>
       }
     }
     RIMP2_HIP_CHECK(hipGetLastError());
+  }
+  if (i8) {
+    std::vector<I8GroupItem> gi;
+    for (size_t k : g.order) {  // largest first, as the plan
+      const mg::Item& it = items[k];
+      gi.push_back(I8GroupItem{reinterpret_cast<const double*>(ar.Linv + g.lo[k]),
+                               reinterpret_cast<double*>(ar.T + g.lo[k]), it.X, it.B, it.n, it.nvirt, it.nocc,
+                               it.status});
+    }
+    transform_i8_group(G, s, gi);
   }
 }
 

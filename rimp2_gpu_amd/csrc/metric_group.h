@@ -264,8 +264,11 @@ struct Arenas {
 };
 
 // Fills steps and blob (the job tables, each 256-byte aligned in the blob).
+// transform = false: the plan ends with the inverse (L^{-1} in the Linv arena);
+// the caller transforms by other means and checks X itself.
 inline void plan(const std::vector<Item>& items, const std::vector<Copy>& copies, const std::vector<Denom>& denoms,
-                 const Layout& g, const Arenas& ar, std::vector<GroupStep>& steps, std::vector<unsigned char>& blob) {
+                 const Layout& g, const Arenas& ar, std::vector<GroupStep>& steps, std::vector<unsigned char>& blob,
+                 bool transform = true) {
   steps.clear();
   blob.clear();
   auto Lk = [&](size_t k) { return reinterpret_cast<double*>(ar.L + g.lo[k]); };
@@ -353,7 +356,7 @@ inline void plan(const std::vector<Item>& items, const std::vector<Copy>& copies
     put(GL_TRI_T, tj.data(), tj.size() * sizeof(GemmJob), (int)tj.size(), tt);
     put(GL_TRI_L, lj.data(), lj.size() * sizeof(GemmJob), (int)lj.size(), tl);
   }
-  {
+  if (transform) {
     std::vector<LinvPackJob> lp;
     std::vector<XPackJob> xp;
     std::vector<TransformJob> tr;

@@ -86,6 +86,28 @@ __device__ inline double col_absmax(const double* __restrict__ col, int64_t n, i
   return fmax(fmax(m0, m1), fmax(m2, m3));
 }
 
+// The same, and z = NaN when some value is NaN or +-Inf (fma(x, 0, z) stays a
+// signed zero for finite x; fmax alone drops NaNs).
+__device__ inline double col_absmax_z(const double* __restrict__ col, int64_t n, int lane, double& z) {
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, z0 = 0.0, z1 = 0.0;
+  int64_t q = lane;
+  for (; q + 192 < n; q += 256) {
+    const double a = col[q], b = col[q + 64], c = col[q + 128], d = col[q + 192];
+    m0 = fmax(m0, fabs(a));
+    m1 = fmax(m1, fabs(b));
+    m2 = fmax(m2, fabs(c));
+    m3 = fmax(m3, fabs(d));
+    z0 = fma(a, 0.0, fma(b, 0.0, z0));
+    z1 = fma(c, 0.0, fma(d, 0.0, z1));
+  }
+  for (; q < n; q += 64) {
+    m0 = fmax(m0, fabs(col[q]));
+    z0 = fma(col[q], 0.0, z0);
+  }
+  z = z0 + z1;
+  return fmax(fmax(m0, m1), fmax(m2, m3));
+}
+
 // sigma[i*nvp + a] = max_Q |B[Q,a,i]| (0 for padding rows); one wave per row.
 __global__ __launch_bounds__(256) void rowmax_kernel(const double* __restrict__ B, int64_t naux, int64_t nvirt,
                                                      int64_t nocc, int64_t nvp, double* __restrict__ sigma) {
@@ -283,6 +305,7 @@ struct PackJob {
   double* sigma;  // row scales written by rowmax_group_kernel (p.sigma reads them)
   int64_t nvp;
   int64_t first;
+  int* status;    // CHECK rowmax: bit 4 set on a NaN / Inf in the job's rows (or null)
 };
 
 // Padded virtual energies of many fragments: out[a] = ev[a] (a < nvirt), 1.0
@@ -341,7 +364,9 @@ __global__ __launch_bounds__(256) void pack_tile_group_kernel(const PackJob* __r
 
 // Row scales of the exact-slice modes for many fragments: one wave per row,
 // each wave taking rows [w chunk, (w + 1) chunk) in order; jobs own rows
-// [first, next first) (row = i * nvp + a).
+// [first, next first) (row = i * nvp + a).  CHECK: a row holding a NaN or
+// Inf sets bit 4 of its job's status.
+template <bool CHECK>
 __global__ __launch_bounds__(256) void rowmax_group_kernel(const PackJob* __restrict__ jobs, int njobs, int64_t total,
                                                            int64_t chunk) {
   const int lane = threadIdx.x & 63;
@@ -353,7 +378,13 @@ __global__ __launch_bounds__(256) void rowmax_group_kernel(const PackJob* __rest
     const PackJob& J = jobs[w.at(row)];
     const int64_t r = row - J.first, i = r / J.nvp, a = r % J.nvp;
     double m = 0.0;
-    if (a < J.p.nvirt) m = col_absmax(J.p.B + J.p.naux * (a + J.p.nvirt * i), J.p.naux, lane);
+    if constexpr (CHECK) {
+      double z = 0.0;
+      if (a < J.p.nvirt) m = col_absmax_z(J.p.B + J.p.naux * (a + J.p.nvirt * i), J.p.naux, lane, z);
+      if (__any(z != z) && lane == 0 && J.status) atomicOr(J.status, 4);
+    } else {
+      if (a < J.p.nvirt) m = col_absmax(J.p.B + J.p.naux * (a + J.p.nvirt * i), J.p.naux, lane);
+    }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
     if (lane == 0) J.sigma[r] = m;
   }
@@ -445,6 +476,7 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
     j.sigma = it.sigma;
     j.nvp = it.g.nvp;
     j.first = ne;
+    j.status = nullptr;
     pj.push_back(j);
     ne += tiles_of(mode, it.g.rb, j.p);
     dj.push_back(PadJob{it.ev, it.ev_padded, it.nvirt, np});
@@ -471,7 +503,7 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
   const int nj = (int)items.size();
   if (mode_slices(mode)) {
     const int64_t rwaves = std::min<int64_t>(nr, kGroupBlocks * 4), rchunk = ceil_div(nr, rwaves);
-    hipLaunchKernelGGL(rowmax_group_kernel, dim3((unsigned)ceil_div(rwaves, 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(rowmax_group_kernel<false>, dim3((unsigned)ceil_div(rwaves, 4)), dim3(256), 0, s,
                        (const PackJob*)(d + orj), nj, nr, rchunk);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
@@ -492,6 +524,47 @@ void pack_B_group(const std::vector<PackItem>& items, int mode, hipStream_t s, D
   }
   RIMP2_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(pad_group_kernel, dim3(grid_blocks(np, 256)), dim3(256), 0, s, (const PadJob*)(d + odj), nj, np);
+  RIMP2_HIP_CHECK(hipGetLastError());
+}
+
+void pack_slices6_group(const std::vector<Slice6Item>& items, hipStream_t s, DevBuf& jobs,
+                        std::vector<unsigned char>& blob) {
+  blob.clear();
+  if (items.empty()) return;
+  std::vector<PackJob> pj, rj;
+  int64_t ne = 0, nr = 0;
+  for (const Slice6Item& it : items) {
+    if (it.nks * 32 < it.nq || it.nrp < it.nrows || it.nrp % 32 || it.nks % 4)
+      fail(RIMP2_ERR_ARG, "grouped six-slice packing geometry");
+    PackJob j;
+    j.p = PackArgs{it.src, it.nq, it.nrows, it.norb, it.nks, it.nrp / 32, it.sigma, it.dst};
+    j.sigma = it.sigma;
+    j.nvp = it.nrp;
+    j.status = it.status;
+    j.first = ne;
+    pj.push_back(j);
+    ne += pack_tiles<PACK_I8S6>(j.p);
+    j.first = nr;
+    rj.push_back(j);
+    nr += it.norb * it.nrp;
+  }
+  auto put = [&](const void* p, size_t bytes) {
+    const size_t off = (size_t)round_up((int64_t)blob.size(), 256);
+    blob.resize(off + bytes);
+    memcpy(blob.data() + off, p, bytes);
+    return off;
+  };
+  const size_t opj = put(pj.data(), pj.size() * sizeof(PackJob));
+  const size_t orj = put(rj.data(), rj.size() * sizeof(PackJob));
+  unsigned char* d = static_cast<unsigned char*>(jobs.ensure(blob.size()));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  const int nj = (int)items.size();
+  const int64_t rwaves = std::min<int64_t>(nr, kGroupBlocks * 4), rchunk = ceil_div(nr, rwaves);
+  hipLaunchKernelGGL(rowmax_group_kernel<true>, dim3((unsigned)ceil_div(rwaves, 4)), dim3(256), 0, s,
+                     (const PackJob*)(d + orj), nj, nr, rchunk);
+  RIMP2_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(pack_tile_group_kernel<PACK_I8S6>, dim3(grid_blocks(ne, 1)), dim3(256), 0, s,
+                     (const PackJob*)(d + opj), nj, ne);
   RIMP2_HIP_CHECK(hipGetLastError());
 }
 

@@ -205,12 +205,29 @@ void metric_transform_i8_range(Ctx* c, int64_t n, int64_t nvirt, int64_t i0, int
 struct MetricGroup {
   DevBuf L, Linv, T, Lp, Xp, jobs;
   std::vector<unsigned char> blob;
+  // the int8 transform of the exact-slice modes: slice arenas, scales, job tables
+  DevBuf Ls, Xs, sig, ijobs, sjobs;
+  std::vector<unsigned char> iblob, sblob;
   void release();
 };
 // Queues on s: the copies, the denominator checks, then the metric pipeline
 // of every item.  The caller synchronizes s before the next call on G.
+// i8: the transform runs on int8 MFMA with six slices per operand
+// (transform_i8_group) instead of FP64.
 void metric_group_run(MetricGroup& G, hipStream_t s, const std::vector<mg::Item>& items,
-                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms);
+                      const std::vector<mg::Copy>& copies, const std::vector<mg::Denom>& denoms, bool i8 = false);
+// One fragment of the grouped int8 transform: L^{-1} (device, n x n), scratch
+// T (n x n, for L^{-T}), X and B [n, nvirt, nocc]; a NaN or Inf in X sets bit
+// 4 of status.
+struct I8GroupItem {
+  const double* Li;
+  double* T;
+  const double* X;
+  double* B;
+  int64_t n, nvirt, nocc;
+  int* status;
+};
+void transform_i8_group(MetricGroup& G, hipStream_t s, const std::vector<I8GroupItem>& items);
 // C = alpha * A * B^T + beta * C (column-major FP64 MFMA GEMM)
 void gemm_nt_f64(hipStream_t s, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
@@ -229,6 +246,18 @@ int mode_slices(int mode);
 // max_blocks (0: no cap) caps the slicing grid.
 void pack_slices6(hipStream_t s, const double* src, int64_t nq, int64_t nrows, int64_t norb, int64_t nks,
                   int64_t nrp, double* sigma, void* dst, unsigned max_blocks = 0);
+// The same for many arrays in two launches (row scales, then slices); a
+// column holding a NaN or Inf sets bit 4 of its item's status (if not null).
+// jobs / blob hold the job tables until the caller synchronizes s.
+struct Slice6Item {
+  const double* src;
+  int64_t nq, nrows, norb, nks, nrp;
+  double* sigma;
+  void* dst;
+  int* status;
+};
+void pack_slices6_group(const std::vector<Slice6Item>& items, hipStream_t s, DevBuf& jobs,
+                        std::vector<unsigned char>& blob);
 // One fragment of a grouped packing (device arrays).
 struct PackItem {
   const double* B;      // FP64 B [naux, nvirt, nocc]

@@ -16,9 +16,8 @@
 // are coalesced.  Wave (wr, wc) holds the 32 x 32 block (a rows wr, Q cols
 // wc).  A 32-deep k-step (one stage) brings 12 KB of X and 12 KB of L^{-T}
 // into LDS through buffer_load ... lds, six 1-KB transfers per wave, in a
-// four-stage ring; the K loop of a Q tile stops at its last row (L^{-1} is
-// lower triangular), rounded up to whole four-stage groups (the operands are
-// zero there).
+// three-stage ring; the K loop of a Q tile stops at its last row (L^{-1} is
+// lower triangular; at least three stages, where the operands are zero).
 #include <algorithm>
 
 #include "pair_core.h"
@@ -51,21 +50,20 @@ struct TransformI8Args {
   int64_t MT, NT;           // Q tiles (64), virtual tiles (64) per orbital
 };
 
-__global__ __launch_bounds__(TI_THREADS) void transform_i8_kernel(TransformI8Args p) {
-  __shared__ __align__(16) unsigned char buf0[TI_SB];
-  __shared__ __align__(16) unsigned char buf1[TI_SB];
-  __shared__ __align__(16) unsigned char buf2[TI_SB];
-  __shared__ __align__(16) unsigned char buf3[TI_SB];
+// Tile u of p (all 256 threads; the arguments and u wave-uniform).
+__device__ __attribute__((always_inline)) inline void transform_i8_tile(const TransformI8Args& p, int64_t u,
+                                                                        unsigned char* __restrict__ buf0,
+                                                                        unsigned char* __restrict__ buf1,
+                                                                        unsigned char* __restrict__ buf2) {
   // tiles by decreasing Q tile (longest K loop first), then orbital and virtual tile
   const int64_t ncol = p.norb * p.NT;
-  const int64_t u = blockIdx.x;
   const int64_t mt = p.MT - 1 - u / ncol, col = u % ncol;
   const int64_t io = col / p.NT, nt = col % p.NT;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
-  // K stages: up to the tile's last Q row, whole groups of four
-  int64_t nst = ((2 * (mt + 1) + 3) / 4) * 4;
+  // K stages: up to the tile's last Q row (at least the three the prologue issues)
+  int64_t nst = 2 * (mt + 1) > 3 ? 2 * (mt + 1) : 3;
   if (nst > p.nks) nst = p.nks;
   const size_t x_ks = (size_t)(p.nvp / 32) * TI_SL * TI_FB, a_ks = (size_t)(p.nqp / 32) * TI_SL * TI_FB;
   const size_t x_orb = (size_t)p.nks * x_ks;
@@ -98,10 +96,13 @@ __global__ __launch_bounds__(TI_THREADS) void transform_i8_kernel(TransformI8Arg
 #pragma unroll
   for (int d = 0; d < Pol::ND; d++) Pol::zero(acc[d]);
   static_assert(Pol::MMA_PER_CALL == 21, "six slices, diagonals up to 5");
+  // step st: stage st + 1 has landed (stage st + 2 may be in flight), every
+  // wave has read stage st into registers; the freed buffer takes stage st + 3
+  // (re-fetches of the last stage past the end keep the counts uniform)
   auto step = [&](int64_t st, Ops& cur, Ops& nxt, unsigned char* freed, const unsigned char* next) {
-    RIMP2_WAIT_VMCNT_LGKM0(2 * TI_NI);
+    RIMP2_WAIT_VMCNT_LGKM0(TI_NI);
     pipeline_barrier();
-    issue(st + 4 < nst ? st + 4 : nst - 1, freed);
+    issue(st + 3 < nst ? st + 3 : nst - 1, freed);
     load(next, nxt);
     Pol::mma(acc, cur.r, cur.c);
     // 21 MFMAs, 12 LDS reads and 6 DMAs interleaved
@@ -118,16 +119,26 @@ __global__ __launch_bounds__(TI_THREADS) void transform_i8_kernel(TransformI8Arg
   issue(0, buf0);
   issue(1, buf1);
   issue(2, buf2);
-  issue(3, buf3);
-  RIMP2_WAIT_VMCNT(3 * TI_NI);
+  RIMP2_WAIT_VMCNT(2 * TI_NI);
   pipeline_barrier();
   load(buf0, ra_);
-  for (int64_t st = 0; st < nst; st += 4) {  // nst is a multiple of 4
+  // three buffers and two register sets: the pattern repeats every six steps;
+  // the last 0-5 steps run straight-line
+  int64_t st = 0;
+  for (; st + 6 <= nst; st += 6) {
     step(st, ra_, rb_, buf0, buf1);
     step(st + 1, rb_, ra_, buf1, buf2);
-    step(st + 2, ra_, rb_, buf2, buf3);
-    step(st + 3, rb_, ra_, buf3, buf0);
+    step(st + 2, ra_, rb_, buf2, buf0);
+    step(st + 3, rb_, ra_, buf0, buf1);
+    step(st + 4, ra_, rb_, buf1, buf2);
+    step(st + 5, rb_, ra_, buf2, buf0);
   }
+  const int64_t rem = nst - st;
+  if (rem > 0) step(st, ra_, rb_, buf0, buf1);
+  if (rem > 1) step(st + 1, rb_, ra_, buf1, buf2);
+  if (rem > 2) step(st + 2, ra_, rb_, buf2, buf0);
+  if (rem > 3) step(st + 3, rb_, ra_, buf0, buf1);
+  if (rem > 4) step(st + 4, ra_, rb_, buf1, buf2);
   RIMP2_WAIT_VMCNT_LGKM0(0);  // trailing re-fetches are done before the workgroup ends
   // epilogue: element (a row, Q col) of the wave's block
   const int64_t i = p.i0 + io;
@@ -141,12 +152,69 @@ __global__ __launch_bounds__(TI_THREADS) void transform_i8_kernel(TransformI8Arg
   }
 }
 
-// L^{-T}[P + n Q] = L^{-1}[Q + n P] by 64 x 64 tiles through LDS.
-__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ Li, int64_t n,
-                                                        double* __restrict__ Lt) {
-  __shared__ double t[64][65];
+// Three 24-KB stage buffers: two workgroups per CU (144 of its 160 KB), so one
+// workgroup's pipeline fill and epilogue run under the other's MFMAs.
+#define RIMP2_TI_LDS                                   \
+  __shared__ __align__(16) unsigned char buf0[TI_SB]; \
+  __shared__ __align__(16) unsigned char buf1[TI_SB]; \
+  __shared__ __align__(16) unsigned char buf2[TI_SB]
+
+__global__ __launch_bounds__(TI_THREADS, 2) void transform_i8_kernel(TransformI8Args p) {
+  RIMP2_TI_LDS;
+  transform_i8_tile(p, blockIdx.x, buf0, buf1, buf2);
+}
+
+// Grouped form: job j owns the tiles [first, next job's first).
+struct TransformI8Job {
+  TransformI8Args a;
+  int64_t first;
+};
+
+template <class Job>
+__device__ inline int find_job(const Job* __restrict__ jobs, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Tiles of all jobs, grid-stride beyond the grid; a barrier between tiles keeps
+// every wave's last reads of the ring ahead of the next tile's DMA.
+__global__ __launch_bounds__(TI_THREADS, 2) void transform_i8_group_kernel(const TransformI8Job* __restrict__ jobs,
+                                                                        int njobs, int64_t total) {
+  RIMP2_TI_LDS;
+  using pairk::uni;
+  for (int64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const int j = pairk::uni_int(find_job(jobs, njobs, u));
+    const TransformI8Args& J = jobs[j].a;
+    TransformI8Args a;  // wave-uniform copies: the buffer resources stay in scalar registers
+    a.Xp = uni(J.Xp);
+    a.Ap = uni(J.Ap);
+    a.sX = uni(J.sX);
+    a.sA = uni(J.sA);
+    a.B = uni(J.B);
+    a.n = uni(J.n);
+    a.nvirt = uni(J.nvirt);
+    a.nvp = uni(J.nvp);
+    a.nqp = uni(J.nqp);
+    a.nks = uni(J.nks);
+    a.i0 = 0;
+    a.norb = uni(J.norb);
+    a.MT = uni(J.MT);
+    a.NT = uni(J.NT);
+    transform_i8_tile(a, u - uni(jobs[j].first), buf0, buf1, buf2);
+    pipeline_barrier();
+  }
+}
+
+// L^{-T}[P + n Q] = L^{-1}[Q + n P] by 64 x 64 tiles through LDS (tile b).
+__device__ inline void transpose_tile(const double* __restrict__ Li, int64_t n, double* __restrict__ Lt, int64_t b,
+                                      double (&t)[64][65]) {
   const int64_t tiles = (n + 63) / 64;
-  const int64_t bq = blockIdx.x % tiles, bp = blockIdx.x / tiles;
+  const int64_t bq = b % tiles, bp = b / tiles;
   const int tid = threadIdx.x;
 #pragma unroll 4
   for (int r = 0; r < 16; r++) {
@@ -160,6 +228,28 @@ __global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict
     const int c = (tid >> 6) + 4 * r;  // column Q of L^{-T}
     const int64_t P = bp * 64 + (tid & 63), Q = bq * 64 + c;
     if (P < n && Q < n) Lt[P + n * Q] = t[tid & 63][c];
+  }
+}
+
+__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ Li, int64_t n,
+                                                        double* __restrict__ Lt) {
+  __shared__ double t[64][65];
+  transpose_tile(Li, n, Lt, blockIdx.x, t);
+}
+
+struct TransposeJob {
+  const double* Li;
+  double* Lt;
+  int64_t n, first;
+};
+
+__global__ __launch_bounds__(256) void transpose_group_kernel(const TransposeJob* __restrict__ jobs, int njobs,
+                                                              int64_t total) {
+  __shared__ double t[64][65];
+  for (int64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const TransposeJob& J = jobs[find_job(jobs, njobs, u)];
+    transpose_tile(J.Li, J.n, J.Lt, u - J.first, t);
+    __syncthreads();  // t is reused by the next tile
   }
 }
 
@@ -240,6 +330,82 @@ void metric_transform_i8_range(Ctx* c, int64_t n, int64_t nvirt, int64_t i0, int
     hipLaunchKernelGGL(transform_i8_kernel, dim3((unsigned)(per * a.norb)), dim3(TI_THREADS), 0, c->stream, a);
     RIMP2_HIP_CHECK(hipGetLastError());
   }
+}
+
+void transform_i8_group(MetricGroup& G, hipStream_t s, const std::vector<I8GroupItem>& items) {
+  if (items.empty()) return;
+  // arena offsets (256-byte aligned): L^-T slices, X slices, and the scales
+  std::vector<size_t> lo(items.size()), xo(items.size()), so(items.size());
+  size_t ln = 0, xn = 0, sn = 0;
+  auto take = [](size_t& at, size_t bytes) {
+    const size_t o = at;
+    at += (size_t)round_up((int64_t)bytes, (int64_t)256);
+    return o;
+  };
+  for (size_t k = 0; k < items.size(); k++) {
+    const I8GroupItem& it = items[k];
+    const I8Geom g = i8_geom(it.n, it.nvirt);
+    if ((size_t)g.nks * (size_t)(g.nqp / 32) * TI_SL * TI_FB > 0xFFFFFFFFull || x_slices_bytes_per_orb(g) > 0xFFFFFFFFull)
+      fail(RIMP2_ERR_ARG, "int8 transform: slices exceed 4 GiB per slab (naux=%lld nvirt=%lld)", (long long)it.n,
+           (long long)it.nvirt);
+    lo[k] = take(ln, metric_i8_linv_bytes(it.n));
+    xo[k] = take(xn, metric_i8_x_bytes(it.n, it.nvirt, it.nocc));
+    so[k] = take(sn, (size_t)(g.nqp + g.nvp * it.nocc) * sizeof(double));
+  }
+  unsigned char* Ls = static_cast<unsigned char*>(G.Ls.ensure(ln));
+  unsigned char* Xs = static_cast<unsigned char*>(G.Xs.ensure(xn));
+  unsigned char* sg = static_cast<unsigned char*>(G.sig.ensure(sn));
+  std::vector<TransposeJob> tj;
+  std::vector<Slice6Item> sl;
+  std::vector<TransformI8Job> tr;
+  int64_t nt = 0, nu = 0;
+  for (size_t k = 0; k < items.size(); k++) {
+    const I8GroupItem& it = items[k];
+    const I8Geom g = i8_geom(it.n, it.nvirt);
+    double* sA = reinterpret_cast<double*>(sg + so[k]);
+    double* sX = sA + g.nqp;
+    tj.push_back(TransposeJob{it.Li, it.T, it.n, nt});
+    nt += ceil_div(it.n, (int64_t)64) * ceil_div(it.n, (int64_t)64);
+    sl.push_back(Slice6Item{it.T, it.n, it.n, 1, g.nks, g.nqp, sA, Ls + lo[k], nullptr});
+    sl.push_back(Slice6Item{it.X, it.n, it.nvirt, it.nocc, g.nks, g.nvp, sX, Xs + xo[k], it.status});
+    TransformI8Job j;
+    j.a.Xp = Xs + xo[k];
+    j.a.Ap = Ls + lo[k];
+    j.a.sX = sX;
+    j.a.sA = sA;
+    j.a.B = it.B;
+    j.a.n = it.n;
+    j.a.nvirt = it.nvirt;
+    j.a.nvp = g.nvp;
+    j.a.nqp = g.nqp;
+    j.a.nks = g.nks;
+    j.a.i0 = 0;
+    j.a.norb = it.nocc;
+    j.a.MT = g.nqp / 64;
+    j.a.NT = g.nvp / 64;
+    j.first = nu;
+    tr.push_back(j);
+    nu += j.a.MT * j.a.NT * it.nocc;
+  }
+  G.iblob.clear();
+  auto put = [&](const void* p, size_t bytes) {
+    const size_t off = (size_t)round_up((int64_t)G.iblob.size(), (int64_t)256);
+    G.iblob.resize(off + bytes);
+    memcpy(G.iblob.data() + off, p, bytes);
+    return off;
+  };
+  const size_t otj = put(tj.data(), tj.size() * sizeof(TransposeJob));
+  const size_t otr = put(tr.data(), tr.size() * sizeof(TransformI8Job));
+  unsigned char* d = static_cast<unsigned char*>(G.ijobs.ensure(G.iblob.size()));
+  RIMP2_HIP_CHECK(hipMemcpyAsync(d, G.iblob.data(), G.iblob.size(), hipMemcpyHostToDevice, s));
+  const int nj = (int)items.size();
+  hipLaunchKernelGGL(transpose_group_kernel, dim3(grid_blocks(nt, 1)), dim3(256), 0, s,
+                     (const TransposeJob*)(d + otj), nj, nt);
+  RIMP2_HIP_CHECK(hipGetLastError());
+  pack_slices6_group(sl, s, G.sjobs, G.sblob);
+  hipLaunchKernelGGL(transform_i8_group_kernel, dim3(grid_blocks(nu, 1)), dim3(TI_THREADS), 0, s,
+                     (const TransformI8Job*)(d + otr), nj, nu);
+  RIMP2_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace rimp2

@@ -164,10 +164,12 @@ def test_batch_prepare_device_matches_host(ctx, rimp2, mode):
     np.testing.assert_array_equal(pp_d, pp_h)
 
 
-def test_batch_prepare_device_errors(ctx, rimp2):
+@pytest.mark.parametrize("mode", ["fp64", "i8s5"])
+def test_batch_prepare_device_errors(ctx, rimp2, mode):
     """The device-side input checks name the failing fragment: a metric that
     is not positive definite, a non-finite coefficient, a denominator >= 0;
-    the context stays usable."""
+    the context stays usable.  In the exact-slice modes the (P|ia) check is
+    part of the int8 transform's slicing."""
     good = rimp2.gen_fragment(3, 40, 10, 3)
     P, V, eo, ev = good
     Vbad = V.copy(order="F")
@@ -179,17 +181,17 @@ def test_batch_prepare_device_errors(ctx, rimp2):
     cases = [((P, Vbad, eo, ev), rimp2.ERR_NOT_SPD), ((Pbad, V, eo, ev), rimp2.ERR_NUMERIC),
              ((P, V, eo, ev - 10.0), rimp2.ERR_NUMERIC), ((P, V, eon, ev), rimp2.ERR_NUMERIC)]
     for bad, code in cases:
-        say("batch prepare device error", code)
+        say("batch prepare device error", mode, code)
         descs, ptrs = _to_device_descs(ctx, [good, bad])
         try:
             with pytest.raises(rimp2.Rimp2Error) as ei:
-                ctx.batch_prepare_device(descs, "fp64")
+                ctx.batch_prepare_device(descs, mode)
         finally:
             for p in ptrs:
                 ctx.free(p)
         assert ei.value.code == code
         assert "fragment 1" in str(ei.value)
-    ef, _ = ctx.batch([good], "fp64")
+    ef, _ = ctx.batch([good], mode)
     assert ef[0] < 0
 
 
