@@ -254,8 +254,10 @@ int main(int argc, char** argv) {
     reg(x.ev.data(), x.ev.size() * 8, "ev");
     reg(x.status.get(), 8, "status");
   }
-  auto run_case = [&](int bad_frag, int bad_kind) -> int {
+  auto run_case = [&](int bad_frag, int bad_kind, bool transform = true) -> int {
     // bad_kind: 0 none, 1 non-positive pivot, 2 NaN in X, 3 denominator >= 0
+    // transform = false: the plan of the int8 transform's callers, which ends
+    // with L^{-1}; X is neither read nor checked by it
     const size_t nreg = g_regions.size();  // this case's buffers are unregistered at its end
     std::vector<Item> items;
     std::vector<Copy> copies;
@@ -288,7 +290,10 @@ int main(int argc, char** argv) {
     Arenas ar{L.data(), Li.data(), T.data(), Lp.data(), Xp.data()};
     std::vector<GroupStep> steps;
     std::vector<unsigned char> blob;
-    plan(items, copies, denoms, g, ar, steps, blob);
+    plan(items, copies, denoms, g, ar, steps, blob, transform);
+    for (const GroupStep& st : steps)
+      if (!transform && (st.kind == GL_LPACK || st.kind == GL_XPACK || st.kind == GL_TRANSFORM))
+        die("transform step planned with transform = false", (int)st.kind);
     Hazards hz;
     for (const GroupStep& st : steps) {
       const unsigned char* jp = blob.data() + st.off;
@@ -451,7 +456,8 @@ int main(int argc, char** argv) {
     // compare with the direct computation
     for (size_t f = 0; f < F.size(); f++) {
       Frag& x = F[f];
-      const int want = ((int)f == bad_frag) ? (bad_kind == 1 ? 2 : bad_kind == 2 ? 4 : bad_kind == 3 ? 8 : 0) : 0;
+      int want = ((int)f == bad_frag) ? (bad_kind == 1 ? 2 : bad_kind == 2 ? 4 : bad_kind == 3 ? 8 : 0) : 0;
+      if (!transform && bad_kind == 2 && !x.fitted) want = 0;  // X is the int8 slicing's to check
       if ((x.status[0] & want) != want || (x.status[0] & ~want)) die("status word", (long long)f, x.status[0]);
       if (want) continue;
       const int64_t n = x.n, nc = x.nvirt * x.nocc;
@@ -491,7 +497,7 @@ int main(int argc, char** argv) {
           lierr = std::fmax(lierr, std::fabs(Lis[r + n * c] - li[r + n * c]));
           limax = std::fmax(limax, std::fabs(li[r + n * c]));
         }
-      for (int64_t c = 0; c < nc; c++)
+      for (int64_t c = 0; c < (transform ? nc : 0); c++)
         for (int64_t q = 0; q < n; q++) {
           double s = 0.0;
           for (int64_t p = 0; p <= q; p++) s += li[q + n * p] * x.X[p + n * c];
@@ -512,6 +518,11 @@ int main(int argc, char** argv) {
   run_case(3, 2);   // NaN in X
   run_case(8, 3);   // denominator >= 0
   run_case(nd, 2);  // NaN in the fitted copy
+  // the plan without the transform (exact-slice modes): L and L^{-1} as above,
+  // no packing or transform step, no access to the packed-operand arenas
+  run_case(-1, 0, false);
+  run_case(6, 1, false);
+  run_case(3, 2, false);
   std::printf("OK %d\n", launches);
   return 0;
 }

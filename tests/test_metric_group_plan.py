@@ -5,7 +5,9 @@ what another tile of it writes, and L, L^{-1} and B = L^{-1} X of fragments
 from 1 to 320 auxiliary functions (below, at and above the 64-wide panel,
 full and ragged doubling groups) must match a direct Cholesky, forward
 substitution and product to 1e-12; the status words of a bad pivot, a NaN
-input and a denominator >= 0 must flag exactly their fragment."""
+input and a denominator >= 0 must flag exactly their fragment.  The plan
+without the transform (the exact-slice modes, which transform on int8 MFMA
+after it) is run too: L and L^{-1} as above, and no packing or transform step."""
 import os
 import shutil
 import subprocess
