@@ -53,8 +53,10 @@ struct TransformI8Args {
 // Tile u of p (all 256 threads; the arguments and u wave-uniform).  COLS: tiles
 // column by column (orbital, virtual tile), the Q tiles of a column together,
 // longest K loop first: they read the same X slab, which the launch keeps on
-// one XCD (xcd_unit); at 64/512/2048 this cut the HBM fetch of X re-reads and
-// the transform from 1.40 to 1.25 ms.  Else Q tile by Q tile (longest first),
+// one XCD (xcd_unit), so what each column re-reads is L^{-T}, small enough for
+// the Infinity Cache, where Q tile by Q tile every X slab came from HBM once
+// per Q tile (64/512/2048: MFMA busy 0.56 -> 0.71, 1.39 -> 1.22 ms).  Else Q
+// tile by Q tile (longest first),
 // the columns inside: the grouped launch of many small fragments, whose X
 // slabs stay in the caches, measured 0.4 ms faster per 256-fragment prepare
 // in that order.
