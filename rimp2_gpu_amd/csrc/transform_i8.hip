@@ -4,7 +4,9 @@
 // scaled by its own largest |value|: X by (auxiliary) column (i, a) over P,
 // L^{-T} by column Q over P (a row of L^{-1}).  Every slice product with
 // s + t <= 5 (21 of the 36) accumulates exactly in int32, one accumulator per
-// diagonal d = s + t; the dropped products sit below 254^{-6} of the scale.
+// diagonal d = s + t (at most 6 products a diagonal: |acc| <= 6 x 127^2 x naux
+// < 2^31 for naux <= 16384, the bound every exact-slice entry point checks);
+// the dropped products sit below 254^{-6} of the scale.
 // The FP64 epilogue forms sum_d acc_d 254^{-d} / 127^2 x sigma_Q x sigma_ia.
 // With five slices the transform's own error in E would be ~10x the pair
 // kernel's; with six it is ~1/30 of it (numpy model at 2048/512/8 and
