@@ -291,6 +291,35 @@ def test_device_path_flags_bad_input(ctx, rimp2):
     ctx.sync()  # status was cleared
 
 
+def test_int8_transform_at_the_largest_exact_slice_naux(ctx, rimp2):
+    """naux = 16384, the exact-slice modes' limit (the int8 transform's int32
+    diagonals hold at most 6 x 127^2 x naux < 2^31): its K loops run up to 512
+    stages.  B against the FP64 transform of the same device-generated inputs,
+    and E within 1 uEh of fp64; one more auxiliary function is refused."""
+    naux, nvirt, nocc = 16384, 32, 2
+    say("int8 transform at naux", naux)
+    b = [ctx.alloc(naux * nvirt * nocc * 8), ctx.alloc(naux * naux * 8), ctx.alloc(nocc * 8), ctx.alloc(nvirt * 8)]
+    dpe, det = ctx.alloc(nocc * nocc * 8), ctx.alloc(8)
+    E, B = {}, {}
+    try:
+        ctx.gen_device(41, naux, nvirt, nocc, *b)
+        for mode in ("fp64", "i8s5"):
+            ctx.energy_device(naux, nvirt, nocc, *b, mode, dpe, det)
+            ctx.sync()
+            E[mode] = float(ctx.from_device(det, (1,))[0])
+            B[mode] = ctx.get_B(naux, nvirt, nocc)
+        with pytest.raises(rimp2.Rimp2Error) as ei:
+            ctx.energy_device(naux + 1, nvirt, nocc, *b, "i8s5", dpe, det)
+        assert ei.value.code == rimp2.ERR_ARG
+    finally:
+        for p in b + [dpe, det]:
+            ctx.free(p)
+    err = np.max(np.abs(B["i8s5"] - B["fp64"])) / np.abs(B["fp64"]).max()
+    say("B relative error", err, "dE uEh", (E["i8s5"] - E["fp64"]) * 1e6)
+    assert err <= TRANSFORM_TOL["i8s5"]
+    assert abs(E["i8s5"] - E["fp64"]) < 1e-6
+
+
 @pytest.mark.parametrize("shape", [(33, 5, 3), (256, 129, 33)])
 def test_non_finite_pia_is_found_at_any_position(ctx, rimp2, shape):
     """The device check of (P|ia) reads 16-byte pairs on a capped grid: a NaN
